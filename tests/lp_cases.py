@@ -1,0 +1,170 @@
+"""Known-answer LPs of the reference test suite and seeded workloads, as LP objects.
+
+The small LPs restate tests/test_pdhg.cpp:54-436 of the reference (same data, same
+published optima); the MPS fixtures are the reference's own model files copied to
+tests/golden/ (data, Netlib/MRPL models) with optima from
+SOVEREIGN_SOLVER_BLUEPRINT/test_models/reference.json and apps/benchmark_runner.cpp:707-709.
+"""
+import os
+
+import numpy as np
+
+import vajra_pdhg as vp
+
+INF = 1e20
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+CFG0 = dict(seed=0x5EED2026, m=2000, n=5000, k=6)
+
+
+def cfg0_lp():
+    return vp.gen_random_lp(CFG0["seed"], CFG0["m"], CFG0["n"], CFG0["k"])
+
+
+def kat_lps():
+    """name -> (LP, expected objective or None, expected x or None)."""
+    L = vp.LP.from_dense
+    cases = {}
+    # test_pdhg.cpp:54-99  min -x1 - 2 x2, x1+x2<=4, x1<=3, x2<=3 -> -7 at (1,3)
+    cases["two_var"] = (L([[1, 1], [1, 0], [0, 1]], [-1, -2], [0, 0], [INF, INF], [-INF] * 3, [4, 3, 3]), -7.0,
+                        [1.0, 3.0])
+    # test_pdhg.cpp:102-142 equality LP -> 13
+    cases["equality"] = (L([[1, 1, 1], [2, 1, 0]], [1, 2, 3], [0, 0, 0], [INF] * 3, [6, 5], [6, 5]), 13.0, None)
+    # test_pdhg.cpp:145-188 >=, <=, ranged rows -> 4
+    cases["ranged"] = (L([[1, 1], [1, -1], [2, 1]], [2, 3], [0, 0], [10, 10], [2, -INF, 1], [INF, 4, 6]), 4.0, None)
+    # test_pdhg.cpp:191-231 free variable -> -5 at (-10, 15)
+    cases["free_var"] = (L([[1, 1]], [2, 1], [-10, -INF], [10, INF], [5], [5]), -5.0, [-10.0, 15.0])
+    # test_pdhg.cpp:234-272 min/max symmetry -> -15 / 15
+    cases["min_sym"] = (L([[1, 1]], [-2, -3], [0, 0], [INF, INF], [-INF], [5]), -15.0, None)
+    cases["max_sym"] = (L([[1, 1]], [2, 3], [0, 0], [INF, INF], [-INF], [5], maximize=True), 15.0, None)
+    # test_pdhg.cpp:405-436 GPU-request LP -> 3
+    cases["gpu_fallback"] = (L([[1, 1]], [1, 1], [0, 0], [10, 10], [3], [10]), 3.0, None)
+    # test_pdhg.cpp:275-314 presolve LP -> 6 with x3 = 5 (solved here without presolve)
+    cases["presolve_lp"] = (L([[1, 1, 0], [0, 1, 0]], [-1, -1, 2], [0, 0, 5], [INF, INF, 5], [-INF, -INF], [4, 10]),
+                            6.0, None)
+    return cases
+
+
+def infeasible_lp():
+    # test_pdhg.cpp:317-348
+    return vp.LP.from_dense([[1, 1], [1, 1]], [1, 1], [0, 0], [10, 10], [-INF, 3], [1, INF])
+
+
+# --- MPS reader (fixed and free format; ROWS, COLUMNS, RHS, RANGES, BOUNDS) for fixtures ---
+def read_mps(path):
+    rows = {}          # name -> (index, type)
+    row_order = []
+    obj_name = None
+    cols = {}          # name -> index
+    entries = {}       # (row_idx, col_idx) -> value
+    cvec = {}
+    rhs = {}
+    ranges = {}
+    bounds = {}
+    section = None
+    offset = 0.0
+    maximize = False
+    with open(path) as f:
+        for raw in f:
+            line = raw.rstrip("\n")
+            if not line.strip() or line.startswith("*"):
+                continue
+            if not line[0].isspace():
+                tok = line.split()
+                section = tok[0].upper()
+                if section == "OBJSENSE" and len(tok) > 1:
+                    maximize = tok[1].upper().startswith("MAX")
+                continue
+            tok = line.split()
+            if section == "OBJSENSE":
+                maximize = tok[0].upper().startswith("MAX")
+            elif section == "ROWS":
+                t, name = tok[0].upper(), tok[1]
+                if t == "N":
+                    if obj_name is None:
+                        obj_name = name
+                    continue
+                rows[name] = (len(row_order), t)
+                row_order.append(name)
+            elif section == "COLUMNS":
+                if "MARKER" in line:
+                    continue
+                cname = tok[0]
+                if cname not in cols:
+                    cols[cname] = len(cols)
+                j = cols[cname]
+                for q in range(1, len(tok) - 1, 2):
+                    rname, val = tok[q], float(tok[q + 1])
+                    if rname == obj_name:
+                        cvec[j] = cvec.get(j, 0.0) + val
+                    elif rname in rows:
+                        key = (rows[rname][0], j)
+                        entries[key] = entries.get(key, 0.0) + val
+            elif section == "RHS":
+                toks = tok[1:] if len(tok) % 2 == 1 else tok
+                for q in range(0, len(toks) - 1, 2):
+                    rname, val = toks[q], float(toks[q + 1])
+                    if rname == obj_name:
+                        offset = -val
+                    else:
+                        rhs[rname] = val
+            elif section == "RANGES":
+                toks = tok[1:] if len(tok) % 2 == 1 else tok
+                for q in range(0, len(toks) - 1, 2):
+                    ranges[toks[q]] = float(toks[q + 1])
+            elif section == "BOUNDS":
+                bt = tok[0].upper()
+                cname = tok[2] if len(tok) >= 3 else tok[1]
+                val = float(tok[3]) if len(tok) >= 4 else 0.0
+                bounds.setdefault(cname, []).append((bt, val))
+    m, n = len(row_order), len(cols)
+    lr, ur = np.zeros(m), np.zeros(m)
+    for name, (i, t) in rows.items():
+        b = rhs.get(name, 0.0)
+        if t == "E":
+            lo, hi = b, b
+            if name in ranges:
+                r = ranges[name]
+                lo, hi = (b, b + abs(r)) if r >= 0 else (b - abs(r), b)
+        elif t == "L":
+            lo, hi = -INF, b
+            if name in ranges:
+                lo = b - abs(ranges[name])
+        else:
+            lo, hi = b, INF
+            if name in ranges:
+                hi = b + abs(ranges[name])
+        lr[i], ur[i] = lo, hi
+    lc, uc = np.zeros(n), np.full(n, INF)
+    for cname, bl in bounds.items():
+        j = cols[cname]
+        for bt, val in bl:
+            if bt == "UP":
+                uc[j] = val
+                if val < 0 and lc[j] == 0.0:
+                    lc[j] = -INF
+            elif bt == "LO":
+                lc[j] = val
+            elif bt == "FX":
+                lc[j] = uc[j] = val
+            elif bt == "FR":
+                lc[j], uc[j] = -INF, INF
+            elif bt == "MI":
+                lc[j] = -INF
+            elif bt == "PL":
+                uc[j] = INF
+    A = np.zeros((m, n))
+    for (i, j), v in entries.items():
+        A[i, j] = v
+    c = np.zeros(n)
+    for j, v in cvec.items():
+        c[j] = v
+    return vp.LP.from_dense(A, c, lc, uc, lr, ur, maximize=maximize, offset=offset,
+                            name=os.path.splitext(os.path.basename(path))[0])
+
+
+MPS_OPTIMA = {"afiro": -464.75314286, "crude_blend": 214.145945946}
+
+
+def mps_lp(name):
+    return read_mps(os.path.join(GOLDEN, f"{name}.mps"))

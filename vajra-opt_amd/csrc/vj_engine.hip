@@ -1,0 +1,869 @@
+// Restarted reflected-Halpern PDHG engine for one LP or a block-diagonal batch of LPs.
+//
+// Per iteration: primal_step (A^T y SpMV + projection + extrapolation + Halpern step)
+// and dual_step (A x_bar SpMV + row prox + Halpern step). Every K iterations (one
+// "block") the FP64 check runs: two KKT passes on the FP64 scaled matrix, the per-instance
+// rel-KKT reduction, the control kernel (restart / primal weight / convergence, all on
+// the device) and the conditional restart kernels. A block has fixed kernel arguments,
+// so it is captured once as a hipGraph and replayed; the host only reads the per-instance
+// state (a few hundred bytes) once per block, one block behind, into pinned memory.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <initializer_list>
+#include <memory>
+#include <vector>
+
+#include "vj_context.hpp"
+#include "vj_kernels.hpp"
+
+namespace vj {
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+#define VJ_DISPATCH_G(G_, ...)                         \
+    switch (G_) {                                      \
+        case 2: { constexpr int GG = 2; __VA_ARGS__; } break;   \
+        case 4: { constexpr int GG = 4; __VA_ARGS__; } break;   \
+        case 8: { constexpr int GG = 8; __VA_ARGS__; } break;   \
+        case 16: { constexpr int GG = 16; __VA_ARGS__; } break; \
+        case 32: { constexpr int GG = 32; __VA_ARGS__; } break; \
+        default: { set_error("unsupported lanes per row " + std::to_string(G_)); return VJ_ERR_INVALID_ARGUMENT; } \
+    }
+
+inline int tiles_for(int rows, int G) { return grid_for(rows, kBlock / G); }
+
+uint16_t to_bf16_bits(double v) {
+    float f = static_cast<float>(v);
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return static_cast<uint16_t>(u >> 16);  // inf/nan
+    const uint32_t lsb = (u >> 16) & 1u;
+    u += 0x7fffu + lsb;  // round to nearest even
+    return static_cast<uint16_t>(u >> 16);
+}
+
+__global__ void convert_f32(int64_t len, const double* __restrict__ a, float* __restrict__ b) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < len) b[i] = static_cast<float>(a[i]);
+}
+__global__ void convert_bf16(int64_t len, const double* __restrict__ a, bf16_t* __restrict__ b) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= len) return;
+    float f = static_cast<float>(a[i]);
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) != 0x7f800000u) u += 0x7fffu + ((u >> 16) & 1u);
+    b[i].bits = static_cast<uint16_t>(u >> 16);
+}
+
+// c_s = C c ; lc_s = lc / C ; uc_s = uc / C (infinite bounds stay infinite)
+__global__ void scale_cols_data(int n, const double* __restrict__ C, const double* __restrict__ c,
+                                const double* __restrict__ lc, const double* __restrict__ uc,
+                                double* __restrict__ c_s, double* __restrict__ lc_s,
+                                double* __restrict__ uc_s) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const double s = C[j];
+    c_s[j] = c[j] * s;
+    lc_s[j] = lc[j] / s;
+    uc_s[j] = uc[j] / s;
+}
+__global__ void scale_rows_data(int m, const double* __restrict__ R, const double* __restrict__ lr,
+                                const double* __restrict__ ur, double* __restrict__ lr_s,
+                                double* __restrict__ ur_s) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    lr_s[i] = lr[i] * R[i];
+    ur_s[i] = ur[i] * R[i];
+}
+// per-instance objective/bound rescaling and the unscale factors
+__global__ void rescale_cols(int n, const int* __restrict__ col_inst, const double* __restrict__ cs,
+                             const double* __restrict__ bs, const double* __restrict__ C,
+                             double* __restrict__ c_s, double* __restrict__ lc_s,
+                             double* __restrict__ uc_s, double* __restrict__ ic,
+                             double* __restrict__ px) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int q = col_inst ? col_inst[j] : 0;
+    c_s[j] *= cs[q];
+    lc_s[j] *= bs[q];
+    uc_s[j] *= bs[q];
+    ic[j] = 1.0 / (cs[q] * C[j]);
+    px[j] = C[j] / bs[q];
+}
+__global__ void rescale_rows(int m, const int* __restrict__ row_inst, const double* __restrict__ cs,
+                             const double* __restrict__ bs, const double* __restrict__ R,
+                             double* __restrict__ lr_s, double* __restrict__ ur_s,
+                             double* __restrict__ ir, double* __restrict__ py) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int q = row_inst ? row_inst[i] : 0;
+    lr_s[i] *= bs[q];
+    ur_s[i] *= bs[q];
+    ir[i] = 1.0 / (bs[q] * R[i]);
+    py[i] = R[i] / cs[q];
+}
+// x0 = clamp(0, lc, uc)
+__global__ void initial_point(int n, const double* __restrict__ lc, const double* __restrict__ uc,
+                              double* __restrict__ x) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    double v = 0.0;
+    if (v < lc[j]) v = lc[j];
+    if (v > uc[j]) v = uc[j];
+    x[j] = v;
+}
+// v = u / sqrt(nrm2[inst])
+__global__ void normalize_seg(int n, const int* __restrict__ inst_of, const double* __restrict__ nrm2,
+                              const double* __restrict__ u, double* __restrict__ v) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int q = inst_of ? inst_of[j] : 0;
+    const double s = nrm2[q];
+    v[j] = s > 0.0 ? u[j] / sqrt(s) : 0.0;
+}
+__global__ void init_unit_seg(int n, const int* __restrict__ inst_of, const int* __restrict__ off,
+                              double* __restrict__ v) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int q = inst_of ? inst_of[j] : 0;
+    v[j] = 1.0 / sqrt(static_cast<double>(off[q + 1] - off[q]));
+}
+// original-space solution: x = xfull * px ; y_canonical = yfull * py
+__global__ void unscale_out(int len, const double* __restrict__ a, const double* __restrict__ f,
+                            double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < len) out[i] = a[i] * f[i];
+}
+
+}  // namespace
+
+struct FastEngine {
+    int prec = VJ_PREC_FP64;
+    int count = 0, m = 0, n = 0;
+    int64_t nnz = 0;
+    bool multi = false;
+    int GA = 8, GAt = 4, K = 64;
+    double rho = 1.0;
+    DevBuf av_s, tv_s, av_lo, tv_lo;
+    DevBuf Rsc, Csc, rstep, cstep, rnorm, cnorm;
+    DevBuf ir, ic, pxf, pyf;
+    DevBuf c_s, lc_s, uc_s, lr_s, ur_s;
+    DevBuf x, xbar, y, dhat, lcs, ucs, lrs, urs, yhat_t;
+    DevBuf xhat, yhat, xfull, yfull, ax_full, aty_full;
+    DevBuf part_x, part_y, part_kr, part_kc, part_s;
+    DevBuf kkt8, st, offset_d, norm_b_d, norm_c_d, objfac_d, inst_tmp_a, inst_tmp_b, hist;
+    DevBuf vtmp_n, wtmp_m, utmp_n;
+    int hist_cap = 0;
+    hipGraphExec_t gexec = nullptr;
+    InstState* h_state[2] = {nullptr, nullptr};
+    std::vector<double> spectral;
+    std::vector<double> h_hist;
+    ~FastEngine() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        for (auto* p : h_state)
+            if (p) (void)hipHostFree(p);
+    }
+    size_t tsize() const { return prec == VJ_PREC_FP64 ? sizeof(double) : sizeof(float); }
+};
+
+void destroy_fast(FastEngine* e) { delete e; }
+
+namespace {
+
+// enough slots for the largest lane count (32 lanes: 8 rows per tile)
+int partial_slots(int rows, int count) { return tiles_for(rows, 32) + count + 1; }
+
+// ---- launch helpers ---------------------------------------------------------------
+int require(std::initializer_list<const void*> ptrs, const char* where);
+
+template <class V, class T>
+int launch_primal(vj_ctx* ctx, FastEngine& E, const V* tv, T* x, T* xbar, const T* y, int it, int flags) {
+    Problem& P = ctx->prob;
+    const int tiles = tiles_for(E.n, E.GAt);
+    VJ_TRY(require({P.tp.p, P.ti.p, tv, y, E.dhat.p, E.lcs.p, E.ucs.p, x, xbar, E.xhat.p, E.xfull.p, E.st.p,
+                    E.part_x.p, E.multi ? P.col_inst.p : E.st.p}, "primal_step"));
+    if (E.multi) {
+        VJ_DISPATCH_G(E.GAt, hipLaunchKernelGGL((primal_step<V, T, GG, true>), dim3(tiles), dim3(kBlock), 0, ctx->stream,
+            E.n, P.tp.as<int>(), P.ti.as<int>(), tv, y, E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), x, xbar,
+            E.xhat.as<double>(), E.xfull.as<double>(), E.st.as<InstState>(), P.col_inst.as<int>(),
+            E.part_x.as<double>(), it, flags, E.rho));
+    } else {
+        VJ_DISPATCH_G(E.GAt, hipLaunchKernelGGL((primal_step<V, T, GG, false>), dim3(tiles), dim3(kBlock), 0, ctx->stream,
+            E.n, P.tp.as<int>(), P.ti.as<int>(), tv, y, E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), x, xbar,
+            E.xhat.as<double>(), E.xfull.as<double>(), E.st.as<InstState>(), P.col_inst.as<int>(),
+            E.part_x.as<double>(), it, flags, E.rho));
+    }
+    VJ_LAUNCHED();
+    return VJ_OK;
+}
+
+// FP64 iterates read the FP64 centre directly; FP32/BF16 read its working-precision copy.
+template <class T>
+T* yhat_working(FastEngine& E) {
+    return E.prec == VJ_PREC_FP64 ? reinterpret_cast<T*>(E.yhat.p) : E.yhat_t.as<T>();
+}
+
+// Every device pointer a kernel dereferences must be live: refuse to launch otherwise.
+int require(std::initializer_list<const void*> ptrs, const char* where) {
+    for (const void* q : ptrs)
+        if (!q) {
+            set_error(std::string("internal: null device buffer passed to ") + where);
+            return VJ_ERR_LAUNCH;
+        }
+    return VJ_OK;
+}
+
+template <class V, class T>
+int launch_dual(vj_ctx* ctx, FastEngine& E, const V* av, const T* xbar, T* y, int it, int flags) {
+    Problem& P = ctx->prob;
+    const int tiles = tiles_for(E.m, E.GA);
+    const T* yht = yhat_working<T>(E);
+    VJ_TRY(require({P.rp.p, P.ci.p, av, xbar, yht, E.lrs.p, E.urs.p, y, E.yhat.p, E.yfull.p, E.st.p, E.part_y.p,
+                    E.multi ? P.row_inst.p : E.st.p}, "dual_step"));
+    if (E.multi) {
+        VJ_DISPATCH_G(E.GA, hipLaunchKernelGGL((dual_step<V, T, GG, true>), dim3(tiles), dim3(kBlock), 0, ctx->stream,
+            E.m, P.rp.as<int>(), P.ci.as<int>(), av, xbar, yht, E.lrs.as<T>(), E.urs.as<T>(), y,
+            E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(), P.row_inst.as<int>(),
+            E.part_y.as<double>(), it, flags, E.rho));
+    } else {
+        VJ_DISPATCH_G(E.GA, hipLaunchKernelGGL((dual_step<V, T, GG, false>), dim3(tiles), dim3(kBlock), 0, ctx->stream,
+            E.m, P.rp.as<int>(), P.ci.as<int>(), av, xbar, yht, E.lrs.as<T>(), E.urs.as<T>(), y,
+            E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(), P.row_inst.as<int>(),
+            E.part_y.as<double>(), it, flags, E.rho));
+    }
+    VJ_LAUNCHED();
+    return VJ_OK;
+}
+
+// KKT passes + finish. scaled=true: in-loop check on the scaled problem (writes ax_full /
+// aty_full, unscale arrays applied); scaled=false: audit on original data.
+int launch_kkt(vj_ctx* ctx, FastEngine* E, int GA, int GAt, bool multi, const double* x,
+               const double* y, bool scaled, int only_active, double* part_r, double* part_c,
+               double* out8) {
+    Problem& P = ctx->prob;
+    const int m = P.m, n = P.n;
+    const double* av = scaled ? E->av_s.as<double>() : P.av.as<double>();
+    const double* tv = scaled ? E->tv_s.as<double>() : P.tv.as<double>();
+    const double* lr = scaled ? E->lr_s.as<double>() : P.lr.as<double>();
+    const double* ur = scaled ? E->ur_s.as<double>() : P.ur.as<double>();
+    const double* lc = scaled ? E->lc_s.as<double>() : P.lc.as<double>();
+    const double* uc = scaled ? E->uc_s.as<double>() : P.uc.as<double>();
+    const double* c = scaled ? E->c_s.as<double>() : P.c_can.as<double>();
+    const double* ir = scaled ? E->ir.as<double>() : nullptr;
+    const double* ic = scaled ? E->ic.as<double>() : nullptr;
+    const double* py = scaled ? E->pyf.as<double>() : nullptr;
+    double* axo = scaled ? E->ax_full.as<double>() : nullptr;
+    double* atyo = scaled ? E->aty_full.as<double>() : nullptr;
+    const InstState* st = E ? E->st.as<InstState>() : nullptr;
+    const int tr = tiles_for(m, GA), tc = tiles_for(n, GAt);
+    VJ_TRY(require({P.rp.p, P.ci.p, av, x, y, lr, ur, P.tp.p, P.ti.p, tv, c, lc, uc, part_r, part_c, out8,
+                    P.d_row_off.p, P.d_col_off.p, E->norm_b_d.p, E->norm_c_d.p, E->offset_d.p,
+                    multi ? P.row_inst.p : part_r, multi ? P.col_inst.p : part_r,
+                    only_active ? static_cast<const void*>(st) : part_r},
+                   "kkt passes"));
+    if (scaled) VJ_TRY(require({ir, ic, py, axo, atyo, E->objfac_d.p}, "kkt passes (scaled)"));
+    if (multi) {
+        VJ_DISPATCH_G(GA, hipLaunchKernelGGL((kkt_rows<GG, true>), dim3(tr), dim3(kBlock), 0, ctx->stream, m,
+            P.rp.as<int>(), P.ci.as<int>(), av, x, y, lr, ur, ir, py, axo, st, only_active, P.row_inst.as<int>(), part_r));
+        VJ_DISPATCH_G(GAt, hipLaunchKernelGGL((kkt_cols<GG, true>), dim3(tc), dim3(kBlock), 0, ctx->stream, n,
+            P.tp.as<int>(), P.ti.as<int>(), tv, y, x, c, lc, uc, ic, atyo, st, only_active, P.col_inst.as<int>(), part_c));
+    } else {
+        VJ_DISPATCH_G(GA, hipLaunchKernelGGL((kkt_rows<GG, false>), dim3(tr), dim3(kBlock), 0, ctx->stream, m,
+            P.rp.as<int>(), P.ci.as<int>(), av, x, y, lr, ur, ir, py, axo, st, only_active, P.row_inst.as<int>(), part_r));
+        VJ_DISPATCH_G(GAt, hipLaunchKernelGGL((kkt_cols<GG, false>), dim3(tc), dim3(kBlock), 0, ctx->stream, n,
+            P.tp.as<int>(), P.ti.as<int>(), tv, y, x, c, lc, uc, ic, atyo, st, only_active, P.col_inst.as<int>(), part_c));
+    }
+    VJ_LAUNCHED();
+    const double* objfac = scaled ? E->objfac_d.as<double>() : nullptr;
+    hipLaunchKernelGGL(kkt_finish, dim3(P.count), dim3(kBlock), 0, ctx->stream, part_r, kBlock / GA, part_c,
+                       kBlock / GAt, P.d_row_off.as<int>(), P.d_col_off.as<int>(), E ? E->norm_b_d.as<double>() : nullptr,
+                       E ? E->norm_c_d.as<double>() : nullptr, E ? E->offset_d.as<double>() : nullptr, objfac, out8);
+    VJ_LAUNCHED();
+    return VJ_OK;
+}
+
+// Per-instance sum of squares of a vector (mode 0) or of the finite-bound vector (mode 1).
+int seg_norm2(vj_ctx* ctx, FastEngine& E, int len, int mode, const double* v, const double* lo,
+              const double* hi, bool rows, double* out) {
+    Problem& P = ctx->prob;
+    const int tiles = grid_for(len, kBlock);
+    const int* inst = rows ? P.row_inst.as<int>() : P.col_inst.as<int>();
+    if (E.multi)
+        hipLaunchKernelGGL((seg_sumsq<true>), dim3(tiles), dim3(kBlock), 0, ctx->stream, len, mode, v, lo, hi, inst,
+                           E.part_s.as<double>());
+    else
+        hipLaunchKernelGGL((seg_sumsq<false>), dim3(tiles), dim3(kBlock), 0, ctx->stream, len, mode, v, lo, hi, inst,
+                           E.part_s.as<double>());
+    VJ_LAUNCHED();
+    hipLaunchKernelGGL(seg_finish, dim3(P.count), dim3(kBlock), 0, ctx->stream, E.part_s.as<double>(), 1, 0,
+                       rows ? P.d_row_off.as<int>() : P.d_col_off.as<int>(), kBlock, out);
+    VJ_LAUNCHED();
+    return VJ_OK;
+}
+
+int spmv_f64(vj_ctx* ctx, int rows, const int* rp, const int* ci, const double* av, const double* x, double* y, int G) {
+    VJ_DISPATCH_G(G, hipLaunchKernelGGL((spmv_rows_f64<GG>), dim3(tiles_for(rows, GG)), dim3(kBlock), 0, ctx->stream,
+                                        rows, rp, ci, av, x, y));
+    VJ_LAUNCHED();
+    return VJ_OK;
+}
+
+int choose_lanes(double avg_nnz) {
+    if (avg_nnz <= 3.0) return 2;
+    if (avg_nnz <= 8.0) return 4;
+    if (avg_nnz <= 24.0) return 8;
+    if (avg_nnz <= 64.0) return 16;
+    return 32;
+}
+
+// ---- setup: scaling, step sizes, initial centre -------------------------------------
+int setup(vj_ctx* ctx, FastEngine& E) {
+    Problem& P = ctx->prob;
+    const vj_params& prm = ctx->params;
+    const int m = E.m, n = E.n, cnt = E.count;
+    const int64_t nnz = E.nnz;
+    hipStream_t s = ctx->stream;
+    const size_t T = E.tsize();
+    VJ_TRY(E.av_s.alloc(nnz * 8));
+    VJ_TRY(E.tv_s.alloc(nnz * 8));
+    VJ_HIP(hipMemcpyAsync(E.av_s.p, P.av.p, nnz * 8, hipMemcpyDeviceToDevice, s));
+    VJ_HIP(hipMemcpyAsync(E.tv_s.p, P.tv.p, nnz * 8, hipMemcpyDeviceToDevice, s));
+    for (DevBuf* b : {&E.Rsc, &E.rstep, &E.rnorm, &E.ir, &E.pyf, &E.lr_s, &E.ur_s, &E.yhat, &E.yfull, &E.ax_full, &E.wtmp_m})
+        VJ_TRY(b->alloc(static_cast<size_t>(m) * 8));
+    for (DevBuf* b : {&E.Csc, &E.cstep, &E.cnorm, &E.ic, &E.pxf, &E.c_s, &E.lc_s, &E.uc_s, &E.xhat, &E.xfull,
+                      &E.aty_full, &E.vtmp_n, &E.utmp_n})
+        VJ_TRY(b->alloc(static_cast<size_t>(n) * 8));
+    for (DevBuf* b : {&E.x, &E.xbar, &E.dhat, &E.lcs, &E.ucs}) VJ_TRY(b->alloc(static_cast<size_t>(n) * T));
+    for (DevBuf* b : {&E.y, &E.lrs, &E.urs}) VJ_TRY(b->alloc(static_cast<size_t>(m) * T));
+    if (E.prec != VJ_PREC_FP64) VJ_TRY(E.yhat_t.alloc(static_cast<size_t>(m) * T));
+    const int px_slots = partial_slots(n, cnt), py_slots = partial_slots(m, cnt);
+    VJ_TRY(E.part_x.alloc(static_cast<size_t>(px_slots) * 3 * 8));
+    VJ_TRY(E.part_y.alloc(static_cast<size_t>(py_slots) * 3 * 8));
+    VJ_TRY(E.part_kr.alloc(static_cast<size_t>(py_slots) * 3 * 8));
+    VJ_TRY(E.part_kc.alloc(static_cast<size_t>(px_slots) * 3 * 8));
+    VJ_TRY(E.part_s.alloc(static_cast<size_t>(grid_for(std::max(m, n), kBlock) + cnt + 1) * 8));
+    VJ_TRY(E.kkt8.alloc(static_cast<size_t>(cnt) * 8 * 8));
+    VJ_TRY(E.st.alloc(static_cast<size_t>(cnt) * sizeof(InstState)));
+    for (DevBuf* b : {&E.offset_d, &E.norm_b_d, &E.norm_c_d, &E.objfac_d, &E.inst_tmp_a, &E.inst_tmp_b})
+        VJ_TRY(b->alloc(static_cast<size_t>(cnt) * 8));
+    VJ_HIP(hipMemcpyAsync(E.offset_d.p, P.offset_i.data(), cnt * 8, hipMemcpyHostToDevice, s));
+    VJ_HIP(hipMemcpyAsync(E.norm_b_d.p, P.norm_b.data(), cnt * 8, hipMemcpyHostToDevice, s));
+    VJ_HIP(hipMemcpyAsync(E.norm_c_d.p, P.norm_c.data(), cnt * 8, hipMemcpyHostToDevice, s));
+
+    const int* rp = P.rp.as<int>();
+    const int* ci = P.ci.as<int>();
+    const int* tp = P.tp.as<int>();
+    const int* ti = P.ti.as<int>();
+    double* av = E.av_s.as<double>();
+    double* tv = E.tv_s.as<double>();
+    const int GA = E.GA, GAt = E.GAt;
+    hipLaunchKernelGGL(fill_f64, dim3(grid_for(m, kBlock)), dim3(kBlock), 0, s, m, E.Rsc.as<double>(), 1.0);
+    hipLaunchKernelGGL(fill_f64, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, E.Csc.as<double>(), 1.0);
+    VJ_LAUNCHED();
+
+    // Ruiz l_inf equilibration (fixed number of passes), then Pock-Chambolle (alpha = 1).
+    const int passes = prm.enable_scaling ? prm.ruiz_iterations : 0;
+    for (int pass = 0; pass < passes + (prm.enable_scaling && prm.pock_chambolle ? 1 : 0); ++pass) {
+        const int op = pass < passes ? 0 : 1;
+        VJ_DISPATCH_G(GA, hipLaunchKernelGGL((row_abs_reduce<GG>), dim3(tiles_for(m, GG)), dim3(kBlock), 0, s, m, rp, av,
+                                             op, E.rnorm.as<double>()));
+        VJ_DISPATCH_G(GAt, hipLaunchKernelGGL((row_abs_reduce<GG>), dim3(tiles_for(n, GG)), dim3(kBlock), 0, s, n, tp, tv,
+                                              op, E.cnorm.as<double>()));
+        hipLaunchKernelGGL(make_scale_step, dim3(grid_for(m, kBlock)), dim3(kBlock), 0, s, m, E.rnorm.as<double>(),
+                           E.rstep.as<double>(), E.Rsc.as<double>());
+        hipLaunchKernelGGL(make_scale_step, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, E.cnorm.as<double>(),
+                           E.cstep.as<double>(), E.Csc.as<double>());
+        VJ_DISPATCH_G(GA, hipLaunchKernelGGL((scale_rows<GG>), dim3(tiles_for(m, GG)), dim3(kBlock), 0, s, m, rp, ci, av,
+                                             E.rstep.as<double>(), E.cstep.as<double>()));
+        VJ_DISPATCH_G(GAt, hipLaunchKernelGGL((scale_rows<GG>), dim3(tiles_for(n, GG)), dim3(kBlock), 0, s, n, tp, ti, tv,
+                                              E.cstep.as<double>(), E.rstep.as<double>()));
+        VJ_LAUNCHED();
+    }
+    hipLaunchKernelGGL(scale_cols_data, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, E.Csc.as<double>(),
+                       P.c_can.as<double>(), P.lc.as<double>(), P.uc.as<double>(), E.c_s.as<double>(),
+                       E.lc_s.as<double>(), E.uc_s.as<double>());
+    hipLaunchKernelGGL(scale_rows_data, dim3(grid_for(m, kBlock)), dim3(kBlock), 0, s, m, E.Rsc.as<double>(),
+                       P.lr.as<double>(), P.ur.as<double>(), E.lr_s.as<double>(), E.ur_s.as<double>());
+    VJ_LAUNCHED();
+    // bound / objective rescaling: cs = 1/(1+|c_s|), bs = 1/(1+|b_s|) per instance
+    VJ_TRY(seg_norm2(ctx, E, n, 0, E.c_s.as<double>(), nullptr, nullptr, false, E.inst_tmp_a.as<double>()));
+    VJ_TRY(seg_norm2(ctx, E, m, 1, nullptr, E.lr_s.as<double>(), E.ur_s.as<double>(), true, E.inst_tmp_b.as<double>()));
+    std::vector<double> cn2(cnt), bn2(cnt), cs(cnt, 1.0), bs(cnt, 1.0), objfac(cnt), omega0(cnt, 1.0);
+    VJ_HIP(hipMemcpyAsync(cn2.data(), E.inst_tmp_a.p, cnt * 8, hipMemcpyDeviceToHost, s));
+    VJ_HIP(hipMemcpyAsync(bn2.data(), E.inst_tmp_b.p, cnt * 8, hipMemcpyDeviceToHost, s));
+    VJ_HIP(hipStreamSynchronize(s));
+    for (int q = 0; q < cnt; ++q) {
+        const double cnorm = std::sqrt(cn2[q]), bnorm = std::sqrt(bn2[q]);
+        if (prm.bound_objective_rescaling) {
+            cs[q] = 1.0 / (1.0 + cnorm);
+            bs[q] = 1.0 / (1.0 + bnorm);
+        }
+        objfac[q] = 1.0 / (cs[q] * bs[q]);
+        const double cr = cnorm * cs[q], br = bnorm * bs[q];
+        omega0[q] = (cr > 1e-10 && br > 1e-10) ? cr / br : 1.0;
+    }
+    VJ_HIP(hipMemcpyAsync(E.inst_tmp_a.p, cs.data(), cnt * 8, hipMemcpyHostToDevice, s));
+    VJ_HIP(hipMemcpyAsync(E.inst_tmp_b.p, bs.data(), cnt * 8, hipMemcpyHostToDevice, s));
+    VJ_HIP(hipMemcpyAsync(E.objfac_d.p, objfac.data(), cnt * 8, hipMemcpyHostToDevice, s));
+    const int* cinst = E.multi ? P.col_inst.as<int>() : nullptr;
+    const int* rinst = E.multi ? P.row_inst.as<int>() : nullptr;
+    hipLaunchKernelGGL(rescale_cols, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, cinst, E.inst_tmp_a.as<double>(),
+                       E.inst_tmp_b.as<double>(), E.Csc.as<double>(), E.c_s.as<double>(), E.lc_s.as<double>(),
+                       E.uc_s.as<double>(), E.ic.as<double>(), E.pxf.as<double>());
+    hipLaunchKernelGGL(rescale_rows, dim3(grid_for(m, kBlock)), dim3(kBlock), 0, s, m, rinst, E.inst_tmp_a.as<double>(),
+                       E.inst_tmp_b.as<double>(), E.Rsc.as<double>(), E.lr_s.as<double>(), E.ur_s.as<double>(),
+                       E.ir.as<double>(), E.pyf.as<double>());
+    VJ_LAUNCHED();
+
+    // Power iteration on A_s^T A_s per instance: lambda ~ |A_s|_2^2.
+    double* v = E.vtmp_n.as<double>();
+    double* u = E.utmp_n.as<double>();
+    double* w = E.wtmp_m.as<double>();
+    hipLaunchKernelGGL(init_unit_seg, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, cinst, P.d_col_off.as<int>(), v);
+    VJ_LAUNCHED();
+    const int power_iters = 64;
+    for (int it = 0; it < power_iters; ++it) {
+        VJ_TRY(spmv_f64(ctx, m, rp, ci, av, v, w, GA));
+        VJ_TRY(spmv_f64(ctx, n, tp, ti, tv, w, u, GAt));
+        VJ_TRY(seg_norm2(ctx, E, n, 0, u, nullptr, nullptr, false, E.inst_tmp_a.as<double>()));
+        hipLaunchKernelGGL(normalize_seg, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, cinst,
+                           E.inst_tmp_a.as<double>(), u, v);
+        VJ_LAUNCHED();
+    }
+    std::vector<double> lam2(cnt);
+    VJ_HIP(hipMemcpyAsync(lam2.data(), E.inst_tmp_a.p, cnt * 8, hipMemcpyDeviceToHost, s));
+    VJ_HIP(hipStreamSynchronize(s));
+    E.spectral.assign(cnt, 1.0);
+    for (int q = 0; q < cnt; ++q) {
+        const double lam = std::sqrt(lam2[q]);  // |A^T A v| with |v| = 1
+        E.spectral[q] = lam > 0.0 ? std::sqrt(lam) : 1.0;
+    }
+
+    // low-precision matrix copies
+    if (E.prec == VJ_PREC_FP32) {
+        VJ_TRY(E.av_lo.alloc(nnz * 4));
+        VJ_TRY(E.tv_lo.alloc(nnz * 4));
+        hipLaunchKernelGGL(convert_f32, dim3(grid_for(nnz, kBlock)), dim3(kBlock), 0, s, nnz, av, E.av_lo.as<float>());
+        hipLaunchKernelGGL(convert_f32, dim3(grid_for(nnz, kBlock)), dim3(kBlock), 0, s, nnz, tv, E.tv_lo.as<float>());
+        VJ_LAUNCHED();
+    } else if (E.prec == VJ_PREC_BF16) {
+        VJ_TRY(E.av_lo.alloc(nnz * 2));
+        VJ_TRY(E.tv_lo.alloc(nnz * 2));
+        hipLaunchKernelGGL(convert_bf16, dim3(grid_for(nnz, kBlock)), dim3(kBlock), 0, s, nnz, av, E.av_lo.as<bf16_t>());
+        hipLaunchKernelGGL(convert_bf16, dim3(grid_for(nnz, kBlock)), dim3(kBlock), 0, s, nnz, tv, E.tv_lo.as<bf16_t>());
+        VJ_LAUNCHED();
+    }
+
+    // state
+    std::vector<InstState> hs(cnt);
+    for (int q = 0; q < cnt; ++q) {
+        InstState& z = hs[q];
+        std::memset(&z, 0, sizeof(z));
+        z.eta = prm.step_scale / E.spectral[q];
+        z.omega = omega0[q];
+        z.tau = z.eta / z.omega;
+        z.sigma = z.eta * z.omega;
+        z.norm_b = P.norm_b[q];
+        z.norm_c = P.norm_c[q];
+        z.obj_scale = cs[q];
+        z.bound_scale = bs[q];
+        z.tol = prm.tolerance;
+        z.max_iters = prm.max_iterations;
+        z.active = 1;
+        z.restarted_prev = 1;
+        z.r0 = INFINITY;
+        z.r_prev = INFINITY;
+        z.time_to_target = -1.0;
+    }
+    VJ_HIP(hipMemcpyAsync(E.st.p, hs.data(), cnt * sizeof(InstState), hipMemcpyHostToDevice, s));
+
+    // initial centre: x0 = clamp(0, lc_s, uc_s), y0 = 0
+    hipLaunchKernelGGL(initial_point, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, E.lc_s.as<double>(),
+                       E.uc_s.as<double>(), E.xfull.as<double>());
+    hipLaunchKernelGGL(fill_f64, dim3(grid_for(m, kBlock)), dim3(kBlock), 0, s, m, E.yfull.as<double>(), 0.0);
+    VJ_LAUNCHED();
+    VJ_TRY(launch_kkt(ctx, &E, GA, GAt, E.multi, E.xfull.as<double>(), E.yfull.as<double>(), true, 0,
+                      E.part_kr.as<double>(), E.part_kc.as<double>(), E.kkt8.as<double>()));
+    // host staging vectors of this function are read by queued copies: finish them here
+    VJ_HIP(hipStreamSynchronize(s));
+    return VJ_OK;
+}
+
+template <class T>
+int launch_restart(vj_ctx* ctx, FastEngine& E, int force) {
+    Problem& P = ctx->prob;
+    hipStream_t s = ctx->stream;
+    T* yhat_t = yhat_working<T>(E);
+    VJ_TRY(require({E.st.p, E.xfull.p, E.aty_full.p, E.c_s.p, E.lc_s.p, E.uc_s.p, E.xhat.p, E.x.p, E.dhat.p,
+                    E.lcs.p, E.ucs.p, E.yfull.p, E.ax_full.p, E.lr_s.p, E.ur_s.p, E.yhat.p, yhat_t, E.y.p,
+                    E.lrs.p, E.urs.p, E.multi ? P.row_inst.p : E.st.p, E.multi ? P.col_inst.p : E.st.p},
+                   "restart kernels"));
+    if (E.multi) {
+        hipLaunchKernelGGL((restart_cols<T, true>), dim3(grid_for(E.n, kBlock)), dim3(kBlock), 0, s, E.n,
+                           E.st.as<InstState>(), P.col_inst.as<int>(), force, E.xfull.as<double>(), E.aty_full.as<double>(),
+                           E.c_s.as<double>(), E.lc_s.as<double>(), E.uc_s.as<double>(), E.xhat.as<double>(), E.x.as<T>(),
+                           E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>());
+        hipLaunchKernelGGL((restart_rows<T, true>), dim3(grid_for(E.m, kBlock)), dim3(kBlock), 0, s, E.m,
+                           E.st.as<InstState>(), P.row_inst.as<int>(), force, E.yfull.as<double>(), E.ax_full.as<double>(),
+                           E.lr_s.as<double>(), E.ur_s.as<double>(), E.yhat.as<double>(), yhat_t, E.y.as<T>(),
+                           E.lrs.as<T>(), E.urs.as<T>());
+    } else {
+        hipLaunchKernelGGL((restart_cols<T, false>), dim3(grid_for(E.n, kBlock)), dim3(kBlock), 0, s, E.n,
+                           E.st.as<InstState>(), P.col_inst.as<int>(), force, E.xfull.as<double>(), E.aty_full.as<double>(),
+                           E.c_s.as<double>(), E.lc_s.as<double>(), E.uc_s.as<double>(), E.xhat.as<double>(), E.x.as<T>(),
+                           E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>());
+        hipLaunchKernelGGL((restart_rows<T, false>), dim3(grid_for(E.m, kBlock)), dim3(kBlock), 0, s, E.m,
+                           E.st.as<InstState>(), P.row_inst.as<int>(), force, E.yfull.as<double>(), E.ax_full.as<double>(),
+                           E.lr_s.as<double>(), E.ur_s.as<double>(), E.yhat.as<double>(), yhat_t, E.y.as<T>(),
+                           E.lrs.as<T>(), E.urs.as<T>());
+    }
+    VJ_LAUNCHED();
+    return VJ_OK;
+}
+
+// One block: K iterations, the FP64 check, control and the conditional restart.
+template <class V, class T>
+int enqueue_block(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
+    for (int it = 0; it < E.K; ++it) {
+        int flags = 0;
+        if (it == 0) flags |= 1;
+        if (it == E.K - 1) flags |= 2;
+        VJ_TRY(launch_primal<V, T>(ctx, E, tv, E.x.as<T>(), E.xbar.as<T>(), E.y.as<T>(), it, flags));
+        VJ_TRY(launch_dual<V, T>(ctx, E, av, E.xbar.as<T>(), E.y.as<T>(), it, flags));
+    }
+    VJ_TRY(launch_kkt(ctx, &E, E.GA, E.GAt, E.multi, E.xfull.as<double>(), E.yfull.as<double>(), true, 1,
+                      E.part_kr.as<double>(), E.part_kc.as<double>(), E.kkt8.as<double>()));
+    ControlArgs a;
+    a.px = E.part_x.as<double>();
+    a.Rx = kBlock / E.GAt;
+    a.py = E.part_y.as<double>();
+    a.Ry = kBlock / E.GA;
+    a.kkt = E.kkt8.as<double>();
+    a.row_off = ctx->prob.d_row_off.as<int>();
+    a.col_off = ctx->prob.d_col_off.as<int>();
+    a.K = E.K;
+    a.restart_sufficient = 0.2;
+    a.restart_necessary = 0.8;
+    a.restart_artificial = 0.36;
+    a.hist = E.hist_cap > 0 ? E.hist.as<double>() : nullptr;
+    a.hist_cap = E.hist_cap;
+    hipLaunchKernelGGL(control_kernel, dim3(E.count), dim3(kBlock), 0, ctx->stream, E.st.as<InstState>(), a);
+    VJ_LAUNCHED();
+    VJ_TRY(launch_restart<T>(ctx, E, 0));
+    return VJ_OK;
+}
+
+// Time `reps` launches of each iteration kernel (no partials) on scratch copies of the
+// iterates; returns average microseconds.
+template <class V, class T>
+int time_kernels(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv, int reps, double* us_primal, double* us_dual) {
+    hipStream_t s = ctx->stream;
+    const size_t Tn = static_cast<size_t>(E.n) * E.tsize(), Tm = static_cast<size_t>(E.m) * E.tsize();
+    DevBuf xs, xbs, ys;
+    VJ_TRY(xs.alloc(Tn));
+    VJ_TRY(xbs.alloc(Tn));
+    VJ_TRY(ys.alloc(Tm));
+    VJ_HIP(hipMemcpyAsync(xs.p, E.x.p, Tn, hipMemcpyDeviceToDevice, s));
+    VJ_HIP(hipMemcpyAsync(xbs.p, E.xbar.p, Tn, hipMemcpyDeviceToDevice, s));
+    VJ_HIP(hipMemcpyAsync(ys.p, E.y.p, Tm, hipMemcpyDeviceToDevice, s));
+    hipEvent_t e0, e1, e2;
+    VJ_HIP(hipEventCreate(&e0));
+    VJ_HIP(hipEventCreate(&e1));
+    VJ_HIP(hipEventCreate(&e2));
+    // warm-up
+    VJ_TRY(launch_primal<V, T>(ctx, E, tv, xs.as<T>(), xbs.as<T>(), ys.as<T>(), 1, 0));
+    VJ_TRY(launch_dual<V, T>(ctx, E, av, xbs.as<T>(), ys.as<T>(), 1, 0));
+    VJ_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) VJ_TRY(launch_primal<V, T>(ctx, E, tv, xs.as<T>(), xbs.as<T>(), ys.as<T>(), 1, 0));
+    VJ_HIP(hipEventRecord(e1, s));
+    for (int r = 0; r < reps; ++r) VJ_TRY(launch_dual<V, T>(ctx, E, av, xbs.as<T>(), ys.as<T>(), 1, 0));
+    VJ_HIP(hipEventRecord(e2, s));
+    VJ_HIP(hipEventSynchronize(e2));
+    float ms1 = 0, ms2 = 0;
+    VJ_HIP(hipEventElapsedTime(&ms1, e0, e1));
+    VJ_HIP(hipEventElapsedTime(&ms2, e1, e2));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    *us_primal = 1000.0 * ms1 / reps;
+    *us_dual = 1000.0 * ms2 / reps;
+    return VJ_OK;
+}
+
+// Autotune lanes per row for A (dual_step) and A^T (primal_step) on the real data.
+template <class V, class T>
+int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
+    const double avgA = static_cast<double>(E.nnz) / std::max(1, E.m);
+    const double avgAt = static_cast<double>(E.nnz) / std::max(1, E.n);
+    const bool tuneA = ctx->params.lanes_A == 0, tuneAt = ctx->params.lanes_At == 0;
+    E.GA = tuneA ? choose_lanes(avgA) : ctx->params.lanes_A;
+    E.GAt = tuneAt ? choose_lanes(avgAt) : ctx->params.lanes_At;
+    if (!tuneA && !tuneAt) return VJ_OK;
+    if (E.nnz < 200000) return VJ_OK;  // launch-bound sizes: the heuristic is as good as timing
+    const int reps = 3;
+    double bestA = 1e30, bestAt = 1e30;
+    int gA = E.GA, gAt = E.GAt;
+    for (int g : {2, 4, 8, 16, 32}) {
+        if (tuneA) E.GA = g;
+        if (tuneAt) E.GAt = g;
+        double up = 0, ud = 0;
+        VJ_TRY((time_kernels<V, T>(ctx, E, av, tv, reps, &up, &ud)));
+        if (tuneAt && up < bestAt) { bestAt = up; gAt = g; }
+        if (tuneA && ud < bestA) { bestA = ud; gA = g; }
+    }
+    E.GA = gA;
+    E.GAt = gAt;
+    return VJ_OK;
+}
+
+template <class V, class T>
+int run_typed(vj_ctx* ctx, FastEngine& E, vj_result* out, clk::time_point t_start) {
+    hipStream_t s = ctx->stream;
+    const vj_params& prm = ctx->params;
+    const V* av = nullptr;
+    const V* tv = nullptr;
+    if constexpr (sizeof(V) == 8) {
+        av = reinterpret_cast<const V*>(E.av_s.p);
+        tv = reinterpret_cast<const V*>(E.tv_s.p);
+    } else {
+        av = reinterpret_cast<const V*>(E.av_lo.p);
+        tv = reinterpret_cast<const V*>(E.tv_lo.p);
+    }
+    VJ_TRY(launch_restart<T>(ctx, E, 1));  // initial centring
+    VJ_TRY((autotune<V, T>(ctx, E, av, tv)));
+    // after autotuning the scratch runs left the real iterates untouched; recentre anyway
+    VJ_TRY(launch_restart<T>(ctx, E, 1));
+    E.K = prm.check_frequency > 0 ? prm.check_frequency : 64;
+    VJ_HIP(hipStreamSynchronize(s));
+    const auto t_iter0 = clk::now();
+
+    const int cnt = E.count;
+    for (int b = 0; b < 2; ++b)
+        if (!E.h_state[b]) VJ_HIP(hipHostMalloc(reinterpret_cast<void**>(&E.h_state[b]), cnt * sizeof(InstState), 0));
+    bool use_graph = prm.use_graph != 0;
+    if (E.gexec) {
+        (void)hipGraphExecDestroy(E.gexec);
+        E.gexec = nullptr;
+    }
+    if (use_graph) {
+        hipGraph_t g = nullptr;
+        VJ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_block<V, T>(ctx, E, av, tv);
+        hipError_t ec = hipStreamEndCapture(s, &g);
+        if (rc != VJ_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        VJ_HIP(ec);
+        VJ_HIP(hipGraphInstantiate(&E.gexec, g, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(g);
+    }
+    hipEvent_t ev[2], ev0;
+    VJ_HIP(hipEventCreate(&ev[0]));
+    VJ_HIP(hipEventCreate(&ev[1]));
+    VJ_HIP(hipEventCreate(&ev0));
+    VJ_HIP(hipEventRecord(ev0, s));
+    std::vector<double> conv_time(cnt, -1.0);
+    int64_t transfers = 0;
+    bool all_done = false, timed_out = false;
+    for (int64_t b = 0;; ++b) {
+        if (use_graph) VJ_HIP(hipGraphLaunch(E.gexec, s));
+        else VJ_TRY((enqueue_block<V, T>(ctx, E, av, tv)));
+        VJ_HIP(hipMemcpyAsync(E.h_state[b & 1], E.st.p, cnt * sizeof(InstState), hipMemcpyDeviceToHost, s));
+        VJ_HIP(hipEventRecord(ev[b & 1], s));
+        ++transfers;
+        if (b >= 1) {
+            VJ_HIP(hipEventSynchronize(ev[(b - 1) & 1]));
+            const InstState* hs = E.h_state[(b - 1) & 1];
+            float ms = 0.0f;
+            VJ_HIP(hipEventElapsedTime(&ms, ev0, ev[(b - 1) & 1]));
+            const double t_now = 1e-3 * ms;  // device time at the end of the observed block
+            int active = 0;
+            for (int q = 0; q < cnt; ++q) {
+                if (hs[q].active) ++active;
+                if (hs[q].converged && conv_time[q] < 0.0) conv_time[q] = t_now;
+            }
+            if (active == 0) { all_done = true; break; }
+            if (secs(t_start, clk::now()) > prm.time_limit) { timed_out = true; break; }
+        }
+    }
+    VJ_HIP(hipStreamSynchronize(s));
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    (void)hipEventDestroy(ev0);
+    const auto t_iter1 = clk::now();
+    (void)all_done;
+
+    // final state and solution (original space)
+    std::vector<InstState> hs(cnt);
+    VJ_HIP(hipMemcpy(hs.data(), E.st.p, cnt * sizeof(InstState), hipMemcpyDeviceToHost));
+    Problem& P = ctx->prob;
+    VJ_TRY(ctx->x_sol.alloc(static_cast<size_t>(E.n) * 8));
+    VJ_TRY(ctx->y_sol.alloc(static_cast<size_t>(E.m) * 8));
+    hipLaunchKernelGGL(unscale_out, dim3(grid_for(E.n, kBlock)), dim3(kBlock), 0, s, E.n, E.xfull.as<double>(),
+                       E.pxf.as<double>(), ctx->x_sol.as<double>());
+    hipLaunchKernelGGL(unscale_out, dim3(grid_for(E.m, kBlock)), dim3(kBlock), 0, s, E.m, E.yfull.as<double>(),
+                       E.pyf.as<double>(), ctx->y_sol.as<double>());
+    VJ_LAUNCHED();
+    ctx->have_solution = true;
+    std::vector<double> audit(static_cast<size_t>(cnt) * 8);
+    VJ_TRY(audit_device(ctx, ctx->x_sol.as<double>(), ctx->y_sol.as<double>(), audit.data()));
+    if (E.hist_cap > 0) {
+        E.h_hist.resize(static_cast<size_t>(cnt) * E.hist_cap);
+        VJ_HIP(hipMemcpy(E.h_hist.data(), E.hist.p, E.h_hist.size() * 8, hipMemcpyDeviceToHost));
+    }
+    const auto t_end = clk::now();
+    for (int q = 0; q < cnt; ++q) {
+        vj_result& r = out[q];
+        std::memset(&r, 0, sizeof(r));
+        const InstState& z = hs[q];
+        r.converged = z.converged;
+        r.iterations = z.iters;
+        r.restart_count = z.restarts;
+        r.rel_kkt = audit[8 * q + 0];
+        r.rel_primal = audit[8 * q + 1];
+        r.rel_dual = audit[8 * q + 2];
+        r.rel_gap = audit[8 * q + 3];
+        r.primal_objective = audit[8 * q + 4];
+        r.dual_objective = audit[8 * q + 5];
+        r.spectral_norm = E.spectral[q];
+        r.tau = z.tau;
+        r.sigma = z.sigma;
+        r.primal_weight = z.omega;
+        r.primal_residual = z.rel_p;
+        r.dual_residual = z.rel_d;
+        r.duality_gap = z.rel_gap;
+        r.iteration_time = secs(t_iter0, t_iter1);
+        r.total_time = secs(t_start, t_end);
+        r.time_to_target = conv_time[q];
+        r.host_device_transfers = transfers;
+        r.refinements = E.prec == VJ_PREC_FP64 ? 0 : z.restarts;
+        r.lanes_A = E.GA;
+        r.lanes_At = E.GAt;
+        r.check_interval = E.K;
+        r.checks = z.checks;
+        if (z.nan_flag) r.status = VJ_STATUS_NUMERICAL_ERROR;
+        else if (z.converged) r.status = VJ_STATUS_OPTIMAL;
+        else if (timed_out) r.status = VJ_STATUS_TIME_LIMIT;
+        else r.status = VJ_STATUS_ITERATION_LIMIT;
+        (void)P;
+    }
+    return VJ_OK;
+}
+
+}  // namespace
+
+int audit_device(vj_ctx* ctx, const double* d_x, const double* d_y, double* out8) {
+    Problem& P = ctx->prob;
+    const int cnt = P.count;
+    DevBuf pr, pc, o8, offs, nb, nc;
+    VJ_TRY(pr.alloc(static_cast<size_t>(partial_slots(P.m, cnt)) * 3 * 8));
+    VJ_TRY(pc.alloc(static_cast<size_t>(partial_slots(P.n, cnt)) * 3 * 8));
+    VJ_TRY(o8.alloc(static_cast<size_t>(cnt) * 64));
+    VJ_TRY(offs.alloc(static_cast<size_t>(cnt) * 8));
+    VJ_TRY(nb.alloc(static_cast<size_t>(cnt) * 8));
+    VJ_TRY(nc.alloc(static_cast<size_t>(cnt) * 8));
+    hipStream_t s = ctx->stream;
+    VJ_HIP(hipMemcpyAsync(offs.p, P.offset_i.data(), cnt * 8, hipMemcpyHostToDevice, s));
+    VJ_HIP(hipMemcpyAsync(nb.p, P.norm_b.data(), cnt * 8, hipMemcpyHostToDevice, s));
+    VJ_HIP(hipMemcpyAsync(nc.p, P.norm_c.data(), cnt * 8, hipMemcpyHostToDevice, s));
+    const int GA = choose_lanes(static_cast<double>(P.nnz) / std::max(1, P.m));
+    const int GAt = choose_lanes(static_cast<double>(P.nnz) / std::max(1, P.n));
+    // the audit reads the original data only; a temporary engine view carries the norms
+    FastEngine view;
+    view.norm_b_d.p = nb.p;
+    view.norm_c_d.p = nc.p;
+    view.offset_d.p = offs.p;
+    int rc = launch_kkt(ctx, &view, GA, GAt, cnt > 1, d_x, d_y, false, 0, pr.as<double>(), pc.as<double>(),
+                        o8.as<double>());
+    view.norm_b_d.p = nullptr;  // not owned
+    view.norm_c_d.p = nullptr;
+    view.offset_d.p = nullptr;
+    if (rc != VJ_OK) return rc;
+    VJ_HIP(hipMemcpyAsync(out8, o8.p, static_cast<size_t>(cnt) * 64, hipMemcpyDeviceToHost, s));
+    VJ_HIP(hipStreamSynchronize(s));
+    return VJ_OK;
+}
+
+int fast_solve(vj_ctx* ctx, vj_result* out) {
+    const auto t0 = clk::now();
+    Problem& P = ctx->prob;
+    if (!ctx->fast) ctx->fast.reset(new FastEngine());
+    FastEngine& E = *ctx->fast;
+    E.prec = ctx->params.precision;
+    E.count = P.count;
+    E.m = P.m;
+    E.n = P.n;
+    E.nnz = P.nnz;
+    E.multi = P.count > 1;
+    E.rho = ctx->params.reflection;
+    E.GA = ctx->params.lanes_A > 0 ? ctx->params.lanes_A : choose_lanes(static_cast<double>(P.nnz) / std::max(1, P.m));
+    E.GAt = ctx->params.lanes_At > 0 ? ctx->params.lanes_At : choose_lanes(static_cast<double>(P.nnz) / std::max(1, P.n));
+    E.hist_cap = P.count == 1 ? 1 << 14 : 0;
+    if (E.hist_cap) VJ_TRY(E.hist.alloc(static_cast<size_t>(E.hist_cap) * 8));
+    VJ_TRY(setup(ctx, E));
+    VJ_HIP(hipStreamSynchronize(ctx->stream));
+    const double setup_time = secs(t0, clk::now());
+    int rc;
+    switch (E.prec) {
+        case VJ_PREC_FP64: rc = run_typed<double, double>(ctx, E, out, t0); break;
+        case VJ_PREC_FP32: rc = run_typed<float, float>(ctx, E, out, t0); break;
+        case VJ_PREC_BF16: rc = run_typed<bf16_t, float>(ctx, E, out, t0); break;
+        default: set_error("unknown precision"); return VJ_ERR_INVALID_ARGUMENT;
+    }
+    if (rc != VJ_OK) return rc;
+    for (int q = 0; q < P.count; ++q) out[q].setup_time = setup_time;
+    return VJ_OK;
+}
+
+int fast_benchmark(vj_ctx* ctx, int reps, vj_kernel_stat* out, int cap, int* written) {
+    *written = 0;
+    if (!ctx->fast) {
+        set_error("no restarted-mode solve has run on this context");
+        return VJ_ERR_NOT_READY;
+    }
+    FastEngine& E = *ctx->fast;
+    double up = 0, ud = 0;
+    int rc;
+    switch (E.prec) {
+        case VJ_PREC_FP64:
+            rc = time_kernels<double, double>(ctx, E, E.av_s.as<double>(), E.tv_s.as<double>(), reps, &up, &ud);
+            break;
+        case VJ_PREC_FP32:
+            rc = time_kernels<float, float>(ctx, E, E.av_lo.as<float>(), E.tv_lo.as<float>(), reps, &up, &ud);
+            break;
+        default:
+            rc = time_kernels<bf16_t, float>(ctx, E, E.av_lo.as<bf16_t>(), E.tv_lo.as<bf16_t>(), reps, &up, &ud);
+            break;
+    }
+    if (rc != VJ_OK) return rc;
+    const double sv = E.prec == VJ_PREC_FP64 ? 8 : (E.prec == VJ_PREC_FP32 ? 4 : 2);
+    const double sx = E.prec == VJ_PREC_FP64 ? 8 : 4;
+    const double m = E.m, n = E.n, nnz = static_cast<double>(E.nnz);
+    // algorithmic bytes: matrix once, pointers once, gathered vector once, per-row streams
+    // primal_step rows of A^T: reads y (gather, m), x, dhat, lcs, ucs (4n); writes x, xbar (2n)
+    const double b_primal = nnz * (sv + 4) + (n + 1) * 4 + m * sx + 6 * n * sx;
+    // dual_step rows of A: reads xbar (gather, n), y, yhat, lrs, urs (4m); writes y (m)
+    const double b_dual = nnz * (sv + 4) + (m + 1) * 4 + n * sx + 5 * m * sx;
+    const char* names[2] = {"primal_step", "dual_step"};
+    const double us[2] = {up, ud};
+    const double bytes[2] = {b_primal, b_dual};
+    int k = 0;
+    for (int q = 0; q < 2 && k < cap; ++q, ++k) {
+        std::snprintf(out[k].name, sizeof(out[k].name), "%s", names[q]);
+        out[k].avg_us = us[q];
+        out[k].launches = reps;
+        out[k].bytes_per_launch = bytes[q];
+    }
+    *written = k;
+    return VJ_OK;
+}
+
+}  // namespace vj

@@ -1,0 +1,503 @@
+// Kernels of the restarted PDHG engine (single LP and block-diagonal batches).
+//
+// Row-group SpMV: a 256-thread workgroup covers a tile of R = 256/G consecutive rows;
+// each row is handled by G consecutive lanes of a 64-wide wavefront (G = 2..32), which
+// stride over the row's nonzeros so that the wavefront reads one contiguous span of the
+// value and index streams; the group sum is a fixed xor butterfly (deterministic).
+// Epilogues (projection, extrapolation, reflected Halpern step, residual partial sums)
+// run in the group's lane 0 right after the reduction, so each iteration is two kernels
+// with no intermediate vector round trip through HBM.
+#pragma once
+#include "vj_device.hpp"
+#include "vj_internal.hpp"
+
+namespace vj {
+
+constexpr double kInfBound = 1e20;
+
+// ---------------------------------------------------------------------------------
+// Generic FP64 SpMV y = A x over CSR (int32 pointers); used by setup (power iteration).
+template <int G>
+__global__ __launch_bounds__(kBlock) void spmv_rows_f64(int rows, const int* __restrict__ rp,
+                                                        const int* __restrict__ ci,
+                                                        const double* __restrict__ av,
+                                                        const double* __restrict__ x,
+                                                        double* __restrict__ y) {
+    const int r = blockIdx.x * (kBlock / G) + threadIdx.x / G;
+    const int lane = threadIdx.x % G;
+    double acc = 0.0;
+    if (r < rows)
+        for (int p = rp[r] + lane; p < rp[r + 1]; p += G) acc += av[p] * x[ci[p]];
+    acc = group_sum<G>(acc);
+    if (r < rows && lane == 0) y[r] = acc;
+}
+
+// Per-row max |a| (op 0) or sum |a| (op 1).
+template <int G>
+__global__ __launch_bounds__(kBlock) void row_abs_reduce(int rows, const int* __restrict__ rp,
+                                                         const double* __restrict__ av, int op,
+                                                         double* __restrict__ out) {
+    const int r = blockIdx.x * (kBlock / G) + threadIdx.x / G;
+    const int lane = threadIdx.x % G;
+    double acc = 0.0;
+    if (r < rows)
+        for (int p = rp[r] + lane; p < rp[r + 1]; p += G) {
+            const double a = fabs(av[p]);
+            acc = op == 0 ? (acc < a ? a : acc) : acc + a;
+        }
+    acc = op == 0 ? group_max<G>(acc) : group_sum<G>(acc);
+    if (r < rows && lane == 0) out[r] = acc;
+}
+
+// v_p *= s_own[row] * s_other[idx[p]]  (the same product for A and A^T keeps them exact
+// transposes of each other).
+template <int G>
+__global__ __launch_bounds__(kBlock) void scale_rows(int rows, const int* __restrict__ rp,
+                                                     const int* __restrict__ idx,
+                                                     double* __restrict__ av,
+                                                     const double* __restrict__ s_own,
+                                                     const double* __restrict__ s_other) {
+    const int r = blockIdx.x * (kBlock / G) + threadIdx.x / G;
+    const int lane = threadIdx.x % G;
+    if (r >= rows) return;
+    const double so = s_own[r];
+    for (int p = rp[r] + lane; p < rp[r + 1]; p += G) av[p] *= so * s_other[idx[p]];
+}
+
+// factor = norm > 1e-11 ? 1/sqrt(norm) : 1; scale *= factor
+__global__ void make_scale_step(int len, const double* __restrict__ norm,
+                                double* __restrict__ step, double* __restrict__ cum) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= len) return;
+    const double v = norm[i];
+    const double f = v > 1e-11 ? 1.0 / sqrt(v) : 1.0;
+    step[i] = f;
+    cum[i] *= f;
+}
+
+// Elementwise helpers.
+__global__ void fill_f64(int64_t len, double* __restrict__ p, double v) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < len) p[i] = v;
+}
+
+// Per-element contribution for per-instance sums of squares:
+// mode 0: v[i]^2; mode 1: (max finite |lo|,|hi|)^2 (norm of the bound vector).
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void seg_sumsq(int len, int mode, const double* __restrict__ v,
+                                                    const double* __restrict__ lo,
+                                                    const double* __restrict__ hi,
+                                                    const int* __restrict__ inst_of,
+                                                    double* __restrict__ part) {
+    const int tile = blockIdx.x;
+    const int i = tile * kBlock + threadIdx.x;
+    double c[1] = {0.0};
+    if (i < len) {
+        if (mode == 0) {
+            c[0] = v[i] * v[i];
+        } else {
+            double b = 0.0;
+            if (fabs(lo[i]) < kInfBound) b = fabs(lo[i]);
+            if (fabs(hi[i]) < kInfBound && fabs(hi[i]) > b) b = fabs(hi[i]);
+            c[0] = b * b;
+        }
+    }
+    const int last = (tile + 1) * kBlock < len ? (tile + 1) * kBlock - 1 : len - 1;
+    const int f = MULTI ? inst_of[tile * kBlock] : 0;
+    const int l = MULTI ? inst_of[last] : 0;
+    const int my = (MULTI && i < len) ? inst_of[i] : 0;
+    write_tile_partials<1, MULTI>(c, my, f, l, tile, part, 1);
+}
+
+// out[inst] = sum of part column q over the instance's tiles (one block per instance).
+__global__ __launch_bounds__(kBlock) void seg_finish(const double* __restrict__ part, int NQ, int q,
+                                                     const int* __restrict__ off, int R,
+                                                     double* __restrict__ out) {
+    __shared__ double lds[kBlock];
+    const int inst = blockIdx.x;
+    const double s = reduce_instance(part, NQ, q, inst, off[inst], off[inst + 1], R, lds);
+    if (threadIdx.x == 0) out[inst] = s;
+}
+
+// ---------------------------------------------------------------------------------
+// Fused iteration kernels.
+//
+// Coordinates: the engine iterates on a correction (dx, dy) around an FP64 centre
+// (xhat, yhat) that is moved to the current fixed-point estimate at every restart, with
+// shifted data dhat = c - A^T yhat, [lc - xhat, uc - xhat], [lr - A xhat, ur - A xhat].
+// For FP32/BF16 this is FP64 iterative refinement; for FP64 it is exact re-centring.
+// The Halpern anchor is the centre, i.e. zero in correction coordinates.
+//
+// flags bit 0: first iteration of the block (partial q0 = |T(z)-z|^2 part)
+// flags bit 1: last iteration of the block (q1 = |T(z)-z|^2 part, q2 = |T(z)|^2 part,
+//              and T(z) + centre is written in FP64 for the check)
+
+template <class V, class T, int G, bool MULTI>
+__global__ __launch_bounds__(kBlock) void primal_step(
+    int n, const int* __restrict__ tp, const int* __restrict__ ti, const V* __restrict__ tv,
+    const T* __restrict__ y, const T* __restrict__ dhat, const T* __restrict__ lcs,
+    const T* __restrict__ ucs, T* __restrict__ x, T* __restrict__ xbar,
+    const double* __restrict__ xhat, double* __restrict__ xfull,
+    const InstState* __restrict__ st, const int* __restrict__ col_inst, double* __restrict__ part,
+    int it, int flags, double rho) {
+    constexpr int R = kBlock / G;
+    const int tile = blockIdx.x;
+    const int r0 = tile * R;
+    const int j = r0 + static_cast<int>(threadIdx.x) / G;
+    const int lane = threadIdx.x % G;
+    int f = 0, l = 0;
+    if (MULTI) {
+        f = col_inst[r0];
+        l = col_inst[(r0 + R < n ? r0 + R : n) - 1];
+    }
+    bool any = false;
+    for (int q = f; q <= l; ++q) any |= (st[q].active != 0);
+    if (!any) return;
+    const int my = (MULTI && j < n) ? col_inst[j] : 0;
+    const bool act = (j < n) && (!MULTI || st[my].active != 0);
+    double c[3] = {0.0, 0.0, 0.0};
+    T acc = 0;
+    if (act) {
+        const int p1 = tp[j + 1];
+        for (int p = tp[j] + lane; p < p1; p += G) acc += static_cast<T>(load_val(tv, p)) * y[ti[p]];
+    }
+    acc = group_sum<G>(acc);
+    if (act && lane == 0) {
+        const T tau = static_cast<T>(st[my].tau);
+        const double kk = static_cast<double>(st[my].k + it);
+        const T lam = static_cast<T>((kk + 1.0) / (kk + 2.0));
+        const T xk = x[j];
+        T xt = xk - tau * (dhat[j] - acc);
+        const T lo = lcs[j], hi = ucs[j];
+        xt = xt < lo ? lo : xt;
+        xt = xt > hi ? hi : xt;
+        xbar[j] = static_cast<T>(2) * xt - xk;
+        x[j] = lam * ((static_cast<T>(1) + static_cast<T>(rho)) * xt - static_cast<T>(rho) * xk);
+        const double d = static_cast<double>(xt) - static_cast<double>(xk);
+        if (flags & 1) c[0] = d * d;
+        if (flags & 2) {
+            c[1] = d * d;
+            c[2] = static_cast<double>(xt) * static_cast<double>(xt);
+            xfull[j] = xhat[j] + static_cast<double>(xt);
+        }
+    }
+    if (flags) {
+        const int mask = ((flags & 1) ? 1 : 0) | ((flags & 2) ? 6 : 0);
+        write_tile_partials<3, MULTI>(c, my, f, l, tile, part, mask);
+    }
+}
+
+template <class V, class T, int G, bool MULTI>
+__global__ __launch_bounds__(kBlock) void dual_step(
+    int m, const int* __restrict__ rp, const int* __restrict__ ci, const V* __restrict__ av,
+    const T* __restrict__ xbar, const T* __restrict__ yhat_t, const T* __restrict__ lrs,
+    const T* __restrict__ urs, T* __restrict__ y, const double* __restrict__ yhat,
+    double* __restrict__ yfull, const InstState* __restrict__ st, const int* __restrict__ row_inst,
+    double* __restrict__ part, int it, int flags, double rho) {
+    constexpr int R = kBlock / G;
+    const int tile = blockIdx.x;
+    const int r0 = tile * R;
+    const int i = r0 + static_cast<int>(threadIdx.x) / G;
+    const int lane = threadIdx.x % G;
+    int f = 0, l = 0;
+    if (MULTI) {
+        f = row_inst[r0];
+        l = row_inst[(r0 + R < m ? r0 + R : m) - 1];
+    }
+    bool any = false;
+    for (int q = f; q <= l; ++q) any |= (st[q].active != 0);
+    if (!any) return;
+    const int my = (MULTI && i < m) ? row_inst[i] : 0;
+    const bool act = (i < m) && (!MULTI || st[my].active != 0);
+    double c[3] = {0.0, 0.0, 0.0};
+    T w = 0;
+    if (act) {
+        const int p1 = rp[i + 1];
+        for (int p = rp[i] + lane; p < p1; p += G) w += static_cast<T>(load_val(av, p)) * xbar[ci[p]];
+    }
+    w = group_sum<G>(w);
+    if (act && lane == 0) {
+        const T sig = static_cast<T>(st[my].sigma);
+        const double kk = static_cast<double>(st[my].k + it);
+        const T lam = static_cast<T>((kk + 1.0) / (kk + 2.0));
+        const T yk = y[i];
+        const T yh = yhat_t[i];
+        const T lo = lrs[i], hi = urs[i];
+        const T sfull = yh + yk;
+        // prox of the row-bound support function, written per active side:
+        // y+ = y + sigma*(proj_[l,u](w - y/sigma) - w) in full coordinates
+        T yt;
+        if (sig * (w - lo) < sfull) yt = yk + sig * (lo - w);
+        else if (sig * (w - hi) > sfull) yt = yk + sig * (hi - w);
+        else yt = -yh;
+        y[i] = lam * ((static_cast<T>(1) + static_cast<T>(rho)) * yt - static_cast<T>(rho) * yk);
+        const double d = static_cast<double>(yt) - static_cast<double>(yk);
+        if (flags & 1) c[0] = d * d;
+        if (flags & 2) {
+            c[1] = d * d;
+            c[2] = static_cast<double>(yt) * static_cast<double>(yt);
+            yfull[i] = yhat[i] + static_cast<double>(yt);
+        }
+    }
+    if (flags) {
+        const int mask = ((flags & 1) ? 1 : 0) | ((flags & 2) ? 6 : 0);
+        write_tile_partials<3, MULTI>(c, my, f, l, tile, part, mask);
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// FP64 KKT passes (the check, and the audit on original data when the unscale arrays
+// are null). Row pass: ax = A x; q0 = sum r_p^2 (original units), q1 = row part of the
+// dual objective, q2 = row-dual sign violations^2 (original units).
+template <int G, bool MULTI>
+__global__ __launch_bounds__(kBlock) void kkt_rows(
+    int m, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ av,
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ lr,
+    const double* __restrict__ ur, const double* __restrict__ ir, const double* __restrict__ py,
+    double* __restrict__ ax_out, const InstState* __restrict__ st, int only_active,
+    const int* __restrict__ row_inst, double* __restrict__ part) {
+    constexpr int R = kBlock / G;
+    const int tile = blockIdx.x;
+    const int r0 = tile * R;
+    const int i = r0 + static_cast<int>(threadIdx.x) / G;
+    const int lane = threadIdx.x % G;
+    int f = 0, l = 0;
+    if (MULTI) {
+        f = row_inst[r0];
+        l = row_inst[(r0 + R < m ? r0 + R : m) - 1];
+    }
+    if (only_active) {
+        bool any = false;
+        for (int q = f; q <= l; ++q) any |= (st[q].active != 0);
+        if (!any) return;
+    }
+    const int my = (MULTI && i < m) ? row_inst[i] : 0;
+    const bool act = (i < m) && (!only_active || st[my].active != 0);
+    double acc = 0.0;
+    if (act)
+        for (int p = rp[i] + lane; p < rp[i + 1]; p += G) acc += av[p] * x[ci[p]];
+    acc = group_sum<G>(acc);
+    double c[3] = {0.0, 0.0, 0.0};
+    if (act && lane == 0) {
+        if (ax_out) ax_out[i] = acc;
+        const double lo = lr[i], hi = ur[i];
+        const bool lf = lo > -kInfBound, uf = hi < kInfBound;
+        double r = 0.0;
+        if (lf && acc < lo) r = acc - lo;
+        if (uf && acc > hi) r = acc - hi;
+        if (ir) r *= ir[i];
+        c[0] = r * r;
+        const double yi = y[i];
+        double sv = 0.0;
+        if (yi > 0.0) {
+            if (lf) c[1] = yi * lo; else sv = yi;
+        } else if (yi < 0.0) {
+            if (uf) c[1] = yi * hi; else sv = yi;
+        }
+        if (py) sv *= py[i];
+        c[2] = sv * sv;
+    }
+    write_tile_partials<3, MULTI>(c, my, f, l, tile, part, 7);
+}
+
+// Column pass: aty = A^T y; d = c - aty; lambda = projection of d onto the reduced-cost
+// cone of the bounds; q0 = |d - lambda|^2 (original units), q1 = lambda . bounds,
+// q2 = c . x.
+template <int G, bool MULTI>
+__global__ __launch_bounds__(kBlock) void kkt_cols(
+    int n, const int* __restrict__ tp, const int* __restrict__ ti, const double* __restrict__ tv,
+    const double* __restrict__ y, const double* __restrict__ x, const double* __restrict__ c,
+    const double* __restrict__ lc, const double* __restrict__ uc, const double* __restrict__ ic,
+    double* __restrict__ aty_out, const InstState* __restrict__ st, int only_active,
+    const int* __restrict__ col_inst, double* __restrict__ part) {
+    constexpr int R = kBlock / G;
+    const int tile = blockIdx.x;
+    const int r0 = tile * R;
+    const int j = r0 + static_cast<int>(threadIdx.x) / G;
+    const int lane = threadIdx.x % G;
+    int f = 0, l = 0;
+    if (MULTI) {
+        f = col_inst[r0];
+        l = col_inst[(r0 + R < n ? r0 + R : n) - 1];
+    }
+    if (only_active) {
+        bool any = false;
+        for (int q = f; q <= l; ++q) any |= (st[q].active != 0);
+        if (!any) return;
+    }
+    const int my = (MULTI && j < n) ? col_inst[j] : 0;
+    const bool act = (j < n) && (!only_active || st[my].active != 0);
+    double acc = 0.0;
+    if (act)
+        for (int p = tp[j] + lane; p < tp[j + 1]; p += G) acc += tv[p] * y[ti[p]];
+    acc = group_sum<G>(acc);
+    double cc[3] = {0.0, 0.0, 0.0};
+    if (act && lane == 0) {
+        if (aty_out) aty_out[j] = acc;
+        const double d = c[j] - acc;
+        const double lo = lc[j], hi = uc[j];
+        double lam = 0.0;
+        if (d > 0.0 && lo > -kInfBound) lam = d;
+        else if (d < 0.0 && hi < kInfBound) lam = d;
+        double rd = d - lam;
+        if (ic) rd *= ic[j];
+        cc[0] = rd * rd;
+        if (lam > 0.0) cc[1] = lam * lo;
+        else if (lam < 0.0) cc[1] = lam * hi;
+        cc[2] = c[j] * x[j];
+    }
+    write_tile_partials<3, MULTI>(cc, my, f, l, tile, part, 7);
+}
+
+// Per-instance rel-KKT from the two passes (audit: factor = 1, no state update).
+// out8 per instance: rel_kkt, rel_p, rel_d, rel_gap, pobj, dobj, |r_p|, |r_d|.
+__global__ __launch_bounds__(kBlock) void kkt_finish(
+    const double* __restrict__ prow, int Rr, const double* __restrict__ pcol, int Rc,
+    const int* __restrict__ row_off, const int* __restrict__ col_off,
+    const double* __restrict__ norm_b, const double* __restrict__ norm_c,
+    const double* __restrict__ offset, const double* __restrict__ objfac, double* __restrict__ out8) {
+    __shared__ double lds[kBlock];
+    const int inst = blockIdx.x;
+    const int64_t r_a = row_off[inst], r_b = row_off[inst + 1];
+    const int64_t c_a = col_off[inst], c_b = col_off[inst + 1];
+    const double rp2 = reduce_instance(prow, 3, 0, inst, r_a, r_b, Rr, lds);
+    const double dro = reduce_instance(prow, 3, 1, inst, r_a, r_b, Rr, lds);
+    const double rsv = reduce_instance(prow, 3, 2, inst, r_a, r_b, Rr, lds);
+    const double rdc = reduce_instance(pcol, 3, 0, inst, c_a, c_b, Rc, lds);
+    const double dco = reduce_instance(pcol, 3, 1, inst, c_a, c_b, Rc, lds);
+    const double po = reduce_instance(pcol, 3, 2, inst, c_a, c_b, Rc, lds);
+    if (threadIdx.x == 0) {
+        const double fac = objfac ? objfac[inst] : 1.0;
+        const double pobj = offset[inst] + po * fac;
+        const double dobj = offset[inst] + (dro + dco) * fac;
+        const double nrp = sqrt(rp2), nrd = sqrt(rdc + rsv);
+        const double rel_p = nrp / (1.0 + norm_b[inst]);
+        const double rel_d = nrd / (1.0 + norm_c[inst]);
+        const double rel_g = fabs(pobj - dobj) / (1.0 + fabs(pobj) + fabs(dobj));
+        double k = rel_p > rel_d ? rel_p : rel_d;
+        k = k > rel_g ? k : rel_g;
+        if (isnan(rel_p) || isnan(rel_d) || isnan(rel_g)) k = NAN;
+        double* o = out8 + 8 * inst;
+        o[0] = k; o[1] = rel_p; o[2] = rel_d; o[3] = rel_g;
+        o[4] = pobj; o[5] = dobj; o[6] = nrp; o[7] = nrd;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Control: one workgroup per instance, run after the check passes of every block.
+// Restart rule (reflected restarted Halpern PDHG, Lu & Yang): with r the weighted
+// fixed-point residual sqrt(omega |dx|^2 + |dy|^2 / omega), restart when
+// r <= 0.2 r0, or r <= 0.8 r0 and r > r_prev, or the epoch exceeds 0.36 of all
+// iterations. At a restart the primal weight is smoothed as in PDLP:
+// log omega <- 0.5 log(|Dy|/|Dx|) + 0.5 log omega.
+struct ControlArgs {
+    const double* px; int Rx;      // primal-step partials (cols)
+    const double* py; int Ry;      // dual-step partials (rows)
+    const double* kkt;             // out8 per instance from kkt_finish
+    const int* row_off;
+    const int* col_off;
+    int K;
+    double restart_sufficient, restart_necessary, restart_artificial;
+    double* hist;                  // per instance: [checks] rel_kkt history (may be null)
+    int hist_cap;
+};
+
+__global__ __launch_bounds__(kBlock) void control_kernel(InstState* __restrict__ st, ControlArgs a) {
+    __shared__ double lds[kBlock];
+    const int inst = blockIdx.x;
+    if (st[inst].active == 0) return;
+    const int64_t r_a = a.row_off[inst], r_b = a.row_off[inst + 1];
+    const int64_t c_a = a.col_off[inst], c_b = a.col_off[inst + 1];
+    const double xq0 = reduce_instance(a.px, 3, 0, inst, c_a, c_b, a.Rx, lds);
+    const double xq1 = reduce_instance(a.px, 3, 1, inst, c_a, c_b, a.Rx, lds);
+    const double xq2 = reduce_instance(a.px, 3, 2, inst, c_a, c_b, a.Rx, lds);
+    const double yq0 = reduce_instance(a.py, 3, 0, inst, r_a, r_b, a.Ry, lds);
+    const double yq1 = reduce_instance(a.py, 3, 1, inst, r_a, r_b, a.Ry, lds);
+    const double yq2 = reduce_instance(a.py, 3, 2, inst, r_a, r_b, a.Ry, lds);
+    if (threadIdx.x != 0) return;
+    InstState s = st[inst];
+    const double* k8 = a.kkt + 8 * inst;
+    s.iters += a.K;
+    s.k += a.K;
+    s.rel_kkt = k8[0]; s.rel_p = k8[1]; s.rel_d = k8[2]; s.rel_gap = k8[3];
+    s.pobj = k8[4]; s.dobj = k8[5];
+    if (a.hist && s.checks < a.hist_cap) a.hist[static_cast<int64_t>(inst) * a.hist_cap + s.checks] = k8[0];
+    s.checks += 1;
+    s.restart_now = 0;
+    const double r = sqrt(s.omega * xq1 + yq1 / s.omega);
+    const double rf = sqrt(s.omega * xq0 + yq0 / s.omega);
+    if (isnan(k8[0]) || isnan(r) || isnan(rf) || isinf(r)) {
+        s.nan_flag = 1;
+        s.active = 0;
+    } else if (k8[0] <= s.tol) {
+        s.converged = 1;
+        s.active = 0;
+    } else if (s.iters >= s.max_iters) {
+        s.active = 0;
+    } else {
+        if (s.restarted_prev) {
+            s.r0 = rf;
+            s.r_prev = rf;
+        }
+        const bool restart = (r <= a.restart_sufficient * s.r0) ||
+                             (r <= a.restart_necessary * s.r0 && r > s.r_prev) ||
+                             (static_cast<double>(s.k) >= a.restart_artificial * static_cast<double>(s.iters));
+        s.r_prev = r;
+        s.restarted_prev = 0;
+        if (restart) {
+            s.restart_now = 1;
+            s.restarted_prev = 1;
+            s.restarts += 1;
+            s.k = 0;
+            const double dx = sqrt(xq2), dy = sqrt(yq2);
+            if (dx > 1e-10 && dy > 1e-10) {
+                s.omega = exp(0.5 * log(dy / dx) + 0.5 * log(s.omega));
+                s.tau = s.eta / s.omega;
+                s.sigma = s.eta * s.omega;
+            }
+        }
+    }
+    st[inst] = s;
+}
+
+// Restart (or initial centring, force = 1): move the centre to the last T(z) and
+// rebuild the shifted data in the working precision.
+template <class T, bool MULTI>
+__global__ void restart_cols(int n, const InstState* __restrict__ st, const int* __restrict__ col_inst,
+                             int force, const double* __restrict__ xfull,
+                             const double* __restrict__ aty_full, const double* __restrict__ c_s,
+                             const double* __restrict__ lc_s, const double* __restrict__ uc_s,
+                             double* __restrict__ xhat, T* __restrict__ x, T* __restrict__ dhat,
+                             T* __restrict__ lcs, T* __restrict__ ucs) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int q = MULTI ? col_inst[j] : 0;
+    if (!force && st[q].restart_now == 0) return;
+    const double xf = xfull[j];
+    xhat[j] = xf;
+    x[j] = static_cast<T>(0);
+    dhat[j] = static_cast<T>(c_s[j] - aty_full[j]);
+    lcs[j] = static_cast<T>(lc_s[j] - xf);
+    ucs[j] = static_cast<T>(uc_s[j] - xf);
+}
+
+template <class T, bool MULTI>
+__global__ void restart_rows(int m, const InstState* __restrict__ st, const int* __restrict__ row_inst,
+                             int force, const double* __restrict__ yfull,
+                             const double* __restrict__ ax_full, const double* __restrict__ lr_s,
+                             const double* __restrict__ ur_s, double* __restrict__ yhat,
+                             T* __restrict__ yhat_t, T* __restrict__ y, T* __restrict__ lrs,
+                             T* __restrict__ urs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int q = MULTI ? row_inst[i] : 0;
+    if (!force && st[q].restart_now == 0) return;
+    const double yf = yfull[i];
+    yhat[i] = yf;
+    if (static_cast<void*>(yhat_t) != static_cast<void*>(yhat)) yhat_t[i] = static_cast<T>(yf);
+    y[i] = static_cast<T>(0);
+    lrs[i] = static_cast<T>(lr_s[i] - ax_full[i]);
+    urs[i] = static_cast<T>(ur_s[i] - ax_full[i]);
+}
+
+}  // namespace vj
