@@ -1,0 +1,247 @@
+#!/usr/bin/env python3
+"""Benchmark of the restarted-PDHG LP engine on one MI355X.
+
+Metric (one JSON line on stdout): time to relative KKT 1e-6 on the large random LP
+(m = 1,000,000 equality rows, n = 2,000,000 columns, 12M nonzeros, 6 per column, 10% of the
+columns scaled by 10^U[-3,3]), FP32 iterates/SpMV with FP64 refinement ("value"), beside the
+plain FP64 engine on the same LP, the HBM bandwidth of the dominant SpMV kernel against the
+8 TB/s peak, batched LPs/s (512 uneven random LPs; 128 block-staircase LPs) against
+sequential single solves, and the reference CPU PDHG timed on this host.
+
+A "step" is one complete solve of the workload to rel-KKT 1e-6 with the problem already
+resident in HBM (device setup - scaling, power iteration - included; host generation and
+the PCIe upload excluded and reported separately).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in ("vajra-opt_amd", "oracle"):
+    sys.path.insert(0, os.path.join(ROOT, p))
+
+HBM_PEAK_GBS = 8000.0
+CFG1 = dict(seed=0x5EED2026 + 1, m=1_000_000, n=2_000_000, k=6, frac=0.1, rng=3.0)
+CFG3 = dict(seed=0x5EED2026 + 3, count=512, lo=500, hi=20000)
+CFG4 = dict(seed=0x5EED2026 + 4, count=128, lo=12, hi=96)
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def solve_timed(vp, eng, prec, tol, max_iters, time_limit):
+    p = vp.default_params(vp.MODE_RESTARTED, precision=prec, tolerance=tol, max_iterations=max_iters,
+                          time_limit=time_limit)
+    t0 = time.perf_counter()
+    r = eng.solve(p)
+    wall = time.perf_counter() - t0
+    return r, wall
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--max-iters", type=int, default=400000)
+    ap.add_argument("--time-limit", type=float, default=150.0)
+    ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=CFG3["count"])
+    ap.add_argument("--stair-count", type=int, default=CFG4["count"])
+    ap.add_argument("--seq-sample", type=int, default=8)
+    ap.add_argument("--cpu-iters", type=int, default=60)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp64"])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design note "
+                                   "(row-block A / column-block A^T with x/y all-gathers), not built"}))
+        sys.exit(2)
+
+    import vajra_pdhg as vp
+
+    info = vp.device_info()
+    if not info.available:
+        raise SystemExit("no HIP device")
+    log(f"device {info.name.decode()} {info.arch.decode()} CUs={info.compute_units}")
+
+    # ---- workload: large random LP -------------------------------------------------------
+    t0 = time.perf_counter()
+    lp = vp.gen_random_lp(CFG1["seed"], CFG1["m"], CFG1["n"], CFG1["k"], CFG1["frac"], CFG1["rng"])
+    lp.csr()
+    t_gen = time.perf_counter() - t0
+    log(f"generated {lp.m}x{lp.n} nnz={lp.nnz} in {t_gen:.1f}s")
+    eng = vp.Engine(0)
+    t0 = time.perf_counter()
+    eng.upload(lp)
+    t_up = time.perf_counter() - t0
+    log(f"uploaded in {t_up:.2f}s")
+
+    extra = {}
+    # ---- plain FP64 engine (the baseline the mixed path is measured against) -----------
+    r64 = None
+    if not args.skip_fp64:
+        r64, wall64 = solve_timed(vp, eng, "fp64", args.tol, args.max_iters, args.time_limit)
+        log(f"fp64: {vp.STATUS[r64.status]} iters={r64.iterations} restarts={r64.restart_count} "
+            f"rel_kkt={r64.rel_kkt:.3e} setup={r64.setup_time:.3f}s iter={r64.iteration_time:.3f}s wall={wall64:.3f}s")
+        k64 = eng.benchmark_kernels(reps=30)
+        extra["fp64"] = dict(status=vp.STATUS[r64.status], iterations=r64.iterations, restarts=r64.restart_count,
+                             rel_kkt=r64.rel_kkt, objective=r64.primal_objective, time_to_target_s=r64.time_to_target,
+                             setup_s=r64.setup_time, iteration_s=r64.iteration_time, solve_wall_s=wall64,
+                             lanes=[r64.lanes_A, r64.lanes_At], kernels=k64)
+
+    # ---- headline: FP32 iterates + FP64 refinement ----------------------------------------
+    for w in range(args.warmup):
+        rw, ww = solve_timed(vp, eng, args.precision, args.tol, args.max_iters, args.time_limit)
+        log(f"warmup {w}: {vp.STATUS[rw.status]} iters={rw.iterations} wall={ww:.3f}s")
+    walls, res = [], None
+    for s in range(args.steps):
+        res, wall = solve_timed(vp, eng, args.precision, args.tol, args.max_iters, args.time_limit)
+        walls.append(wall)
+        log(f"step {s}: {args.precision} {vp.STATUS[res.status]} iters={res.iterations} restarts={res.restart_count} "
+            f"rel_kkt={res.rel_kkt:.3e} setup={res.setup_time:.3f}s iter={res.iteration_time:.3f}s wall={wall:.3f}s")
+    kmix = eng.benchmark_kernels(reps=30)
+    x, rd, y = eng.download()
+    # rel-KKT of the returned point, recomputed by the oracle on the host (checker only)
+    oracle_kkt = None
+    try:
+        import oracle as O
+        oracle_kkt = float(O.rel_kkt(lp, x, y)[0])
+    except Exception as e:  # oracle not built: report the device audit only
+        log(f"oracle unavailable: {e}")
+    value = float(np.mean(walls))
+    extra["mixed"] = dict(precision=args.precision, status=vp.STATUS[res.status], iterations=res.iterations,
+                          restarts=res.restart_count, rel_kkt_device=res.rel_kkt, rel_kkt_oracle=oracle_kkt,
+                          objective=res.primal_objective, time_to_target_s=res.time_to_target,
+                          setup_s=res.setup_time, iteration_s=res.iteration_time, lanes=[res.lanes_A, res.lanes_At],
+                          kernels=kmix)
+    if r64 is not None and r64.status == 1 and res.status == 1:
+        extra["speedup_mixed_vs_fp64"] = extra["fp64"]["solve_wall_s"] / value
+        extra["objective_rel_diff_mixed_vs_fp64"] = abs(res.primal_objective - r64.primal_objective) / (
+            1 + abs(r64.primal_objective))
+
+    # ---- roofline of the dominant kernel (HIP events on the engine stream) ----------------
+    ks = extra["fp64"]["kernels"] if "fp64" in extra else kmix
+    dom = max(ks, key=lambda k: k["avg_us"])
+    achieved = dom["bytes_per_launch"] / (dom["avg_us"] * 1e-6) / 1e9
+    roofline = dict(kernel=dom["name"], bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
+                    unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    avg_us=round(dom["avg_us"], 2), bytes_per_launch=dom["bytes_per_launch"],
+                    precision="fp64" if "fp64" in extra else args.precision)
+    spmv_gbs = {f"{k['name']}_{tag}": round(k["bytes_per_launch"] / (k["avg_us"] * 1e-6) / 1e9, 1)
+                for tag, kk in (("fp64", extra.get("fp64", {}).get("kernels", [])), (args.precision, kmix))
+                for k in kk}
+    eng.close()
+
+    # ---- batched LPs/s ---------------------------------------------------------------------
+    if not args.skip_batch:
+        extra["batch_random"] = bench_batch(vp, "random", args)
+        extra["batch_staircase"] = bench_batch(vp, "staircase", args)
+
+    # ---- CPU baseline: the reference CPU PDHG map (oracle) on this host, bounded sample ----
+    cpu = None
+    if not args.skip_cpu:
+        cpu = cpu_baseline(lp, args, r64)
+
+    out = {
+        "metric": "time to rel-KKT 1e-6 + SpMV GB/s: mixed vs FP64 PDHG; batched LPs/sec",
+        "value": round(value, 4),
+        "unit": "s (FP32+FP64-refined restarted PDHG to rel-KKT 1e-6, 1M x 2M LP, resident in HBM)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000 * value, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if args.precision != "fp64" else "f64",
+        "data": "synthetic (seeded generator, include/vajra_lpgen.h)",
+        "config": {"workload": "random equality LP m=1,000,000 n=2,000,000 nnz=12,000,000 (6/col N(0,1), "
+                               "10% cols x10^U[-3,3]), 0<=x<=10, b=A x0, c=A^T y0+s0; stop at rel-KKT 1e-6",
+                   "precision": args.precision, "host_gen_s": round(t_gen, 2), "upload_s": round(t_up, 3)},
+        "roofline": roofline,
+        "spmv_gbs": spmv_gbs,
+        "cpu_baseline": cpu,
+        "detail": extra,
+    }
+    print(json.dumps(out, default=float))
+
+
+def bench_batch(vp, kind, args):
+    if kind == "random":
+        cnt = args.batch_count
+        ms = vp.gen_uniform_ints(CFG3["seed"], cnt, CFG3["lo"], CFG3["hi"])
+        lps = [vp.gen_random_lp(CFG3["seed"] * 1000 + i, int(m), int(2.5 * m), 6) for i, m in enumerate(ms)]
+    else:
+        cnt = args.stair_count
+        ts = vp.gen_uniform_ints(CFG4["seed"], cnt, CFG4["lo"], CFG4["hi"])
+        lps = [vp.gen_staircase_lp(CFG4["seed"] * 1000 + i, int(t)) for i, t in enumerate(ts)]
+    for lp in lps:
+        lp.csr()
+    nnz = sum(lp.nnz for lp in lps)
+    log(f"batch {kind}: {cnt} LPs, sum nnz={nnz}")
+    eng = vp.Engine(0)
+    eng.upload_batch(lps)
+    p = vp.default_params(vp.MODE_RESTARTED, precision=args.precision, tolerance=args.tol,
+                          max_iterations=args.max_iters, time_limit=args.time_limit)
+    t0 = time.perf_counter()
+    res = eng.solve_batch(p)
+    t_batch = time.perf_counter() - t0
+    ok = sum(1 for r in res if r.status == 1)
+    worst = max(r.rel_kkt for r in res)
+    log(f"batch {kind}: {ok}/{cnt} optimal in {t_batch:.2f}s (max iters {max(r.iterations for r in res)}), worst rel_kkt {worst:.2e}")
+    eng.close()
+    # sequential plain FP64 single solves on a sample, same tolerance
+    sample = lps[:: max(1, cnt // args.seq_sample)][: args.seq_sample]
+    seq_t = 0.0
+    seq_ok = 0
+    for lp in sample:
+        e = vp.Engine(0)
+        e.upload(lp)
+        t0 = time.perf_counter()
+        r = e.solve(vp.default_params(vp.MODE_RESTARTED, precision="fp64", tolerance=args.tol,
+                                      max_iterations=args.max_iters, time_limit=args.time_limit))
+        seq_t += time.perf_counter() - t0
+        seq_ok += r.status == 1
+        e.close()
+    seq_lps = len(sample) / seq_t
+    batch_lps = cnt / t_batch
+    log(f"batch {kind}: {batch_lps:.1f} LPs/s batched vs {seq_lps:.1f} LPs/s sequential fp64 ({seq_ok}/{len(sample)} optimal)")
+    return dict(count=cnt, sum_nnz=nnz, optimal=ok, worst_rel_kkt=worst, batch_s=t_batch, batch_lps_per_s=batch_lps,
+                sequential_sample=len(sample), sequential_optimal=seq_ok, sequential_lps_per_s=seq_lps,
+                speedup=batch_lps / seq_lps, precision=args.precision)
+
+
+def cpu_baseline(lp, args, r64):
+    try:
+        import oracle as O
+    except Exception as e:
+        log(f"cpu baseline skipped: {e}")
+        return None
+    iters = args.cpu_iters
+    o = O.default_opts(max_iterations=10**9, stop_at_iter=iters, check_frequency=100)
+    t0 = time.perf_counter()
+    r = O.pdhg_solve(lp, o)
+    wall = time.perf_counter() - t0
+    per_it = r["iteration_time"] / iters
+    gpu_iters = r64.iterations if r64 is not None else None
+    proj = per_it * gpu_iters + r["setup_time"] if gpu_iters else None
+    log(f"cpu: setup {r['setup_time']:.2f}s, {iters} iterations {r['iteration_time']:.2f}s ({per_it*1e3:.1f} ms/it)")
+    return {"value": round(proj, 2) if proj else None, "unit": "s",
+            "cores": 1, "kind": "port",
+            "sample": f"reference CPU PDHG map (oracle, 1 thread) on the same 1M x 2M LP: setup (Ruiz + power "
+                      f"iteration) + {iters} iterations, {wall:.1f}s wall; value = setup + per-iteration time x the "
+                      f"FP64 GPU engine's iteration count ({gpu_iters}); the reference iteration map itself needs "
+                      f"more iterations (it did not reach rel-KKT 1e-6 within 1e6 iterations on the 2,000 x 5,000 LP)",
+            "setup_s": r["setup_time"], "s_per_iteration": per_it, "host_nproc": os.cpu_count()}
+
+
+if __name__ == "__main__":
+    main()

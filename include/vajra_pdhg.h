@@ -103,6 +103,9 @@ typedef struct vj_params {
     int lanes_At;                  /* lanes per row of A^T (1..64, power of 2; 0 = autotune) */
     int refine_rounds;             /* FP32/BF16: max FP64 refinement rounds (0 = 64) */
     int verbose;
+    int col_splits_A;              /* column blocks of A for the XCD-split SpMV: 0 = by vector
+                                      size vs L2, 1 = fused single pass, 2/4/8 = forced */
+    int col_splits_At;             /* same for A^T (blocks of rows of A) */
 } vj_params;
 /* Fill defaults for a mode (reference defaults for VJ_MODE_REFERENCE). */
 void vj_params_default(vj_params* p, int mode);
@@ -124,6 +127,7 @@ typedef struct vj_result {
     int refinements;               /* FP64 refinement rounds used (mixed precision) */
     int lanes_A, lanes_At, check_interval;
     int64_t checks;
+    int col_splits_A, col_splits_At;   /* column blocks used by the XCD-split SpMV (1 = fused) */
 } vj_result;
 
 /* ---- single-LP context (replaces solve_pdhg_cuda_internal, pdhg_cuda.cu:109-375) */

@@ -83,3 +83,17 @@ def test_batch_matches_single(engine):
         k = O.rel_kkt(lp, x, y)
         assert k[0] <= 1e-6
         assert abs(k[0] - r.rel_kkt) <= 1e-9 + 1e-6 * k[0]
+
+
+@pytest.mark.parametrize("prec,sa,sat", [("fp64", 8, 4), ("fp32", 4, 2), ("bf16", 2, 8)])
+def test_column_split_spmv(engine, prec, sa, sat):
+    """The XCD column-split SpMV (block partial sums + ordered finish) solves to the same
+    tolerance as the fused kernels, deterministically (two runs bit-identical)."""
+    lp = lp_cases.cfg0_lp()
+    print(f"[case] split {prec} SA={sa} SAt={sat}", flush=True)
+    r1, x1, _, y1 = solve(engine, lp, precision=prec, col_splits_A=sa, col_splits_At=sat)
+    assert (r1.col_splits_A, r1.col_splits_At) == (sa, sat)
+    assert r1.status == 1, r1.as_dict()
+    check_point(lp, r1, x1, y1, 1e-6)
+    r2, x2, _, y2 = solve(engine, lp, precision=prec, col_splits_A=sa, col_splits_At=sat)
+    assert r1.iterations == r2.iterations and np.array_equal(x1, x2) and np.array_equal(y1, y2)

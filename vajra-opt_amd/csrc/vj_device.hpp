@@ -14,11 +14,15 @@ constexpr int kWave = 64;
 // BF16 matrix values are stored as raw 16-bit patterns (upper half of an FP32).
 struct bf16_t { uint16_t bits; };
 
-__device__ __forceinline__ double load_val(const double* p, int64_t k) { return p[k]; }
-__device__ __forceinline__ float load_val(const float* p, int64_t k) { return p[k]; }
+// Matrix streams are read once per iteration: non-temporal loads keep them from evicting
+// the gathered iterate vectors from L2 and the Infinity Cache.
+__device__ __forceinline__ double load_val(const double* p, int64_t k) { return __builtin_nontemporal_load(p + k); }
+__device__ __forceinline__ float load_val(const float* p, int64_t k) { return __builtin_nontemporal_load(p + k); }
 __device__ __forceinline__ float load_val(const bf16_t* p, int64_t k) {
-    return __uint_as_float(static_cast<uint32_t>(p[k].bits) << 16);
+    const uint16_t b = __builtin_nontemporal_load(&p[k].bits);
+    return __uint_as_float(static_cast<uint32_t>(b) << 16);
 }
+__device__ __forceinline__ int load_idx(const int* p, int64_t k) { return __builtin_nontemporal_load(p + k); }
 
 // Sum over the G consecutive lanes of a group (G a power of two, G <= 64). Fixed
 // butterfly order, so the result is deterministic; every lane receives the sum.
@@ -107,6 +111,23 @@ __device__ __forceinline__ double reduce_instance(const double* __restrict__ par
         for (int t = t0 + static_cast<int>(threadIdx.x); t <= t1; t += kBlock)
             s += part[static_cast<int64_t>(t + inst) * NQ + q];
     }
+    lds256[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) lds256[threadIdx.x] += lds256[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double r = lds256[0];
+    __syncthreads();
+    return r;
+}
+
+// Same over an explicit tile range [t0, t1] of the instance (variable-height tiles).
+__device__ __forceinline__ double reduce_tiles(const double* __restrict__ part, int NQ, int q, int inst, int t0,
+                                               int t1, double* lds256) {
+    double s = 0.0;
+    for (int t = t0 + static_cast<int>(threadIdx.x); t <= t1; t += kBlock)
+        s += part[static_cast<int64_t>(t + inst) * NQ + q];
     lds256[threadIdx.x] = s;
     __syncthreads();
     for (int w = kBlock / 2; w > 0; w >>= 1) {

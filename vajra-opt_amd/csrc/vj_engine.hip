@@ -29,6 +29,7 @@ double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration
 
 #define VJ_DISPATCH_G(G_, ...)                         \
     switch (G_) {                                      \
+        case 1: { constexpr int GG = 1; __VA_ARGS__; } break;   \
         case 2: { constexpr int GG = 2; __VA_ARGS__; } break;   \
         case 4: { constexpr int GG = 4; __VA_ARGS__; } break;   \
         case 8: { constexpr int GG = 8; __VA_ARGS__; } break;   \
@@ -160,6 +161,16 @@ struct FastEngine {
     DevBuf part_x, part_y, part_kr, part_kc, part_s;
     DevBuf kkt8, st, offset_d, norm_b_d, norm_c_d, objfac_d, inst_tmp_a, inst_tmp_b, hist;
     DevBuf vtmp_n, wtmp_m, utmp_n;
+    // XCD-split layouts (single LP): A by column blocks (gathers x_bar), A^T by row blocks
+    // of A (gathers y); S = 1 means the fused single-pass kernels are used
+    int SA = 1, SAt = 1, GsA = 2, GsAt = 2;
+    DevBuf bpA, idxA, valA, psumA, bpAt, idxAt, valAt, psumAt, permA, permAt;
+    int builtSA = 0, builtSAt = 0;   // split layout currently held in bp/idx/perm
+    // tile-streamed kernels: host-built row tiles of A (y side) and A^T (x side) and, per
+    // instance, the first/last tile touching it
+    bool streamA = true, streamAt = true, tiles_built = false;
+    int ntA = 0, ntAt = 0;
+    DevBuf tileA, tileAt, tfA, tlA, tfAt, tlAt;
     int hist_cap = 0;
     hipGraphExec_t gexec = nullptr;
     InstState* h_state[2] = {nullptr, nullptr};
@@ -189,6 +200,32 @@ int launch_primal(vj_ctx* ctx, FastEngine& E, const V* tv, T* x, T* xbar, const 
     const int tiles = tiles_for(E.n, E.GAt);
     VJ_TRY(require({P.tp.p, P.ti.p, tv, y, E.dhat.p, E.lcs.p, E.ucs.p, x, xbar, E.xhat.p, E.xfull.p, E.st.p,
                     E.part_x.p, E.multi ? P.col_inst.p : E.st.p}, "primal_step"));
+    if (E.SAt > 1) {
+        VJ_TRY(require({E.bpAt.p, E.idxAt.p, E.valAt.p, E.psumAt.p}, "split primal_step"));
+        VJ_DISPATCH_G(E.GsAt, hipLaunchKernelGGL((split_partials<V, T, GG>), dim3(E.SAt * tiles_for(E.n, GG)), dim3(kBlock),
+            0, ctx->stream, E.n, E.SAt, E.bpAt.as<int>(), E.idxAt.as<int>(), E.valAt.as<V>(), y, E.psumAt.as<T>(),
+            E.st.as<InstState>()));
+        hipLaunchKernelGGL((primal_finish<T>), dim3(grid_for(E.n, kBlock)), dim3(kBlock), 0, ctx->stream, E.n, E.SAt,
+                           E.psumAt.as<T>(), E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), x, xbar, E.xhat.as<double>(),
+                           E.xfull.as<double>(), E.st.as<InstState>(), E.part_x.as<double>(), it, flags, E.rho);
+        VJ_LAUNCHED();
+        return VJ_OK;
+    }
+    if (E.streamAt) {
+        VJ_TRY(require({E.tileAt.p}, "stream primal_step"));
+        if (E.multi)
+            hipLaunchKernelGGL((stream_primal_step<V, T, true>), dim3(E.ntAt), dim3(kBlock), 0, ctx->stream, E.n,
+                               E.tileAt.as<int>(), P.tp.as<int>(), P.ti.as<int>(), tv, y, E.dhat.as<T>(), E.lcs.as<T>(),
+                               E.ucs.as<T>(), x, xbar, E.xhat.as<double>(), E.xfull.as<double>(), E.st.as<InstState>(),
+                               P.col_inst.as<int>(), E.part_x.as<double>(), it, flags, E.rho);
+        else
+            hipLaunchKernelGGL((stream_primal_step<V, T, false>), dim3(E.ntAt), dim3(kBlock), 0, ctx->stream, E.n,
+                               E.tileAt.as<int>(), P.tp.as<int>(), P.ti.as<int>(), tv, y, E.dhat.as<T>(), E.lcs.as<T>(),
+                               E.ucs.as<T>(), x, xbar, E.xhat.as<double>(), E.xfull.as<double>(), E.st.as<InstState>(),
+                               P.col_inst.as<int>(), E.part_x.as<double>(), it, flags, E.rho);
+        VJ_LAUNCHED();
+        return VJ_OK;
+    }
     if (E.multi) {
         VJ_DISPATCH_G(E.GAt, hipLaunchKernelGGL((primal_step<V, T, GG, true>), dim3(tiles), dim3(kBlock), 0, ctx->stream,
             E.n, P.tp.as<int>(), P.ti.as<int>(), tv, y, E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), x, xbar,
@@ -227,6 +264,32 @@ int launch_dual(vj_ctx* ctx, FastEngine& E, const V* av, const T* xbar, T* y, in
     const T* yht = yhat_working<T>(E);
     VJ_TRY(require({P.rp.p, P.ci.p, av, xbar, yht, E.lrs.p, E.urs.p, y, E.yhat.p, E.yfull.p, E.st.p, E.part_y.p,
                     E.multi ? P.row_inst.p : E.st.p}, "dual_step"));
+    if (E.SA > 1) {
+        VJ_TRY(require({E.bpA.p, E.idxA.p, E.valA.p, E.psumA.p}, "split dual_step"));
+        VJ_DISPATCH_G(E.GsA, hipLaunchKernelGGL((split_partials<V, T, GG>), dim3(E.SA * tiles_for(E.m, GG)), dim3(kBlock),
+            0, ctx->stream, E.m, E.SA, E.bpA.as<int>(), E.idxA.as<int>(), E.valA.as<V>(), xbar, E.psumA.as<T>(),
+            E.st.as<InstState>()));
+        hipLaunchKernelGGL((dual_finish<T>), dim3(grid_for(E.m, kBlock)), dim3(kBlock), 0, ctx->stream, E.m, E.SA,
+                           E.psumA.as<T>(), yht, E.lrs.as<T>(), E.urs.as<T>(), y, E.yhat.as<double>(),
+                           E.yfull.as<double>(), E.st.as<InstState>(), E.part_y.as<double>(), it, flags, E.rho);
+        VJ_LAUNCHED();
+        return VJ_OK;
+    }
+    if (E.streamA) {
+        VJ_TRY(require({E.tileA.p}, "stream dual_step"));
+        if (E.multi)
+            hipLaunchKernelGGL((stream_dual_step<V, T, true>), dim3(E.ntA), dim3(kBlock), 0, ctx->stream, E.m,
+                               E.tileA.as<int>(), P.rp.as<int>(), P.ci.as<int>(), av, xbar, yht, E.lrs.as<T>(),
+                               E.urs.as<T>(), y, E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(),
+                               P.row_inst.as<int>(), E.part_y.as<double>(), it, flags, E.rho);
+        else
+            hipLaunchKernelGGL((stream_dual_step<V, T, false>), dim3(E.ntA), dim3(kBlock), 0, ctx->stream, E.m,
+                               E.tileA.as<int>(), P.rp.as<int>(), P.ci.as<int>(), av, xbar, yht, E.lrs.as<T>(),
+                               E.urs.as<T>(), y, E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(),
+                               P.row_inst.as<int>(), E.part_y.as<double>(), it, flags, E.rho);
+        VJ_LAUNCHED();
+        return VJ_OK;
+    }
     if (E.multi) {
         VJ_DISPATCH_G(E.GA, hipLaunchKernelGGL((dual_step<V, T, GG, true>), dim3(tiles), dim3(kBlock), 0, ctx->stream,
             E.m, P.rp.as<int>(), P.ci.as<int>(), av, xbar, yht, E.lrs.as<T>(), E.urs.as<T>(), y,
@@ -323,6 +386,69 @@ int choose_lanes(double avg_nnz) {
     return 32;
 }
 
+// Split layout of a CSR (rows x cols) into S column blocks of width ceil(cols/S):
+// block-major arrays with absolute row pointers bp[k*(rows+1) + r], column indices and the
+// permutation perm[q] = position of split entry q in the original CSR.
+void build_split(int rows, int cols, const std::vector<int64_t>& rp, const std::vector<int32_t>& ci, int S,
+                 std::vector<int32_t>& bp, std::vector<int32_t>& idx, std::vector<int32_t>& perm) {
+    const int64_t nnz = rp[rows];
+    const int64_t B = (static_cast<int64_t>(cols) + S - 1) / S;
+    bp.assign(static_cast<size_t>(S) * (rows + 1), 0);
+    for (int r = 0; r < rows; ++r)
+        for (int64_t p = rp[r]; p < rp[r + 1]; ++p) bp[static_cast<size_t>(ci[p] / B) * (rows + 1) + r + 1]++;
+    int64_t base = 0;
+    for (int k = 0; k < S; ++k) {
+        int32_t* b = bp.data() + static_cast<size_t>(k) * (rows + 1);
+        b[0] = static_cast<int32_t>(base);
+        for (int r = 0; r < rows; ++r) b[r + 1] += b[r];
+        base = b[rows];
+    }
+    idx.resize(nnz);
+    perm.resize(nnz);
+    std::vector<int32_t> cur(bp);
+    for (int r = 0; r < rows; ++r)
+        for (int64_t p = rp[r]; p < rp[r + 1]; ++p) {
+            const int k = static_cast<int>(ci[p] / B);
+            const int32_t q = cur[static_cast<size_t>(k) * (rows + 1) + r]++;
+            idx[q] = ci[p];
+            perm[q] = static_cast<int32_t>(p);
+        }
+}
+
+// Row tiles for the tile-streamed kernels: at most 256 rows and kStreamCap nonzeros per
+// tile; a row longer than kStreamCap is a tile of its own. first/last: per instance, the
+// tiles holding its first and last row.
+void build_stream_tiles(int rows, const std::vector<int64_t>& rp, const std::vector<int64_t>& off, int count,
+                        std::vector<int32_t>& tile_row, std::vector<int32_t>& first, std::vector<int32_t>& last) {
+    tile_row.assign(1, 0);
+    int r = 0;
+    while (r < rows) {
+        const int64_t start = rp[r];
+        int e = r + 1;
+        if (rp[r + 1] - start <= kStreamCap)
+            while (e < rows && e - r < kBlock && rp[e + 1] - start <= kStreamCap) ++e;
+        tile_row.push_back(e);
+        r = e;
+    }
+    first.resize(count);
+    last.resize(count);
+    auto tile_of = [&](int64_t row) {
+        return static_cast<int32_t>(std::upper_bound(tile_row.begin(), tile_row.end(), row) - tile_row.begin() - 1);
+    };
+    for (int q = 0; q < count; ++q) {
+        first[q] = tile_of(off[q]);
+        last[q] = tile_of(off[q + 1] - 1);
+    }
+}
+
+// Column blocks so that one block of the gathered vector fits in half of an XCD's 4 MB L2.
+int choose_splits(int64_t vec_bytes) {
+    const int64_t target = 2ll << 20;
+    int S = 1;
+    while (S < 8 && vec_bytes > target * S) S *= 2;
+    return S;
+}
+
 // ---- setup: scaling, step sizes, initial centre -------------------------------------
 int setup(vj_ctx* ctx, FastEngine& E) {
     Problem& P = ctx->prob;
@@ -343,7 +469,28 @@ int setup(vj_ctx* ctx, FastEngine& E) {
     for (DevBuf* b : {&E.x, &E.xbar, &E.dhat, &E.lcs, &E.ucs}) VJ_TRY(b->alloc(static_cast<size_t>(n) * T));
     for (DevBuf* b : {&E.y, &E.lrs, &E.urs}) VJ_TRY(b->alloc(static_cast<size_t>(m) * T));
     if (E.prec != VJ_PREC_FP64) VJ_TRY(E.yhat_t.alloc(static_cast<size_t>(m) * T));
-    const int px_slots = partial_slots(n, cnt), py_slots = partial_slots(m, cnt);
+    if (!E.tiles_built) {
+        std::vector<int32_t> tr, f, l;
+        build_stream_tiles(m, P.h_rp, P.row_off, cnt, tr, f, l);
+        E.ntA = static_cast<int>(tr.size()) - 1;
+        VJ_TRY(E.tileA.alloc(tr.size() * 4));
+        VJ_TRY(E.tfA.alloc(f.size() * 4));
+        VJ_TRY(E.tlA.alloc(l.size() * 4));
+        VJ_HIP(hipMemcpy(E.tileA.p, tr.data(), tr.size() * 4, hipMemcpyHostToDevice));
+        VJ_HIP(hipMemcpy(E.tfA.p, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+        VJ_HIP(hipMemcpy(E.tlA.p, l.data(), l.size() * 4, hipMemcpyHostToDevice));
+        build_stream_tiles(n, P.h_tp, P.col_off, cnt, tr, f, l);
+        E.ntAt = static_cast<int>(tr.size()) - 1;
+        VJ_TRY(E.tileAt.alloc(tr.size() * 4));
+        VJ_TRY(E.tfAt.alloc(f.size() * 4));
+        VJ_TRY(E.tlAt.alloc(l.size() * 4));
+        VJ_HIP(hipMemcpy(E.tileAt.p, tr.data(), tr.size() * 4, hipMemcpyHostToDevice));
+        VJ_HIP(hipMemcpy(E.tfAt.p, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+        VJ_HIP(hipMemcpy(E.tlAt.p, l.data(), l.size() * 4, hipMemcpyHostToDevice));
+        E.tiles_built = true;
+    }
+    const int px_slots = std::max(partial_slots(n, cnt), E.ntAt + cnt + 1);
+    const int py_slots = std::max(partial_slots(m, cnt), E.ntA + cnt + 1);
     VJ_TRY(E.part_x.alloc(static_cast<size_t>(px_slots) * 3 * 8));
     VJ_TRY(E.part_y.alloc(static_cast<size_t>(py_slots) * 3 * 8));
     VJ_TRY(E.part_kr.alloc(static_cast<size_t>(py_slots) * 3 * 8));
@@ -496,6 +643,59 @@ int setup(vj_ctx* ctx, FastEngine& E) {
     return VJ_OK;
 }
 
+// Column-split copies of the scaled matrix (single LP). The split pattern depends only on
+// the sparsity pattern and S, so it is built on the host once per upload and S; the values
+// are gathered on the device from the scaled FP64 matrix in the working precision.
+int setup_one_split(vj_ctx* ctx, FastEngine& E, int S, int rows, int cols, const std::vector<int64_t>& rp,
+                    const std::vector<int32_t>& ci, const double* vals_dev, int& built, DevBuf& bp, DevBuf& idx,
+                    DevBuf& perm, DevBuf& val, DevBuf& psum) {
+    hipStream_t s = ctx->stream;
+    const int64_t nnz = rp[rows];
+    if (built != S) {
+        std::vector<int32_t> hbp, hidx, hperm;
+        build_split(rows, cols, rp, ci, S, hbp, hidx, hperm);
+        VJ_TRY(bp.alloc(hbp.size() * 4));
+        VJ_TRY(idx.alloc(static_cast<size_t>(nnz) * 4));
+        VJ_TRY(perm.alloc(static_cast<size_t>(nnz) * 4));
+        VJ_HIP(hipMemcpy(bp.p, hbp.data(), hbp.size() * 4, hipMemcpyHostToDevice));
+        VJ_HIP(hipMemcpy(idx.p, hidx.data(), static_cast<size_t>(nnz) * 4, hipMemcpyHostToDevice));
+        VJ_HIP(hipMemcpy(perm.p, hperm.data(), static_cast<size_t>(nnz) * 4, hipMemcpyHostToDevice));
+        built = S;
+    }
+    const size_t vsz = E.prec == VJ_PREC_FP64 ? 8 : (E.prec == VJ_PREC_FP32 ? 4 : 2);
+    VJ_TRY(val.alloc(static_cast<size_t>(nnz) * vsz));
+    VJ_TRY(psum.alloc(static_cast<size_t>(S) * rows * E.tsize()));
+    const int g = grid_for(nnz, kBlock);
+    if (E.prec == VJ_PREC_FP64)
+        hipLaunchKernelGGL(gather_split_vals<double>, dim3(g), dim3(kBlock), 0, s, nnz, perm.as<int>(), vals_dev, val.as<double>());
+    else if (E.prec == VJ_PREC_FP32)
+        hipLaunchKernelGGL(gather_split_vals<float>, dim3(g), dim3(kBlock), 0, s, nnz, perm.as<int>(), vals_dev, val.as<float>());
+    else
+        hipLaunchKernelGGL(gather_split_vals<bf16_t>, dim3(g), dim3(kBlock), 0, s, nnz, perm.as<int>(), vals_dev, val.as<bf16_t>());
+    VJ_LAUNCHED();
+    return VJ_OK;
+}
+
+int setup_splits(vj_ctx* ctx, FastEngine& E) {
+    Problem& P = ctx->prob;
+    const vj_params& prm = ctx->params;
+    const int64_t T = static_cast<int64_t>(E.tsize());
+    E.SA = E.SAt = 1;
+    if (E.count != 1) return VJ_OK;
+    E.SA = prm.col_splits_A ? prm.col_splits_A : choose_splits(static_cast<int64_t>(E.n) * T);
+    E.SAt = prm.col_splits_At ? prm.col_splits_At : choose_splits(static_cast<int64_t>(E.m) * T);
+    if (E.SA > 1)
+        VJ_TRY(setup_one_split(ctx, E, E.SA, E.m, E.n, P.h_rp, P.h_ci, E.av_s.as<double>(), E.builtSA, E.bpA, E.idxA,
+                               E.permA, E.valA, E.psumA));
+    if (E.SAt > 1)
+        VJ_TRY(setup_one_split(ctx, E, E.SAt, E.n, E.m, P.h_tp, P.h_ti, E.tv_s.as<double>(), E.builtSAt, E.bpAt,
+                               E.idxAt, E.permAt, E.valAt, E.psumAt));
+    // split partial kernels see ~nnz/(rows*S) entries per row: few lanes per row
+    E.GsA = (static_cast<double>(E.nnz) / std::max(1, E.m) / E.SA) > 3.0 ? 4 : 2;
+    E.GsAt = (static_cast<double>(E.nnz) / std::max(1, E.n) / E.SAt) > 3.0 ? 4 : 2;
+    return VJ_OK;
+}
+
 template <class T>
 int launch_restart(vj_ctx* ctx, FastEngine& E, int force) {
     Problem& P = ctx->prob;
@@ -542,9 +742,14 @@ int enqueue_block(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
                       E.part_kr.as<double>(), E.part_kc.as<double>(), E.kkt8.as<double>()));
     ControlArgs a;
     a.px = E.part_x.as<double>();
-    a.Rx = kBlock / E.GAt;
+    a.Rx = E.SAt > 1 ? kBlock : kBlock / E.GAt;   // tile height of the kernel writing the partials
     a.py = E.part_y.as<double>();
-    a.Ry = kBlock / E.GA;
+    a.Ry = E.SA > 1 ? kBlock : kBlock / E.GA;
+    const bool sx = E.SAt == 1 && E.streamAt, sy = E.SA == 1 && E.streamA;
+    a.xt_first = sx ? E.tfAt.as<int>() : nullptr;
+    a.xt_last = sx ? E.tlAt.as<int>() : nullptr;
+    a.yt_first = sy ? E.tfA.as<int>() : nullptr;
+    a.yt_last = sy ? E.tlA.as<int>() : nullptr;
     a.kkt = E.kkt8.as<double>();
     a.row_off = ctx->prob.d_row_off.as<int>();
     a.col_off = ctx->prob.d_col_off.as<int>();
@@ -577,9 +782,10 @@ int time_kernels(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv, int reps,
     VJ_HIP(hipEventCreate(&e0));
     VJ_HIP(hipEventCreate(&e1));
     VJ_HIP(hipEventCreate(&e2));
-    // warm-up
-    VJ_TRY(launch_primal<V, T>(ctx, E, tv, xs.as<T>(), xbs.as<T>(), ys.as<T>(), 1, 0));
-    VJ_TRY(launch_dual<V, T>(ctx, E, av, xbs.as<T>(), ys.as<T>(), 1, 0));
+    for (int w = 0; w < 3; ++w) {   // warm-up (clocks, caches)
+        VJ_TRY(launch_primal<V, T>(ctx, E, tv, xs.as<T>(), xbs.as<T>(), ys.as<T>(), 1, 0));
+        VJ_TRY(launch_dual<V, T>(ctx, E, av, xbs.as<T>(), ys.as<T>(), 1, 0));
+    }
     VJ_HIP(hipEventRecord(e0, s));
     for (int r = 0; r < reps; ++r) VJ_TRY(launch_primal<V, T>(ctx, E, tv, xs.as<T>(), xbs.as<T>(), ys.as<T>(), 1, 0));
     VJ_HIP(hipEventRecord(e1, s));
@@ -597,29 +803,63 @@ int time_kernels(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv, int reps,
     return VJ_OK;
 }
 
-// Autotune lanes per row for A (dual_step) and A^T (primal_step) on the real data.
+// Autotune, per matrix, the SpMV layout of each fused step on the real data: the
+// tile-streamed kernel, a lane-group kernel, or the XCD column split (when the gathered
+// vector exceeds the L2 budget) with a few lanes per block segment.
+struct StepCfg {
+    int S, G;
+    bool stream;
+};
+
 template <class V, class T>
 int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
+    const vj_params& prm = ctx->params;
     const double avgA = static_cast<double>(E.nnz) / std::max(1, E.m);
     const double avgAt = static_cast<double>(E.nnz) / std::max(1, E.n);
-    const bool tuneA = ctx->params.lanes_A == 0, tuneAt = ctx->params.lanes_At == 0;
-    E.GA = tuneA ? choose_lanes(avgA) : ctx->params.lanes_A;
-    E.GAt = tuneAt ? choose_lanes(avgAt) : ctx->params.lanes_At;
-    if (!tuneA && !tuneAt) return VJ_OK;
-    if (E.nnz < 200000) return VJ_OK;  // launch-bound sizes: the heuristic is as good as timing
-    const int reps = 3;
+    const bool tuneA = prm.lanes_A == 0, tuneAt = prm.lanes_At == 0;
+    const int SmA = E.SA, SmAt = E.SAt;   // split width prepared by setup_splits
+    // forced: the caller's col_splits (0 = free choice, 1 = unsplit only, >1 = split only)
+    auto base = [&](bool tune, int lanes, double avg, int Sm, int forced) {
+        std::vector<StepCfg> c;
+        if (!tune) {
+            c.push_back({Sm, lanes, false});
+            return c;
+        }
+        if (forced <= 1 || Sm == 1) {   // unsplit candidates
+            c.push_back({1, choose_lanes(avg), true});
+            for (int g : {4, 8, 16, 32}) c.push_back({1, g, false});
+        }
+        if (Sm > 1)
+            for (int g : {1, 2, 4}) c.push_back({Sm, g, false});
+        return c;
+    };
+    std::vector<StepCfg> cA = base(tuneA, prm.lanes_A, avgA, SmA, prm.col_splits_A);
+    std::vector<StepCfg> cAt = base(tuneAt, prm.lanes_At, avgAt, SmAt, prm.col_splits_At);
+    auto apply = [](const StepCfg& c, int& S, int& G, int& Gs, bool& stream) {
+        S = c.S;
+        stream = c.S == 1 && c.stream;
+        if (c.S > 1) Gs = c.G;
+        else G = std::max(c.G, 2);
+    };
+    apply(cA[0], E.SA, E.GA, E.GsA, E.streamA);
+    apply(cAt[0], E.SAt, E.GAt, E.GsAt, E.streamAt);
+    if (E.nnz < 200000 || (cA.size() == 1 && cAt.size() == 1))
+        return VJ_OK;  // launch-bound sizes: the first candidate is as good as timing
+    const size_t rounds = std::max(cA.size(), cAt.size());
     double bestA = 1e30, bestAt = 1e30;
-    int gA = E.GA, gAt = E.GAt;
-    for (int g : {2, 4, 8, 16, 32}) {
-        if (tuneA) E.GA = g;
-        if (tuneAt) E.GAt = g;
+    StepCfg wA = cA[0], wAt = cAt[0];
+    for (size_t r = 0; r < rounds; ++r) {
+        const StepCfg& a = cA[std::min(r, cA.size() - 1)];
+        const StepCfg& b = cAt[std::min(r, cAt.size() - 1)];
+        apply(a, E.SA, E.GA, E.GsA, E.streamA);
+        apply(b, E.SAt, E.GAt, E.GsAt, E.streamAt);
         double up = 0, ud = 0;
-        VJ_TRY((time_kernels<V, T>(ctx, E, av, tv, reps, &up, &ud)));
-        if (tuneAt && up < bestAt) { bestAt = up; gAt = g; }
-        if (tuneA && ud < bestA) { bestA = ud; gA = g; }
+        VJ_TRY((time_kernels<V, T>(ctx, E, av, tv, 3, &up, &ud)));
+        if (ud < bestA) { bestA = ud; wA = a; }
+        if (up < bestAt) { bestAt = up; wAt = b; }
     }
-    E.GA = gA;
-    E.GAt = gAt;
+    apply(wA, E.SA, E.GA, E.GsA, E.streamA);
+    apply(wAt, E.SAt, E.GAt, E.GsAt, E.streamAt);
     return VJ_OK;
 }
 
@@ -636,6 +876,7 @@ int run_typed(vj_ctx* ctx, FastEngine& E, vj_result* out, clk::time_point t_star
         av = reinterpret_cast<const V*>(E.av_lo.p);
         tv = reinterpret_cast<const V*>(E.tv_lo.p);
     }
+    VJ_TRY(setup_splits(ctx, E));
     VJ_TRY(launch_restart<T>(ctx, E, 1));  // initial centring
     VJ_TRY((autotune<V, T>(ctx, E, av, tv)));
     // after autotuning the scratch runs left the real iterates untouched; recentre anyway
@@ -745,8 +986,10 @@ int run_typed(vj_ctx* ctx, FastEngine& E, vj_result* out, clk::time_point t_star
         r.time_to_target = conv_time[q];
         r.host_device_transfers = transfers;
         r.refinements = E.prec == VJ_PREC_FP64 ? 0 : z.restarts;
-        r.lanes_A = E.GA;
-        r.lanes_At = E.GAt;
+        r.lanes_A = E.SA > 1 ? E.GsA : (E.streamA ? 0 : E.GA);      // 0: tile-streamed kernel
+        r.lanes_At = E.SAt > 1 ? E.GsAt : (E.streamAt ? 0 : E.GAt);
+        r.col_splits_A = E.SA;
+        r.col_splits_At = E.SAt;
         r.check_interval = E.K;
         r.checks = z.checks;
         if (z.nan_flag) r.status = VJ_STATUS_NUMERICAL_ERROR;
@@ -806,6 +1049,8 @@ int fast_solve(vj_ctx* ctx, vj_result* out) {
     E.rho = ctx->params.reflection;
     E.GA = ctx->params.lanes_A > 0 ? ctx->params.lanes_A : choose_lanes(static_cast<double>(P.nnz) / std::max(1, P.m));
     E.GAt = ctx->params.lanes_At > 0 ? ctx->params.lanes_At : choose_lanes(static_cast<double>(P.nnz) / std::max(1, P.n));
+    E.streamA = ctx->params.lanes_A == 0;
+    E.streamAt = ctx->params.lanes_At == 0;
     E.hist_cap = P.count == 1 ? 1 << 14 : 0;
     if (E.hist_cap) VJ_TRY(E.hist.alloc(static_cast<size_t>(E.hist_cap) * 8));
     VJ_TRY(setup(ctx, E));
@@ -832,6 +1077,13 @@ int fast_benchmark(vj_ctx* ctx, int reps, vj_kernel_stat* out, int cap, int* wri
     FastEngine& E = *ctx->fast;
     double up = 0, ud = 0;
     int rc;
+    // after a solve every instance is inactive and the kernels would exit at once: time
+    // them with all instances marked active, then put the final state back
+    std::vector<InstState> saved(E.count), active(E.count);
+    VJ_HIP(hipMemcpy(saved.data(), E.st.p, E.count * sizeof(InstState), hipMemcpyDeviceToHost));
+    active = saved;
+    for (auto& z : active) z.active = 1;
+    VJ_HIP(hipMemcpy(E.st.p, active.data(), E.count * sizeof(InstState), hipMemcpyHostToDevice));
     switch (E.prec) {
         case VJ_PREC_FP64:
             rc = time_kernels<double, double>(ctx, E, E.av_s.as<double>(), E.tv_s.as<double>(), reps, &up, &ud);
@@ -843,6 +1095,7 @@ int fast_benchmark(vj_ctx* ctx, int reps, vj_kernel_stat* out, int cap, int* wri
             rc = time_kernels<bf16_t, float>(ctx, E, E.av_lo.as<bf16_t>(), E.tv_lo.as<bf16_t>(), reps, &up, &ud);
             break;
     }
+    VJ_HIP(hipMemcpy(E.st.p, saved.data(), E.count * sizeof(InstState), hipMemcpyHostToDevice));
     if (rc != VJ_OK) return rc;
     const double sv = E.prec == VJ_PREC_FP64 ? 8 : (E.prec == VJ_PREC_FP32 ? 4 : 2);
     const double sx = E.prec == VJ_PREC_FP64 ? 8 : 4;

@@ -132,6 +132,63 @@ __global__ __launch_bounds__(kBlock) void seg_finish(const double* __restrict__ 
 // flags bit 1: last iteration of the block (q1 = |T(z)-z|^2 part, q2 = |T(z)|^2 part,
 //              and T(z) + centre is written in FP64 for the check)
 
+// Primal epilogue for column j given acc = (A^T y)_j: projection, extrapolation, reflected
+// Halpern step, residual contributions c[] at measured iterations.
+template <class T>
+__device__ __forceinline__ void primal_epilogue(T acc, int j, const InstState& s, int it, int flags, double rho,
+                                                const T* __restrict__ dhat, const T* __restrict__ lcs,
+                                                const T* __restrict__ ucs, T* __restrict__ x, T* __restrict__ xbar,
+                                                const double* __restrict__ xhat, double* __restrict__ xfull,
+                                                double (&c)[3]) {
+    const T tau = static_cast<T>(s.tau);
+    const double kk = static_cast<double>(s.k + it);
+    const T lam = static_cast<T>((kk + 1.0) / (kk + 2.0));
+    const T xk = x[j];
+    T xt = xk - tau * (dhat[j] - acc);
+    const T lo = lcs[j], hi = ucs[j];
+    xt = xt < lo ? lo : xt;
+    xt = xt > hi ? hi : xt;
+    xbar[j] = static_cast<T>(2) * xt - xk;
+    x[j] = lam * ((static_cast<T>(1) + static_cast<T>(rho)) * xt - static_cast<T>(rho) * xk);
+    const double d = static_cast<double>(xt) - static_cast<double>(xk);
+    if (flags & 1) c[0] = d * d;
+    if (flags & 2) {
+        c[1] = d * d;
+        c[2] = static_cast<double>(xt) * static_cast<double>(xt);
+        xfull[j] = xhat[j] + static_cast<double>(xt);
+    }
+}
+
+// Dual epilogue for row i given w = (A x_bar)_i: prox of the row-bound support function
+// written per active side, y+ = y + sigma*(proj_[l,u](w - y/sigma) - w) in full
+// coordinates, then the reflected Halpern step.
+template <class T>
+__device__ __forceinline__ void dual_epilogue(T w, int i, const InstState& s, int it, int flags, double rho,
+                                              const T* __restrict__ yhat_t, const T* __restrict__ lrs,
+                                              const T* __restrict__ urs, T* __restrict__ y,
+                                              const double* __restrict__ yhat, double* __restrict__ yfull,
+                                              double (&c)[3]) {
+    const T sig = static_cast<T>(s.sigma);
+    const double kk = static_cast<double>(s.k + it);
+    const T lam = static_cast<T>((kk + 1.0) / (kk + 2.0));
+    const T yk = y[i];
+    const T yh = yhat_t[i];
+    const T lo = lrs[i], hi = urs[i];
+    const T sfull = yh + yk;
+    T yt;
+    if (sig * (w - lo) < sfull) yt = yk + sig * (lo - w);
+    else if (sig * (w - hi) > sfull) yt = yk + sig * (hi - w);
+    else yt = -yh;
+    y[i] = lam * ((static_cast<T>(1) + static_cast<T>(rho)) * yt - static_cast<T>(rho) * yk);
+    const double d = static_cast<double>(yt) - static_cast<double>(yk);
+    if (flags & 1) c[0] = d * d;
+    if (flags & 2) {
+        c[1] = d * d;
+        c[2] = static_cast<double>(yt) * static_cast<double>(yt);
+        yfull[i] = yhat[i] + static_cast<double>(yt);
+    }
+}
+
 template <class V, class T, int G, bool MULTI>
 __global__ __launch_bounds__(kBlock) void primal_step(
     int n, const int* __restrict__ tp, const int* __restrict__ ti, const V* __restrict__ tv,
@@ -159,28 +216,10 @@ __global__ __launch_bounds__(kBlock) void primal_step(
     T acc = 0;
     if (act) {
         const int p1 = tp[j + 1];
-        for (int p = tp[j] + lane; p < p1; p += G) acc += static_cast<T>(load_val(tv, p)) * y[ti[p]];
+        for (int p = tp[j] + lane; p < p1; p += G) acc += static_cast<T>(load_val(tv, p)) * y[load_idx(ti, p)];
     }
     acc = group_sum<G>(acc);
-    if (act && lane == 0) {
-        const T tau = static_cast<T>(st[my].tau);
-        const double kk = static_cast<double>(st[my].k + it);
-        const T lam = static_cast<T>((kk + 1.0) / (kk + 2.0));
-        const T xk = x[j];
-        T xt = xk - tau * (dhat[j] - acc);
-        const T lo = lcs[j], hi = ucs[j];
-        xt = xt < lo ? lo : xt;
-        xt = xt > hi ? hi : xt;
-        xbar[j] = static_cast<T>(2) * xt - xk;
-        x[j] = lam * ((static_cast<T>(1) + static_cast<T>(rho)) * xt - static_cast<T>(rho) * xk);
-        const double d = static_cast<double>(xt) - static_cast<double>(xk);
-        if (flags & 1) c[0] = d * d;
-        if (flags & 2) {
-            c[1] = d * d;
-            c[2] = static_cast<double>(xt) * static_cast<double>(xt);
-            xfull[j] = xhat[j] + static_cast<double>(xt);
-        }
-    }
+    if (act && lane == 0) primal_epilogue<T>(acc, j, st[my], it, flags, rho, dhat, lcs, ucs, x, xbar, xhat, xfull, c);
     if (flags) {
         const int mask = ((flags & 1) ? 1 : 0) | ((flags & 2) ? 6 : 0);
         write_tile_partials<3, MULTI>(c, my, f, l, tile, part, mask);
@@ -213,35 +252,228 @@ __global__ __launch_bounds__(kBlock) void dual_step(
     T w = 0;
     if (act) {
         const int p1 = rp[i + 1];
-        for (int p = rp[i] + lane; p < p1; p += G) w += static_cast<T>(load_val(av, p)) * xbar[ci[p]];
+        for (int p = rp[i] + lane; p < p1; p += G) w += static_cast<T>(load_val(av, p)) * xbar[load_idx(ci, p)];
     }
     w = group_sum<G>(w);
-    if (act && lane == 0) {
-        const T sig = static_cast<T>(st[my].sigma);
-        const double kk = static_cast<double>(st[my].k + it);
-        const T lam = static_cast<T>((kk + 1.0) / (kk + 2.0));
-        const T yk = y[i];
-        const T yh = yhat_t[i];
-        const T lo = lrs[i], hi = urs[i];
-        const T sfull = yh + yk;
-        // prox of the row-bound support function, written per active side:
-        // y+ = y + sigma*(proj_[l,u](w - y/sigma) - w) in full coordinates
-        T yt;
-        if (sig * (w - lo) < sfull) yt = yk + sig * (lo - w);
-        else if (sig * (w - hi) > sfull) yt = yk + sig * (hi - w);
-        else yt = -yh;
-        y[i] = lam * ((static_cast<T>(1) + static_cast<T>(rho)) * yt - static_cast<T>(rho) * yk);
-        const double d = static_cast<double>(yt) - static_cast<double>(yk);
-        if (flags & 1) c[0] = d * d;
-        if (flags & 2) {
-            c[1] = d * d;
-            c[2] = static_cast<double>(yt) * static_cast<double>(yt);
-            yfull[i] = yhat[i] + static_cast<double>(yt);
+    if (act && lane == 0) dual_epilogue<T>(w, i, st[my], it, flags, rho, yhat_t, lrs, urs, y, yhat, yfull, c);
+    if (flags) {
+        const int mask = ((flags & 1) ? 1 : 0) | ((flags & 2) ? 6 : 0);
+        write_tile_partials<3, MULTI>(c, my, f, l, tile, part, mask);
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Tile-streamed fused steps. The host partitions the rows into tiles of at most 256 rows
+// whose nonzeros fit kStreamCap entries (a longer row forms a tile of its own). A workgroup
+// streams its tile's contiguous span of values and indices with coalesced loads that do
+// not depend on the row structure (many independent loads in flight per lane), writes the
+// products to LDS, then one lane per row adds its segment in index order and runs the
+// epilogue. A single-row tile longer than kStreamCap is reduced by the whole workgroup.
+constexpr int kStreamCap = 4096;
+
+template <class V, class T>
+__device__ __forceinline__ void stream_products(int s0, int s1, const V* __restrict__ val, const int* __restrict__ idx,
+                                                const T* __restrict__ vec, T* __restrict__ lds) {
+    constexpr int U = 4;
+    int p = s0 + static_cast<int>(threadIdx.x);
+    for (; p + (U - 1) * kBlock < s1; p += U * kBlock) {
+        int c[U];
+        T a[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            c[u] = load_idx(idx, p + u * kBlock);
+            a[u] = static_cast<T>(load_val(val, p + u * kBlock));
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) lds[p + u * kBlock - s0] = a[u] * vec[c[u]];
+    }
+    for (; p < s1; p += kBlock) lds[p - s0] = static_cast<T>(load_val(val, p)) * vec[load_idx(idx, p)];
+}
+
+// Whole-workgroup sum of one long row (fixed order: strided lane sums, then a tree).
+template <class V, class T>
+__device__ __forceinline__ T long_row_sum(int s0, int s1, const V* __restrict__ val, const int* __restrict__ idx,
+                                          const T* __restrict__ vec, T* __restrict__ lds) {
+    T acc = 0;
+    for (int p = s0 + static_cast<int>(threadIdx.x); p < s1; p += kBlock)
+        acc += static_cast<T>(load_val(val, p)) * vec[load_idx(idx, p)];
+    lds[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) lds[threadIdx.x] += lds[threadIdx.x + w];
+        __syncthreads();
+    }
+    const T r = lds[0];
+    __syncthreads();
+    return r;
+}
+
+template <class V, class T, bool MULTI>
+__global__ __launch_bounds__(kBlock) void stream_primal_step(
+    int n, const int* __restrict__ tile_row, const int* __restrict__ tp, const int* __restrict__ ti,
+    const V* __restrict__ tv, const T* __restrict__ y, const T* __restrict__ dhat, const T* __restrict__ lcs,
+    const T* __restrict__ ucs, T* __restrict__ x, T* __restrict__ xbar, const double* __restrict__ xhat,
+    double* __restrict__ xfull, const InstState* __restrict__ st, const int* __restrict__ col_inst,
+    double* __restrict__ part, int it, int flags, double rho) {
+    __shared__ T lds[kStreamCap];
+    const int tile = blockIdx.x;
+    const int r0 = tile_row[tile], r1 = tile_row[tile + 1];
+    int f = 0, l = 0;
+    if (MULTI) {
+        f = col_inst[r0];
+        l = col_inst[r1 - 1];
+    }
+    bool any = false;
+    for (int q = f; q <= l; ++q) any |= (st[q].active != 0);
+    if (!any) return;
+    const int s0 = tp[r0], s1 = tp[r1];
+    const int j = r0 + static_cast<int>(threadIdx.x);
+    const int my = (MULTI && j < r1) ? col_inst[j] : 0;
+    double c[3] = {0.0, 0.0, 0.0};
+    if (s1 - s0 > kStreamCap) {  // one long row
+        const T acc = long_row_sum<V, T>(s0, s1, tv, ti, y, lds);
+        if (threadIdx.x == 0 && (!MULTI || st[my].active != 0))
+            primal_epilogue<T>(acc, r0, st[my], it, flags, rho, dhat, lcs, ucs, x, xbar, xhat, xfull, c);
+    } else {
+        stream_products<V, T>(s0, s1, tv, ti, y, lds);
+        __syncthreads();
+        if (j < r1 && (!MULTI || st[my].active != 0)) {
+            T acc = 0;
+            const int e = tp[j + 1] - s0;
+            for (int q = tp[j] - s0; q < e; ++q) acc += lds[q];
+            primal_epilogue<T>(acc, j, st[my], it, flags, rho, dhat, lcs, ucs, x, xbar, xhat, xfull, c);
         }
     }
     if (flags) {
         const int mask = ((flags & 1) ? 1 : 0) | ((flags & 2) ? 6 : 0);
         write_tile_partials<3, MULTI>(c, my, f, l, tile, part, mask);
+    }
+}
+
+template <class V, class T, bool MULTI>
+__global__ __launch_bounds__(kBlock) void stream_dual_step(
+    int m, const int* __restrict__ tile_row, const int* __restrict__ rp, const int* __restrict__ ci,
+    const V* __restrict__ av, const T* __restrict__ xbar, const T* __restrict__ yhat_t, const T* __restrict__ lrs,
+    const T* __restrict__ urs, T* __restrict__ y, const double* __restrict__ yhat, double* __restrict__ yfull,
+    const InstState* __restrict__ st, const int* __restrict__ row_inst, double* __restrict__ part, int it, int flags,
+    double rho) {
+    __shared__ T lds[kStreamCap];
+    const int tile = blockIdx.x;
+    const int r0 = tile_row[tile], r1 = tile_row[tile + 1];
+    int f = 0, l = 0;
+    if (MULTI) {
+        f = row_inst[r0];
+        l = row_inst[r1 - 1];
+    }
+    bool any = false;
+    for (int q = f; q <= l; ++q) any |= (st[q].active != 0);
+    if (!any) return;
+    const int s0 = rp[r0], s1 = rp[r1];
+    const int i = r0 + static_cast<int>(threadIdx.x);
+    const int my = (MULTI && i < r1) ? row_inst[i] : 0;
+    double c[3] = {0.0, 0.0, 0.0};
+    if (s1 - s0 > kStreamCap) {
+        const T w = long_row_sum<V, T>(s0, s1, av, ci, xbar, lds);
+        if (threadIdx.x == 0 && (!MULTI || st[my].active != 0))
+            dual_epilogue<T>(w, r0, st[my], it, flags, rho, yhat_t, lrs, urs, y, yhat, yfull, c);
+    } else {
+        stream_products<V, T>(s0, s1, av, ci, xbar, lds);
+        __syncthreads();
+        if (i < r1 && (!MULTI || st[my].active != 0)) {
+            T w = 0;
+            const int e = rp[i + 1] - s0;
+            for (int q = rp[i] - s0; q < e; ++q) w += lds[q];
+            dual_epilogue<T>(w, i, st[my], it, flags, rho, yhat_t, lrs, urs, y, yhat, yfull, c);
+        }
+    }
+    if (flags) {
+        const int mask = ((flags & 1) ? 1 : 0) | ((flags & 2) ? 6 : 0);
+        write_tile_partials<3, MULTI>(c, my, f, l, tile, part, mask);
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// XCD-aware column-split SpMV (single LP). The matrix is stored as S column blocks,
+// each a CSR over all rows with absolute pointers bp[k*(rows+1) + r]. Workgroup w works on
+// block w % S: consecutive workgroup ids are dispatched round-robin over the 8 XCDs, so
+// each XCD (with S dividing 8) only ever gathers from one block of the vector, which stays
+// resident in that XCD's 4 MB L2. Block partial sums go to part[k*rows + r]; the finish
+// kernels add the S partials in block order (deterministic) and run the epilogue.
+template <class V, class T, int G>
+__global__ __launch_bounds__(kBlock) void split_partials(int rows, int S, const int* __restrict__ bp,
+                                                         const int* __restrict__ bidx, const V* __restrict__ bval,
+                                                         const T* __restrict__ vec, T* __restrict__ part,
+                                                         const InstState* __restrict__ st) {
+    if (st[0].active == 0) return;
+    constexpr int R = kBlock / G;
+    const int k = static_cast<int>(blockIdx.x) % S;
+    const int t = static_cast<int>(blockIdx.x) / S;
+    const int r = t * R + static_cast<int>(threadIdx.x) / G;
+    const int lane = threadIdx.x % G;
+    T acc = 0;
+    if (r < rows) {
+        const int* p = bp + static_cast<int64_t>(k) * (rows + 1);
+        const int q1 = p[r + 1];
+        for (int q = p[r] + lane; q < q1; q += G) acc += static_cast<T>(load_val(bval, q)) * vec[load_idx(bidx, q)];
+    }
+    acc = group_sum<G>(acc);
+    if (r < rows && lane == 0) part[static_cast<int64_t>(k) * rows + r] = acc;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void primal_finish(
+    int n, int S, const T* __restrict__ psum, const T* __restrict__ dhat, const T* __restrict__ lcs,
+    const T* __restrict__ ucs, T* __restrict__ x, T* __restrict__ xbar, const double* __restrict__ xhat,
+    double* __restrict__ xfull, const InstState* __restrict__ st, double* __restrict__ part, int it, int flags,
+    double rho) {
+    if (st[0].active == 0) return;
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    double c[3] = {0.0, 0.0, 0.0};
+    if (j < n) {
+        T acc = psum[j];
+        for (int k = 1; k < S; ++k) acc += psum[static_cast<int64_t>(k) * n + j];
+        primal_epilogue<T>(acc, j, st[0], it, flags, rho, dhat, lcs, ucs, x, xbar, xhat, xfull, c);
+    }
+    if (flags) {
+        const int mask = ((flags & 1) ? 1 : 0) | ((flags & 2) ? 6 : 0);
+        write_tile_partials<3, false>(c, 0, 0, 0, blockIdx.x, part, mask);
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void dual_finish(
+    int m, int S, const T* __restrict__ psum, const T* __restrict__ yhat_t, const T* __restrict__ lrs,
+    const T* __restrict__ urs, T* __restrict__ y, const double* __restrict__ yhat, double* __restrict__ yfull,
+    const InstState* __restrict__ st, double* __restrict__ part, int it, int flags, double rho) {
+    if (st[0].active == 0) return;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    double c[3] = {0.0, 0.0, 0.0};
+    if (i < m) {
+        T w = psum[i];
+        for (int k = 1; k < S; ++k) w += psum[static_cast<int64_t>(k) * m + i];
+        dual_epilogue<T>(w, i, st[0], it, flags, rho, yhat_t, lrs, urs, y, yhat, yfull, c);
+    }
+    if (flags) {
+        const int mask = ((flags & 1) ? 1 : 0) | ((flags & 2) ? 6 : 0);
+        write_tile_partials<3, false>(c, 0, 0, 0, blockIdx.x, part, mask);
+    }
+}
+
+// Gather of the scaled matrix into split order: out[q] = in[perm[q]], converted to V.
+template <class V>
+__global__ void gather_split_vals(int64_t nnz, const int* __restrict__ perm, const double* __restrict__ in,
+                                  V* __restrict__ out) {
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= nnz) return;
+    const double v = in[perm[q]];
+    if constexpr (sizeof(V) == 8) {
+        out[q] = v;
+    } else if constexpr (sizeof(V) == 4) {
+        out[q] = static_cast<float>(v);
+    } else {
+        uint32_t u = __float_as_uint(static_cast<float>(v));
+        if ((u & 0x7f800000u) != 0x7f800000u) u += 0x7fffu + ((u >> 16) & 1u);
+        out[q].bits = static_cast<uint16_t>(u >> 16);
     }
 }
 
@@ -391,8 +623,12 @@ __global__ __launch_bounds__(kBlock) void kkt_finish(
 // iterations. At a restart the primal weight is smoothed as in PDLP:
 // log omega <- 0.5 log(|Dy|/|Dx|) + 0.5 log omega.
 struct ControlArgs {
-    const double* px; int Rx;      // primal-step partials (cols)
-    const double* py; int Ry;      // dual-step partials (rows)
+    const double* px; int Rx;      // primal-step partials (cols), fixed tile height Rx
+    const double* py; int Ry;      // dual-step partials (rows), fixed tile height Ry
+    const int* xt_first;           // or per-instance tile ranges of variable-height tiles
+    const int* xt_last;
+    const int* yt_first;
+    const int* yt_last;
     const double* kkt;             // out8 per instance from kkt_finish
     const int* row_off;
     const int* col_off;
@@ -408,12 +644,14 @@ __global__ __launch_bounds__(kBlock) void control_kernel(InstState* __restrict__
     if (st[inst].active == 0) return;
     const int64_t r_a = a.row_off[inst], r_b = a.row_off[inst + 1];
     const int64_t c_a = a.col_off[inst], c_b = a.col_off[inst + 1];
-    const double xq0 = reduce_instance(a.px, 3, 0, inst, c_a, c_b, a.Rx, lds);
-    const double xq1 = reduce_instance(a.px, 3, 1, inst, c_a, c_b, a.Rx, lds);
-    const double xq2 = reduce_instance(a.px, 3, 2, inst, c_a, c_b, a.Rx, lds);
-    const double yq0 = reduce_instance(a.py, 3, 0, inst, r_a, r_b, a.Ry, lds);
-    const double yq1 = reduce_instance(a.py, 3, 1, inst, r_a, r_b, a.Ry, lds);
-    const double yq2 = reduce_instance(a.py, 3, 2, inst, r_a, r_b, a.Ry, lds);
+    double xq[3], yq[3];
+    for (int q = 0; q < 3; ++q) {
+        xq[q] = a.xt_first ? reduce_tiles(a.px, 3, q, inst, a.xt_first[inst], a.xt_last[inst], lds)
+                           : reduce_instance(a.px, 3, q, inst, c_a, c_b, a.Rx, lds);
+        yq[q] = a.yt_first ? reduce_tiles(a.py, 3, q, inst, a.yt_first[inst], a.yt_last[inst], lds)
+                           : reduce_instance(a.py, 3, q, inst, r_a, r_b, a.Ry, lds);
+    }
+    const double xq0 = xq[0], xq1 = xq[1], xq2 = xq[2], yq0 = yq[0], yq1 = yq[1], yq2 = yq[2];
     if (threadIdx.x != 0) return;
     InstState s = st[inst];
     const double* k8 = a.kkt + 8 * inst;

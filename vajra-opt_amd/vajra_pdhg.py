@@ -48,7 +48,8 @@ class Params(ctypes.Structure):
                 ("ruiz_iterations", _c_int), ("pock_chambolle", _c_int),
                 ("bound_objective_rescaling", _c_int), ("reflection", _c_dbl),
                 ("step_scale", _c_dbl), ("use_graph", _c_int), ("lanes_A", _c_int),
-                ("lanes_At", _c_int), ("refine_rounds", _c_int), ("verbose", _c_int)]
+                ("lanes_At", _c_int), ("refine_rounds", _c_int), ("verbose", _c_int),
+                ("col_splits_A", _c_int), ("col_splits_At", _c_int)]
 
 
 class Result(ctypes.Structure):
@@ -61,7 +62,8 @@ class Result(ctypes.Structure):
                 ("iteration_time", _c_dbl), ("total_time", _c_dbl), ("upload_time", _c_dbl),
                 ("download_time", _c_dbl), ("time_to_target", _c_dbl),
                 ("host_device_transfers", _c_i64), ("refinements", _c_int), ("lanes_A", _c_int),
-                ("lanes_At", _c_int), ("check_interval", _c_int), ("checks", _c_i64)]
+                ("lanes_At", _c_int), ("check_interval", _c_int), ("checks", _c_i64),
+                ("col_splits_A", _c_int), ("col_splits_At", _c_int)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
