@@ -40,7 +40,7 @@ class OrResult(ctypes.Structure):
                 ("spectral_norm", _c_dbl), ("tau", _c_dbl), ("sigma", _c_dbl), ("setup_time", _c_dbl),
                 ("iteration_time", _c_dbl), ("total_time", _c_dbl), ("objective", _c_dbl),
                 ("kkt_first_time", _c_dbl), ("kkt_first_iter", _c_i64), ("kkt_last", _c_dbl),
-                ("monitor_time", _c_dbl)]
+                ("monitor_time", _c_dbl), ("final_tau", _c_dbl), ("final_sigma", _c_dbl)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -480,8 +480,8 @@ int or_pdhg_solve(const or_lp* lp, const or_pdhg_opts* opts, double* x_out,
     res->total_time = iter_end - start_time - res->monitor_time;
     res->iterations = iter;
     res->converged = converged;
-    res->tau = tau;
-    res->sigma = sigma;
+    res->final_tau = tau;
+    res->final_sigma = sigma;
 
     if (traj_x) memcpy(traj_x, x, sizeof(double) * (size_t)n);
     if (traj_xbar) memcpy(traj_xbar, xb, sizeof(double) * (size_t)n);
@@ -508,6 +508,14 @@ int or_pdhg_solve(const or_lp* lp, const or_pdhg_opts* opts, double* x_out,
         or_csc_multiply_transpose(m, n, lp->col_ptr, lp->row_idx, lp->vals, row_dual_out, aty);
         for (int j = 0; j < n; ++j) col_dual_out[j] = lp->c[j] - aty[j];
         free(aty);
+        /* pdhg.cpp:437 -> model.cpp:203-207,303-307: when the status claims a point,
+         * recompute_quality overwrites col_dual with the sense-adjusted reduced costs */
+        const int claims = status == ST_OPTIMAL || status == ST_FEASIBLE || status == ST_UNBOUNDED ||
+                           status == ST_ITERATION_LIMIT || status == ST_TIME_LIMIT;
+        if (claims) {
+            double q[5];
+            or_recompute_quality(lp, x_out, row_dual_out, col_dual_out, q);
+        }
     }
     free(yraw); free(rowv);
     free(A); free(wc); free(wcl); free(wcu); free(wrl); free(wru); free(rs); free(cs);

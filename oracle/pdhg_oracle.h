@@ -77,6 +77,9 @@ typedef struct {
     int64_t kkt_first_iter;  /* -1 if never */
     double kkt_last;         /* rel-KKT at the last check */
     double monitor_time;
+    /* step sizes at the end (tau/sigma above are the initial ones, as PdhgDiagnostics
+     * records them at pdhg.cpp:159-160) */
+    double final_tau, final_sigma;
 } or_pdhg_result;
 
 void or_default_opts(or_pdhg_opts* o);

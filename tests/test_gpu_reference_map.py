@@ -44,7 +44,7 @@ def test_cfg0_trajectory_bit_exact(engine, iters, freq):
     assert r.status == o["status"]
     assert r.iterations == o["iterations"]
     assert r.restart_count == o["restart_count"]
-    assert r.tau == o["tau"] and r.sigma == o["sigma"]
+    assert r.tau == o["final_tau"] and r.sigma == o["final_sigma"]
     assert r.primal_residual == o["primal_residual"]
     assert r.dual_residual == o["dual_residual"]
     assert r.duality_gap == o["duality_gap"]
