@@ -201,9 +201,13 @@ void test_netlib() {
     const double opt[2] = {-464.75314285714285, 214.14594594594598};
     for (int q = 0; q < 2; ++q) {
         const indus::Model M = indus::io::read_mps(g_dir + "/" + files[q]);
-        indus::Options o = pdhg_options(g_gpu ? 1e-8 : 1e-4, g_gpu ? 200000 : 50000);
+        // device: the status guard audits every row at 1e-7 absolute or relative to the row's
+        // own bound, which a relative KKT error of 1e-8 (norms over all rows) does not imply
+        // on afiro; 1e-10 does
+        indus::Options o = pdhg_options(g_gpu ? 1e-10 : 1e-4, g_gpu ? 200000 : 50000);
         o.enable_presolve = true;
         const indus::Solution s = indus::solve(M, o);
+        std::cout << "  status " << indus::to_string(s.status) << ": " << s.status_message << "\n";
         // CPU: the reference's 1e-4 run is downgraded by the 1e-7 status guard
         CHECK(s.status == indus::SolveStatus::kOptimal ||
               (!g_gpu && s.status_message.find("Status downgraded") == 0));
@@ -214,7 +218,7 @@ void test_netlib() {
         if (g_gpu) {
             const auto k = indus::verifier::relative_kkt(M, s);
             std::cout << "    rel-KKT " << k.rel_kkt << "\n";
-            CHECK(k.rel_kkt <= 1e-8 * 1.0000001);
+            CHECK(k.rel_kkt <= 1e-10 * 1.0000001);
         }
     }
 }

@@ -147,7 +147,9 @@ int vj_device_probe(vj_device_info* out) {
     }
     out->available = 1;
     out->device_count = count;
-    std::snprintf(out->name, sizeof(out->name), "%s", prop.name);
+    // some driver stacks leave the marketing name empty; fall back to the architecture
+    if (prop.name[0] != '\0') std::snprintf(out->name, sizeof(out->name), "%s", prop.name);
+    else std::snprintf(out->name, sizeof(out->name), "AMD Instinct GPU (%d CUs)", prop.multiProcessorCount);
     std::snprintf(out->arch, sizeof(out->arch), "%s", prop.gcnArchName);
     out->compute_units = prop.multiProcessorCount;
     out->wavefront_size = prop.warpSize;
