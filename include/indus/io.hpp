@@ -17,6 +17,15 @@ namespace indus::io {
 
 Model read_mps(const std::string& filepath);
 
+// CPLEX-style LP files, with the behaviour of src/io/lp_reader.cpp:223-793 of the reference:
+// MINIMIZE/MAXIMIZE, SUBJECT TO, BOUNDS, GENERALS, BINARY, END sections; "\" comments;
+// "name:" labels (the objective's becomes the model name); ranged rows "lhs op expr op rhs";
+// bounds "x >= a", "x <= b", "x = v", "x free", "a <= x <= b" with INF/INFINITY; default
+// bounds 0 <= x < +inf; a quadratic objective part "[ ... ] / 2" is read into Q when the
+// bracket starts a term (after "+ [" the bracket is skipped and its contents are read as
+// linear terms, as the reference does).
+Model read_lp(const std::string& filepath);
+
 // .sol text (writer.cpp:85-166 of the reference): header block, then one line per column
 // and per row with name, value and dual.
 void write_solution(const Solution& solution, const Model& model, const std::string& filepath);

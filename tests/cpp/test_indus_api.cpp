@@ -6,12 +6,16 @@
 //   test_indus_api gpu <data_dir>   MI355X engine (algorithm "pdhg_cuda", require_gpu)
 //
 //   test_indus_api dump <file.mps> <iters> <tol> <cpu|gpu-reference|gpu>
+//   test_indus_api presolve_scale <m> <n>
 //
 // data_dir holds afiro.mps and crude_blend.mps. Exit status 0 iff every check passed.
+#include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <functional>
+#include <iomanip>
 #include <iostream>
 #include <sstream>
 #include <string>
@@ -23,6 +27,7 @@
 #include "indus/pdhg.hpp"
 #include "indus/presolve.hpp"
 #include "indus/verifier.hpp"
+#include "vajra_lpgen.h"
 
 namespace {
 
@@ -299,6 +304,82 @@ void test_mps_reader() {
     CHECK(Q.col_lower[2] == 2.5 && Q.col_upper[2] == 2.5);
 }
 
+// writes M in LP format (every row as "lo <= expr <= hi" or one-sided) for a round trip
+void write_lp_file(const indus::Model& M, const std::string& path) {
+    std::ofstream f(path);
+    f << std::setprecision(17);
+    f << (M.sense == indus::ObjSense::kMaximize ? "Maximize\n" : "Minimize\n") << " obj:";
+    for (int j = 0; j < M.num_cols; ++j) f << (M.c[j] < 0 ? " - " : " + ") << std::fabs(M.c[j]) << " " << M.col_names[j];
+    f << "\nSubject To\n";
+    const indus::la::SparseMatrixCSR R = M.A.to_csr();
+    for (int i = 0; i < M.num_rows; ++i) {
+        f << " " << M.row_names[i] << ": ";
+        const bool lo = M.row_lower[i] > -1e20, hi = M.row_upper[i] < 1e20;
+        if (lo && hi && M.row_lower[i] != M.row_upper[i]) f << M.row_lower[i] << " <= ";
+        for (int64_t k = R.row_ptr[i]; k < R.row_ptr[i + 1]; ++k)
+            f << (R.values[k] < 0 ? " - " : " + ") << std::fabs(R.values[k]) << " " << M.col_names[R.col_idx[k]];
+        if (lo && hi && M.row_lower[i] == M.row_upper[i]) f << " = " << M.row_upper[i] << "\n";
+        else if (hi) f << " <= " << M.row_upper[i] << "\n";
+        else f << " >= " << M.row_lower[i] << "\n";
+    }
+    f << "Bounds\n";
+    for (int j = 0; j < M.num_cols; ++j) {
+        const double l = M.col_lower[j], u = M.col_upper[j];
+        if (l <= -1e20 && u >= 1e20) f << " " << M.col_names[j] << " free\n";
+        else
+            f << " " << (l <= -1e20 ? -1e30 : l) << " <= " << M.col_names[j] << " <= " << (u >= 1e20 ? 1e30 : u) << "\n";
+    }
+    f << "End\n";
+}
+
+void test_lp_reader() {
+    // afiro through an LP-format round trip reads into the same LP
+    const indus::Model A = indus::io::read_mps(g_dir + "/afiro.mps");
+    const std::string path = "/tmp/indus_afiro_roundtrip.lp";
+    write_lp_file(A, path);
+    const indus::Model B = indus::io::read_lp(path);
+    std::remove(path.c_str());
+    CHECK(B.num_rows == A.num_rows && B.num_cols == A.num_cols && B.A.nnz() == A.A.nnz());
+    CHECK(B.name == "obj");
+    bool same = B.A.col_ptr == A.A.col_ptr && B.A.row_idx == A.A.row_idx && B.A.values == A.A.values && B.c == A.c &&
+                B.row_lower == A.row_lower && B.row_upper == A.row_upper;
+    CHECK(same);
+    for (int j = 0; j < A.num_cols; ++j) {
+        CHECK(B.col_lower[j] == (A.col_lower[j] <= -1e20 ? -1e30 : A.col_lower[j]));
+        CHECK(B.col_upper[j] == (A.col_upper[j] >= 1e20 ? 1e30 : A.col_upper[j]));
+    }
+    // the grammar's corners
+    const std::string q = "/tmp/indus_quirks_test.lp";
+    {
+        std::ofstream f(q);
+        f << "\\ comment line\nMAXIMIZE\n profit: 3 x + 2y - z [ 2 x ^2 + 4 x * y ] / 2\nS.T.\n"
+             " c1: x + y <= 4\n 2 x - y >= -2\n r3: -1 <= x - z <= 5.5\n c4: y = 1.5e0\n c5: x + z > 1\n"
+             "Bounds\n y <= 10\n -inf <= z <= 3\n x >= -2\n w free\n 1 <= v <= INF\n"
+             "General\n v\nBinary\n b\nEnd\n ignored: q + r <= 1\n";
+    }
+    const indus::Model L = indus::io::read_lp(q);
+    std::remove(q.c_str());
+    CHECK(L.sense == indus::ObjSense::kMaximize);
+    CHECK(L.name == "profit");
+    CHECK(L.num_rows == 5);
+    CHECK(L.row_names[1] == "R1");
+    CHECK(L.row_lower[0] == -1e20 && L.row_upper[0] == 4);
+    CHECK(L.row_lower[1] == -2 && L.row_upper[1] == 1e20);
+    CHECK(L.row_lower[2] == -1 && L.row_upper[2] == 5.5);
+    CHECK(L.row_lower[3] == 1.5 && L.row_upper[3] == 1.5);
+    CHECK(L.row_lower[4] == 1 && L.row_upper[4] == 1e20);
+    // columns in order of appearance: x, y, z, w, v, b
+    CHECK(L.num_cols == 6 && L.col_names[0] == "x" && L.col_names[1] == "y" && L.col_names[2] == "z");
+    CHECK(L.c[0] == 3 && L.c[1] == 2 && L.c[2] == -1);
+    CHECK(L.Q.nnz() == 2);
+    CHECK(L.col_lower[1] == 0 && L.col_upper[1] == 10);
+    CHECK(L.col_lower[2] == -1e20 && L.col_upper[2] == 3);
+    CHECK(L.col_lower[0] == -2 && L.col_upper[0] == 1e20);
+    CHECK(L.col_lower[3] == -1e20 && L.col_upper[3] == 1e20);
+    CHECK(L.col_lower[4] == 1 && L.col_upper[4] == 1e20 && L.col_type[4] == indus::VarType::kInteger);
+    CHECK(L.col_lower[5] == 0 && L.col_upper[5] == 1 && L.col_type[5] == indus::VarType::kInteger);
+}
+
 void test_writers() {
     const indus::Model M = two_var();
     indus::Options o = pdhg_options();
@@ -452,8 +533,75 @@ int dump(int argc, char** argv) {
     return 0;
 }
 
+// presolve_scale <m> <n>: the presolve -> postsolve handoff on a large random LP with
+// planted reductions (every 50th column fixed at its feasible value, 1000 empty rows, 1000
+// singleton rows); prints the timings and checks that postsolve restores the feasible point.
+int presolve_scale(int argc, char** argv) {
+    if (argc < 4) return 2;
+    const int m = std::atoi(argv[2]), n = std::atoi(argv[3]), k = 6, extra = 1000;
+    const int64_t nnz = static_cast<int64_t>(n) * k;
+    std::vector<int64_t> cp(n + 1);
+    std::vector<int32_t> ri(nnz);
+    std::vector<double> v(nnz), c(n), lc(n), uc(n), lr(m), ur(m), x0(n), y0(m);
+    if (vj_gen_random_lp(7, m, n, k, 0.0, 0.0, cp.data(), ri.data(), v.data(), c.data(), lc.data(), uc.data(),
+                         lr.data(), ur.data(), x0.data(), y0.data()) != 0)
+        return 3;
+    std::vector<indus::la::Triplet> t;
+    t.reserve(static_cast<size_t>(nnz) + extra);
+    for (int j = 0; j < n; ++j)
+        for (int64_t p = cp[j]; p < cp[j + 1]; ++p) t.push_back({ri[p], j, v[p]});
+    indus::Model M;
+    M.num_rows = m + 2 * extra;
+    M.num_cols = n;
+    M.c = c;
+    M.col_lower = lc;
+    M.col_upper = uc;
+    M.row_lower = lr;
+    M.row_upper = ur;
+    for (int e = 0; e < extra; ++e) {            // empty rows
+        M.row_lower.push_back(-1.0);
+        M.row_upper.push_back(1.0);
+    }
+    for (int e = 0; e < extra; ++e) {            // singleton rows x_j <= 20 (looser than the bound 10)
+        const int j = (e * 7919) % n;
+        t.push_back({m + extra + e, j, 1.0});
+        M.row_lower.push_back(-1e20);
+        M.row_upper.push_back(20.0);
+    }
+    for (int j = 0; j < n; j += 50) M.col_lower[j] = M.col_upper[j] = x0[j];   // fixed columns
+    M.A.set_from_triplets(M.num_rows, n, t);
+    const auto t0 = std::chrono::steady_clock::now();
+    const auto pr = indus::presolve::PresolveEngine::apply(M);
+    const auto t1 = std::chrono::steady_clock::now();
+    const indus::Model& R = pr.presolved_model;
+    indus::Solution red;
+    red.status = indus::SolveStatus::kOptimal;
+    red.col_value.resize(R.num_cols);
+    for (int jr = 0; jr < R.num_cols; ++jr) red.col_value[jr] = x0[pr.stack.reduced_to_orig_col[jr]];
+    red.row_dual.assign(R.num_rows, 0.0);
+    R.A.multiply(red.col_value, red.row_value);
+    const auto t2 = std::chrono::steady_clock::now();
+    const indus::Solution full = indus::presolve::PresolveEngine::postsolve(red, pr.stack, M);
+    const auto t3 = std::chrono::steady_clock::now();
+    double maxdiff = 0.0;
+    for (int j = 0; j < n; ++j) maxdiff = std::max(maxdiff, std::fabs(full.col_value[j] - x0[j]));
+    const auto ver = indus::verifier::verify_solution(M, full, 1e-6);
+    std::printf("presolve m=%d n=%d nnz=%lld reductions=%d (fixed %d, empty rows %d, singleton rows %d, "
+                "redundant %d) reduced %dx%d apply_s=%.3f postsolve_s=%.3f max|x-x0|=%.3g primal_feasible=%d\n",
+                M.num_rows, n, static_cast<long long>(M.A.nnz()), pr.total_reductions(), pr.num_fixed_cols,
+                pr.num_empty_rows, pr.num_singleton_rows, pr.num_redundant_rows, R.num_rows, R.num_cols,
+                std::chrono::duration<double>(t1 - t0).count(), std::chrono::duration<double>(t3 - t2).count(),
+                maxdiff, ver.primal_feasible ? 1 : 0);
+    // a singleton row on a fixed column empties first and is removed as an empty row
+    const bool ok = pr.num_fixed_cols >= (n + 49) / 50 && pr.num_empty_rows >= extra &&
+                    pr.num_empty_rows + pr.num_singleton_rows + pr.num_redundant_rows >= 2 * extra &&
+                    R.num_rows <= m && maxdiff == 0.0 && ver.primal_feasible;
+    return ok ? 0 : 1;
+}
+
 int main(int argc, char** argv) {
     if (argc >= 2 && std::string(argv[1]) == "dump") return dump(argc, argv);
+    if (argc >= 2 && std::string(argv[1]) == "presolve_scale") return presolve_scale(argc, argv);
     if (argc < 3 || (std::string(argv[1]) != "cpu" && std::string(argv[1]) != "gpu")) {
         std::cerr << "usage: test_indus_api cpu|gpu <data_dir>\n";
         return 2;
@@ -473,6 +621,7 @@ int main(int argc, char** argv) {
         {"netlib", test_netlib},
         {"device_and_fallback", test_device_and_fallback},
         {"mps_reader", test_mps_reader},
+        {"lp_reader", test_lp_reader},
         {"writers", test_writers},
         {"verifier_rejects", test_verifier_rejects},
     };

@@ -11,18 +11,6 @@ namespace vj {
 constexpr int kBlock = 256;          // threads per workgroup (4 wavefronts of 64)
 constexpr int kWave = 64;
 
-// Workgroups of a launch are dispatched round-robin over the 8 XCDs (workgroup b runs on
-// XCD b % 8, each XCD with its own 4 MB L2). xcd_tile maps workgroup b of a grid padded to
-// xcd_grid(tiles) onto tile (b % 8) * per + b / 8: each XCD walks one contiguous range of
-// tiles in order, so the vector entries a range gathers (one instance's, in a batch) are
-// cached by one L2 instead of all eight. Tiles >= `tiles` are padding and exit at once.
-constexpr int kXcds = 8;
-__host__ __device__ inline int xcd_grid(int tiles) { return ((tiles + kXcds - 1) / kXcds) * kXcds; }
-__device__ __forceinline__ int xcd_tile(int tiles) {
-    const int b = static_cast<int>(blockIdx.x), per = (tiles + kXcds - 1) / kXcds;
-    return (b % kXcds) * per + b / kXcds;
-}
-
 // BF16 matrix values are stored as raw 16-bit patterns (upper half of an FP32).
 struct bf16_t { uint16_t bits; };
 

@@ -171,7 +171,6 @@ struct FastEngine {
     bool streamA = true, streamAt = true, tiles_built = false;
     int ntA = 0, ntAt = 0;
     DevBuf tileA, tileAt, tfA, tlA, tfAt, tlAt;
-    DevBuf tdA, tdAt, tiA, tiAt;     // TileDesc / TileInst per stream tile
     int hist_cap = 0;
     hipGraphExec_t gexec = nullptr;
     InstState* h_state[2] = {nullptr, nullptr};
@@ -213,29 +212,27 @@ int launch_primal(vj_ctx* ctx, FastEngine& E, const V* tv, T* x, T* xbar, const 
         return VJ_OK;
     }
     if (E.streamAt) {
-        VJ_TRY(require({E.tdAt.p, E.tiAt.p, P.d_col_off.p}, "stream primal_step"));
+        VJ_TRY(require({E.tileAt.p}, "stream primal_step"));
         if (E.multi)
-            hipLaunchKernelGGL((stream_primal_step<V, T, true>), dim3(xcd_grid(E.ntAt)), dim3(kBlock), 0, ctx->stream,
-                               E.ntAt, E.tdAt.as<TileDesc>(), E.tiAt.as<TileInst>(), P.d_col_off.as<int>(), P.tp.as<int>(),
-                               P.ti.as<int>(), tv, y, E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), x, xbar,
-                               E.xhat.as<double>(), E.xfull.as<double>(), E.st.as<InstState>(), E.part_x.as<double>(),
-                               it, flags, E.rho);
+            hipLaunchKernelGGL((stream_primal_step<V, T, true>), dim3(E.ntAt), dim3(kBlock), 0, ctx->stream, E.n,
+                               E.tileAt.as<int>(), P.tp.as<int>(), P.ti.as<int>(), tv, y, E.dhat.as<T>(), E.lcs.as<T>(),
+                               E.ucs.as<T>(), x, xbar, E.xhat.as<double>(), E.xfull.as<double>(), E.st.as<InstState>(),
+                               P.col_inst.as<int>(), E.part_x.as<double>(), it, flags, E.rho);
         else
-            hipLaunchKernelGGL((stream_primal_step<V, T, false>), dim3(xcd_grid(E.ntAt)), dim3(kBlock), 0, ctx->stream,
-                               E.ntAt, E.tdAt.as<TileDesc>(), E.tiAt.as<TileInst>(), P.d_col_off.as<int>(), P.tp.as<int>(),
-                               P.ti.as<int>(), tv, y, E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), x, xbar,
-                               E.xhat.as<double>(), E.xfull.as<double>(), E.st.as<InstState>(), E.part_x.as<double>(),
-                               it, flags, E.rho);
+            hipLaunchKernelGGL((stream_primal_step<V, T, false>), dim3(E.ntAt), dim3(kBlock), 0, ctx->stream, E.n,
+                               E.tileAt.as<int>(), P.tp.as<int>(), P.ti.as<int>(), tv, y, E.dhat.as<T>(), E.lcs.as<T>(),
+                               E.ucs.as<T>(), x, xbar, E.xhat.as<double>(), E.xfull.as<double>(), E.st.as<InstState>(),
+                               P.col_inst.as<int>(), E.part_x.as<double>(), it, flags, E.rho);
         VJ_LAUNCHED();
         return VJ_OK;
     }
     if (E.multi) {
-        VJ_DISPATCH_G(E.GAt, hipLaunchKernelGGL((primal_step<V, T, GG, true>), dim3(xcd_grid(tiles)), dim3(kBlock), 0, ctx->stream,
+        VJ_DISPATCH_G(E.GAt, hipLaunchKernelGGL((primal_step<V, T, GG, true>), dim3(tiles), dim3(kBlock), 0, ctx->stream,
             E.n, P.tp.as<int>(), P.ti.as<int>(), tv, y, E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), x, xbar,
             E.xhat.as<double>(), E.xfull.as<double>(), E.st.as<InstState>(), P.col_inst.as<int>(),
             E.part_x.as<double>(), it, flags, E.rho));
     } else {
-        VJ_DISPATCH_G(E.GAt, hipLaunchKernelGGL((primal_step<V, T, GG, false>), dim3(xcd_grid(tiles)), dim3(kBlock), 0, ctx->stream,
+        VJ_DISPATCH_G(E.GAt, hipLaunchKernelGGL((primal_step<V, T, GG, false>), dim3(tiles), dim3(kBlock), 0, ctx->stream,
             E.n, P.tp.as<int>(), P.ti.as<int>(), tv, y, E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), x, xbar,
             E.xhat.as<double>(), E.xfull.as<double>(), E.st.as<InstState>(), P.col_inst.as<int>(),
             E.part_x.as<double>(), it, flags, E.rho));
@@ -279,27 +276,27 @@ int launch_dual(vj_ctx* ctx, FastEngine& E, const V* av, const T* xbar, T* y, in
         return VJ_OK;
     }
     if (E.streamA) {
-        VJ_TRY(require({E.tdA.p, E.tiA.p, P.d_row_off.p}, "stream dual_step"));
+        VJ_TRY(require({E.tileA.p}, "stream dual_step"));
         if (E.multi)
-            hipLaunchKernelGGL((stream_dual_step<V, T, true>), dim3(xcd_grid(E.ntA)), dim3(kBlock), 0, ctx->stream,
-                               E.ntA, E.tdA.as<TileDesc>(), E.tiA.as<TileInst>(), P.d_row_off.as<int>(), P.rp.as<int>(),
-                               P.ci.as<int>(), av, xbar, yht, E.lrs.as<T>(), E.urs.as<T>(), y, E.yhat.as<double>(),
-                               E.yfull.as<double>(), E.st.as<InstState>(), E.part_y.as<double>(), it, flags, E.rho);
+            hipLaunchKernelGGL((stream_dual_step<V, T, true>), dim3(E.ntA), dim3(kBlock), 0, ctx->stream, E.m,
+                               E.tileA.as<int>(), P.rp.as<int>(), P.ci.as<int>(), av, xbar, yht, E.lrs.as<T>(),
+                               E.urs.as<T>(), y, E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(),
+                               P.row_inst.as<int>(), E.part_y.as<double>(), it, flags, E.rho);
         else
-            hipLaunchKernelGGL((stream_dual_step<V, T, false>), dim3(xcd_grid(E.ntA)), dim3(kBlock), 0, ctx->stream,
-                               E.ntA, E.tdA.as<TileDesc>(), E.tiA.as<TileInst>(), P.d_row_off.as<int>(), P.rp.as<int>(),
-                               P.ci.as<int>(), av, xbar, yht, E.lrs.as<T>(), E.urs.as<T>(), y, E.yhat.as<double>(),
-                               E.yfull.as<double>(), E.st.as<InstState>(), E.part_y.as<double>(), it, flags, E.rho);
+            hipLaunchKernelGGL((stream_dual_step<V, T, false>), dim3(E.ntA), dim3(kBlock), 0, ctx->stream, E.m,
+                               E.tileA.as<int>(), P.rp.as<int>(), P.ci.as<int>(), av, xbar, yht, E.lrs.as<T>(),
+                               E.urs.as<T>(), y, E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(),
+                               P.row_inst.as<int>(), E.part_y.as<double>(), it, flags, E.rho);
         VJ_LAUNCHED();
         return VJ_OK;
     }
     if (E.multi) {
-        VJ_DISPATCH_G(E.GA, hipLaunchKernelGGL((dual_step<V, T, GG, true>), dim3(xcd_grid(tiles)), dim3(kBlock), 0, ctx->stream,
+        VJ_DISPATCH_G(E.GA, hipLaunchKernelGGL((dual_step<V, T, GG, true>), dim3(tiles), dim3(kBlock), 0, ctx->stream,
             E.m, P.rp.as<int>(), P.ci.as<int>(), av, xbar, yht, E.lrs.as<T>(), E.urs.as<T>(), y,
             E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(), P.row_inst.as<int>(),
             E.part_y.as<double>(), it, flags, E.rho));
     } else {
-        VJ_DISPATCH_G(E.GA, hipLaunchKernelGGL((dual_step<V, T, GG, false>), dim3(xcd_grid(tiles)), dim3(kBlock), 0, ctx->stream,
+        VJ_DISPATCH_G(E.GA, hipLaunchKernelGGL((dual_step<V, T, GG, false>), dim3(tiles), dim3(kBlock), 0, ctx->stream,
             E.m, P.rp.as<int>(), P.ci.as<int>(), av, xbar, yht, E.lrs.as<T>(), E.urs.as<T>(), y,
             E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(), P.row_inst.as<int>(),
             E.part_y.as<double>(), it, flags, E.rho));
@@ -423,18 +420,13 @@ void build_split(int rows, int cols, const std::vector<int64_t>& rp, const std::
 // tiles holding its first and last row.
 void build_stream_tiles(int rows, const std::vector<int64_t>& rp, const std::vector<int64_t>& off, int count,
                         std::vector<int32_t>& tile_row, std::vector<int32_t>& first, std::vector<int32_t>& last) {
-    // in a batch (count > 1) tiles never cross an instance boundary, so that a tile belongs
-    // to exactly one instance and one workgroup can own all tiles of an instance
     tile_row.assign(1, 0);
     int r = 0;
-    int q = 0;
     while (r < rows) {
-        while (count > 1 && q + 1 < count && off[q + 1] <= r) ++q;
-        const int lim = count > 1 ? static_cast<int>(off[q + 1]) : rows;
         const int64_t start = rp[r];
         int e = r + 1;
         if (rp[r + 1] - start <= kStreamCap)
-            while (e < lim && e - r < kBlock && rp[e + 1] - start <= kStreamCap) ++e;
+            while (e < rows && e - r < kBlock && rp[e + 1] - start <= kStreamCap) ++e;
         tile_row.push_back(e);
         r = e;
     }
@@ -443,25 +435,9 @@ void build_stream_tiles(int rows, const std::vector<int64_t>& rp, const std::vec
     auto tile_of = [&](int64_t row) {
         return static_cast<int32_t>(std::upper_bound(tile_row.begin(), tile_row.end(), row) - tile_row.begin() - 1);
     };
-    for (int k = 0; k < count; ++k) {
-        first[k] = tile_of(off[k]);
-        last[k] = tile_of(off[k + 1] - 1);
-    }
-}
-
-// Packed per-tile descriptors of the stream kernels (rows, nonzero span, instance range).
-void build_tile_descs(const std::vector<int32_t>& tile_row, const std::vector<int64_t>& rp,
-                      const std::vector<int64_t>& off, std::vector<TileDesc>& td, std::vector<TileInst>& ti) {
-    const size_t nt = tile_row.size() - 1;
-    td.resize(nt);
-    ti.resize(nt);
-    auto inst_of = [&](int64_t row) {
-        return static_cast<int>(std::upper_bound(off.begin(), off.end(), row) - off.begin() - 1);
-    };
-    for (size_t t = 0; t < nt; ++t) {
-        const int r0 = tile_row[t], r1 = tile_row[t + 1];
-        td[t] = TileDesc{r0, r1, static_cast<int>(rp[r0]), static_cast<int>(rp[r1])};
-        ti[t] = TileInst{inst_of(r0), inst_of(r1 - 1)};
+    for (int q = 0; q < count; ++q) {
+        first[q] = tile_of(off[q]);
+        last[q] = tile_of(off[q + 1] - 1);
     }
 }
 
@@ -503,13 +479,6 @@ int setup(vj_ctx* ctx, FastEngine& E) {
         VJ_HIP(hipMemcpy(E.tileA.p, tr.data(), tr.size() * 4, hipMemcpyHostToDevice));
         VJ_HIP(hipMemcpy(E.tfA.p, f.data(), f.size() * 4, hipMemcpyHostToDevice));
         VJ_HIP(hipMemcpy(E.tlA.p, l.data(), l.size() * 4, hipMemcpyHostToDevice));
-        std::vector<TileDesc> td;
-        std::vector<TileInst> tq;
-        build_tile_descs(tr, P.h_rp, P.row_off, td, tq);
-        VJ_TRY(E.tdA.alloc(td.size() * sizeof(TileDesc)));
-        VJ_TRY(E.tiA.alloc(tq.size() * sizeof(TileInst)));
-        VJ_HIP(hipMemcpy(E.tdA.p, td.data(), td.size() * sizeof(TileDesc), hipMemcpyHostToDevice));
-        VJ_HIP(hipMemcpy(E.tiA.p, tq.data(), tq.size() * sizeof(TileInst), hipMemcpyHostToDevice));
         build_stream_tiles(n, P.h_tp, P.col_off, cnt, tr, f, l);
         E.ntAt = static_cast<int>(tr.size()) - 1;
         VJ_TRY(E.tileAt.alloc(tr.size() * 4));
@@ -518,11 +487,6 @@ int setup(vj_ctx* ctx, FastEngine& E) {
         VJ_HIP(hipMemcpy(E.tileAt.p, tr.data(), tr.size() * 4, hipMemcpyHostToDevice));
         VJ_HIP(hipMemcpy(E.tfAt.p, f.data(), f.size() * 4, hipMemcpyHostToDevice));
         VJ_HIP(hipMemcpy(E.tlAt.p, l.data(), l.size() * 4, hipMemcpyHostToDevice));
-        build_tile_descs(tr, P.h_tp, P.col_off, td, tq);
-        VJ_TRY(E.tdAt.alloc(td.size() * sizeof(TileDesc)));
-        VJ_TRY(E.tiAt.alloc(tq.size() * sizeof(TileInst)));
-        VJ_HIP(hipMemcpy(E.tdAt.p, td.data(), td.size() * sizeof(TileDesc), hipMemcpyHostToDevice));
-        VJ_HIP(hipMemcpy(E.tiAt.p, tq.data(), tq.size() * sizeof(TileInst), hipMemcpyHostToDevice));
         E.tiles_built = true;
     }
     const int px_slots = std::max(partial_slots(n, cnt), E.ntAt + cnt + 1);

@@ -132,41 +132,22 @@ __global__ __launch_bounds__(kBlock) void seg_finish(const double* __restrict__ 
 // flags bit 1: last iteration of the block (q1 = |T(z)-z|^2 part, q2 = |T(z)|^2 part,
 //              and T(z) + centre is written in FP64 for the check)
 
-// The epilogues are split into a load phase (the row's own data, which does not depend on
-// the SpMV and can be fetched before it) and an update phase.
-template <class T>
-struct PrimalRow {
-    T xk, d, lo, hi;   // x_j, shifted cost dhat_j, shifted bounds
-};
-template <class T>
-struct DualRow {
-    T yk, yh, lo, hi;  // y_i, centre yhat_i (working precision), shifted bounds
-};
-template <class T>
-__device__ __forceinline__ PrimalRow<T> primal_load(int j, const T* __restrict__ dhat, const T* __restrict__ lcs,
-                                                    const T* __restrict__ ucs, const T* __restrict__ x) {
-    return {x[j], dhat[j], lcs[j], ucs[j]};
-}
-template <class T>
-__device__ __forceinline__ DualRow<T> dual_load(int i, const T* __restrict__ yhat_t, const T* __restrict__ lrs,
-                                                const T* __restrict__ urs, const T* __restrict__ y) {
-    return {y[i], yhat_t[i], lrs[i], urs[i]};
-}
-
-// Primal update for column j given acc = (A^T y)_j: projection, extrapolation, reflected
+// Primal epilogue for column j given acc = (A^T y)_j: projection, extrapolation, reflected
 // Halpern step, residual contributions c[] at measured iterations.
 template <class T>
-__device__ __forceinline__ void primal_update(T acc, const PrimalRow<T>& in, int j, double tau_d, int64_t k, int it,
-                                              int flags, double rho, T* __restrict__ x, T* __restrict__ xbar,
-                                              const double* __restrict__ xhat, double* __restrict__ xfull,
-                                              double (&c)[3]) {
-    const T tau = static_cast<T>(tau_d);
-    const double kk = static_cast<double>(k + it);
+__device__ __forceinline__ void primal_epilogue(T acc, int j, const InstState& s, int it, int flags, double rho,
+                                                const T* __restrict__ dhat, const T* __restrict__ lcs,
+                                                const T* __restrict__ ucs, T* __restrict__ x, T* __restrict__ xbar,
+                                                const double* __restrict__ xhat, double* __restrict__ xfull,
+                                                double (&c)[3]) {
+    const T tau = static_cast<T>(s.tau);
+    const double kk = static_cast<double>(s.k + it);
     const T lam = static_cast<T>((kk + 1.0) / (kk + 2.0));
-    const T xk = in.xk;
-    T xt = xk - tau * (in.d - acc);
-    xt = xt < in.lo ? in.lo : xt;
-    xt = xt > in.hi ? in.hi : xt;
+    const T xk = x[j];
+    T xt = xk - tau * (dhat[j] - acc);
+    const T lo = lcs[j], hi = ucs[j];
+    xt = xt < lo ? lo : xt;
+    xt = xt > hi ? hi : xt;
     xbar[j] = static_cast<T>(2) * xt - xk;
     x[j] = lam * ((static_cast<T>(1) + static_cast<T>(rho)) * xt - static_cast<T>(rho) * xk);
     const double d = static_cast<double>(xt) - static_cast<double>(xk);
@@ -178,22 +159,25 @@ __device__ __forceinline__ void primal_update(T acc, const PrimalRow<T>& in, int
     }
 }
 
-// Dual update for row i given w = (A x_bar)_i: prox of the row-bound support function
+// Dual epilogue for row i given w = (A x_bar)_i: prox of the row-bound support function
 // written per active side, y+ = y + sigma*(proj_[l,u](w - y/sigma) - w) in full
 // coordinates, then the reflected Halpern step.
 template <class T>
-__device__ __forceinline__ void dual_update(T w, const DualRow<T>& in, int i, double sigma_d, int64_t k, int it,
-                                            int flags, double rho, T* __restrict__ y, const double* __restrict__ yhat,
-                                            double* __restrict__ yfull, double (&c)[3]) {
-    const T sig = static_cast<T>(sigma_d);
-    const double kk = static_cast<double>(k + it);
+__device__ __forceinline__ void dual_epilogue(T w, int i, const InstState& s, int it, int flags, double rho,
+                                              const T* __restrict__ yhat_t, const T* __restrict__ lrs,
+                                              const T* __restrict__ urs, T* __restrict__ y,
+                                              const double* __restrict__ yhat, double* __restrict__ yfull,
+                                              double (&c)[3]) {
+    const T sig = static_cast<T>(s.sigma);
+    const double kk = static_cast<double>(s.k + it);
     const T lam = static_cast<T>((kk + 1.0) / (kk + 2.0));
-    const T yk = in.yk;
-    const T yh = in.yh;
+    const T yk = y[i];
+    const T yh = yhat_t[i];
+    const T lo = lrs[i], hi = urs[i];
     const T sfull = yh + yk;
     T yt;
-    if (sig * (w - in.lo) < sfull) yt = yk + sig * (in.lo - w);
-    else if (sig * (w - in.hi) > sfull) yt = yk + sig * (in.hi - w);
+    if (sig * (w - lo) < sfull) yt = yk + sig * (lo - w);
+    else if (sig * (w - hi) > sfull) yt = yk + sig * (hi - w);
     else yt = -yh;
     y[i] = lam * ((static_cast<T>(1) + static_cast<T>(rho)) * yt - static_cast<T>(rho) * yk);
     const double d = static_cast<double>(yt) - static_cast<double>(yk);
@@ -205,48 +189,6 @@ __device__ __forceinline__ void dual_update(T w, const DualRow<T>& in, int i, do
     }
 }
 
-// One-call forms for kernels that have no SpMV latency to hide.
-template <class T>
-__device__ __forceinline__ void primal_epilogue(T acc, int j, const InstState& s, int it, int flags, double rho,
-                                                const T* __restrict__ dhat, const T* __restrict__ lcs,
-                                                const T* __restrict__ ucs, T* __restrict__ x, T* __restrict__ xbar,
-                                                const double* __restrict__ xhat, double* __restrict__ xfull,
-                                                double (&c)[3]) {
-    primal_update<T>(acc, primal_load<T>(j, dhat, lcs, ucs, x), j, s.tau, s.k, it, flags, rho, x, xbar, xhat, xfull, c);
-}
-template <class T>
-__device__ __forceinline__ void dual_epilogue(T w, int i, const InstState& s, int it, int flags, double rho,
-                                              const T* __restrict__ yhat_t, const T* __restrict__ lrs,
-                                              const T* __restrict__ urs, T* __restrict__ y,
-                                              const double* __restrict__ yhat, double* __restrict__ yfull,
-                                              double (&c)[3]) {
-    dual_update<T>(w, dual_load<T>(i, yhat_t, lrs, urs, y), i, s.sigma, s.k, it, flags, rho, y, yhat, yfull, c);
-}
-
-// Strided dot product of one row segment [p0, p1) taken by `lane` of a group of G lanes,
-// in ascending order, with U independent predicated loads in flight per step (the same
-// sum as the plain strided loop, without its serialised load latency).
-template <int G, class V, class T>
-__device__ __forceinline__ T lane_dot(int p0, int p1, int lane, const V* __restrict__ val,
-                                      const int* __restrict__ idx, const T* __restrict__ vec) {
-    constexpr int U = 8;
-    T acc = 0;
-    for (int p = p0 + lane; p < p1; p += U * G) {
-        int c[U];
-        T a[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int q = p + u * G;
-            c[u] = q < p1 ? load_idx(idx, q) : 0;
-            a[u] = q < p1 ? static_cast<T>(load_val(val, q)) : static_cast<T>(0);
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (p + u * G < p1) acc += a[u] * vec[c[u]];
-    }
-    return acc;
-}
-
 template <class V, class T, int G, bool MULTI>
 __global__ __launch_bounds__(kBlock) void primal_step(
     int n, const int* __restrict__ tp, const int* __restrict__ ti, const V* __restrict__ tv,
@@ -256,9 +198,8 @@ __global__ __launch_bounds__(kBlock) void primal_step(
     const InstState* __restrict__ st, const int* __restrict__ col_inst, double* __restrict__ part,
     int it, int flags, double rho) {
     constexpr int R = kBlock / G;
-    const int tile = xcd_tile((n + R - 1) / R);
+    const int tile = blockIdx.x;
     const int r0 = tile * R;
-    if (r0 >= n) return;
     const int j = r0 + static_cast<int>(threadIdx.x) / G;
     const int lane = threadIdx.x % G;
     int f = 0, l = 0;
@@ -275,7 +216,7 @@ __global__ __launch_bounds__(kBlock) void primal_step(
     T acc = 0;
     if (act) {
         const int p1 = tp[j + 1];
-        acc = lane_dot<G, V, T>(tp[j], p1, lane, tv, ti, y);
+        for (int p = tp[j] + lane; p < p1; p += G) acc += static_cast<T>(load_val(tv, p)) * y[load_idx(ti, p)];
     }
     acc = group_sum<G>(acc);
     if (act && lane == 0) primal_epilogue<T>(acc, j, st[my], it, flags, rho, dhat, lcs, ucs, x, xbar, xhat, xfull, c);
@@ -293,9 +234,8 @@ __global__ __launch_bounds__(kBlock) void dual_step(
     double* __restrict__ yfull, const InstState* __restrict__ st, const int* __restrict__ row_inst,
     double* __restrict__ part, int it, int flags, double rho) {
     constexpr int R = kBlock / G;
-    const int tile = xcd_tile((m + R - 1) / R);
+    const int tile = blockIdx.x;
     const int r0 = tile * R;
-    if (r0 >= m) return;
     const int i = r0 + static_cast<int>(threadIdx.x) / G;
     const int lane = threadIdx.x % G;
     int f = 0, l = 0;
@@ -312,7 +252,7 @@ __global__ __launch_bounds__(kBlock) void dual_step(
     T w = 0;
     if (act) {
         const int p1 = rp[i + 1];
-        w = lane_dot<G, V, T>(rp[i], p1, lane, av, ci, xbar);
+        for (int p = rp[i] + lane; p < p1; p += G) w += static_cast<T>(load_val(av, p)) * xbar[load_idx(ci, p)];
     }
     w = group_sum<G>(w);
     if (act && lane == 0) dual_epilogue<T>(w, i, st[my], it, flags, rho, yhat_t, lrs, urs, y, yhat, yfull, c);
@@ -328,42 +268,26 @@ __global__ __launch_bounds__(kBlock) void dual_step(
 // streams its tile's contiguous span of values and indices with coalesced loads that do
 // not depend on the row structure (many independent loads in flight per lane), writes the
 // products to LDS, then one lane per row adds its segment in index order and runs the
-// update. A single-row tile longer than kStreamCap is reduced by the whole workgroup.
-//
-// Latency: a tile is short (a few thousand nonzeros), so the number of dependent memory
-// round trips matters as much as bandwidth. The tile descriptor is one 16-byte load; the
-// row's own data, its nonzero span, the instance state and the activity test are issued
-// together in the second round trip, before the nonzero stream.
+// epilogue. A single-row tile longer than kStreamCap is reduced by the whole workgroup.
 constexpr int kStreamCap = 4096;
-
-struct alignas(16) TileDesc {
-    int r0, r1;   // rows [r0, r1)
-    int s0, s1;   // nonzeros [s0, s1)
-};
-struct alignas(8) TileInst {
-    int f, l;     // first and last instance with rows in the tile
-};
 
 template <class V, class T>
 __device__ __forceinline__ void stream_products(int s0, int s1, const V* __restrict__ val, const int* __restrict__ idx,
                                                 const T* __restrict__ vec, T* __restrict__ lds) {
-    // A tile holds at most kStreamCap = 16 * kBlock nonzeros, so one fully unrolled,
-    // predicated round covers it: every load of the tile is issued before the first use.
-    constexpr int U = kStreamCap / kBlock;
-    const int p = s0 + static_cast<int>(threadIdx.x);
-    int c[U];
-    T a[U];
+    constexpr int U = 4;
+    int p = s0 + static_cast<int>(threadIdx.x);
+    for (; p + (U - 1) * kBlock < s1; p += U * kBlock) {
+        int c[U];
+        T a[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const int q = p + u * kBlock;
-        c[u] = q < s1 ? load_idx(idx, q) : 0;
-        a[u] = q < s1 ? static_cast<T>(load_val(val, q)) : static_cast<T>(0);
-    }
+        for (int u = 0; u < U; ++u) {
+            c[u] = load_idx(idx, p + u * kBlock);
+            a[u] = static_cast<T>(load_val(val, p + u * kBlock));
+        }
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const int q = p + u * kBlock;
-        if (q < s1) lds[q - s0] = a[u] * vec[c[u]];
+        for (int u = 0; u < U; ++u) lds[p + u * kBlock - s0] = a[u] * vec[c[u]];
     }
+    for (; p < s1; p += kBlock) lds[p - s0] = static_cast<T>(load_val(val, p)) * vec[load_idx(idx, p)];
 }
 
 // Whole-workgroup sum of one long row (fixed order: strided lane sums, then a tree).
@@ -384,127 +308,87 @@ __device__ __forceinline__ T long_row_sum(int s0, int s1, const V* __restrict__ 
     return r;
 }
 
-// Instance of row r among the instances [f, l] of a tile (offsets off[q] = first row of q),
-// with that instance's step size, epoch index and activity. Every candidate's state is
-// loaded unconditionally so the loads issue together.
-struct RowInst {
-    int inst;
-    int active;
-    double step;
-    int64_t k;
-};
-template <bool PRIMAL, bool MULTI>
-__device__ __forceinline__ RowInst row_instance(int r, int f, int l, const int* __restrict__ off,
-                                                const InstState* __restrict__ st, bool& any) {
-    RowInst ri{f, st[f].active, PRIMAL ? st[f].tau : st[f].sigma, st[f].k};
-    any = ri.active != 0;
-    if (MULTI)
-        for (int q = f + 1; q <= l; ++q) {
-            const int a = st[q].active;
-            const double stp = PRIMAL ? st[q].tau : st[q].sigma;
-            const int64_t kq = st[q].k;
-            any |= a != 0;
-            if (r >= off[q]) ri = RowInst{q, a, stp, kq};
-        }
-    return ri;
-}
-
 template <class V, class T, bool MULTI>
 __global__ __launch_bounds__(kBlock) void stream_primal_step(
-    int ntiles, const TileDesc* __restrict__ tdesc, const TileInst* __restrict__ tinst, const int* __restrict__ col_off,
-    const int* __restrict__ tp, const int* __restrict__ ti, const V* __restrict__ tv, const T* __restrict__ y,
-    const T* __restrict__ dhat, const T* __restrict__ lcs, const T* __restrict__ ucs, T* __restrict__ x,
-    T* __restrict__ xbar, const double* __restrict__ xhat, double* __restrict__ xfull,
-    const InstState* __restrict__ st, double* __restrict__ part, int it, int flags, double rho) {
+    int n, const int* __restrict__ tile_row, const int* __restrict__ tp, const int* __restrict__ ti,
+    const V* __restrict__ tv, const T* __restrict__ y, const T* __restrict__ dhat, const T* __restrict__ lcs,
+    const T* __restrict__ ucs, T* __restrict__ x, T* __restrict__ xbar, const double* __restrict__ xhat,
+    double* __restrict__ xfull, const InstState* __restrict__ st, const int* __restrict__ col_inst,
+    double* __restrict__ part, int it, int flags, double rho) {
     __shared__ T lds[kStreamCap];
-    const int tile = xcd_tile(ntiles);
-    if (tile >= ntiles) return;
-    const TileDesc td = tdesc[tile];
+    const int tile = blockIdx.x;
+    const int r0 = tile_row[tile], r1 = tile_row[tile + 1];
     int f = 0, l = 0;
     if (MULTI) {
-        const TileInst tq = tinst[tile];
-        f = tq.f;
-        l = tq.l;
+        f = col_inst[r0];
+        l = col_inst[r1 - 1];
     }
-    const int j = td.r0 + static_cast<int>(threadIdx.x);
-    const bool row = j < td.r1;
     bool any = false;
-    const RowInst ri = row_instance<true, MULTI>(j, f, l, col_off, st, any);
-    PrimalRow<T> in{};
-    int pb = 0, pe = 0;
-    if (row) {
-        pb = tp[j];
-        pe = tp[j + 1];
-        in = primal_load<T>(j, dhat, lcs, ucs, x);
-    }
+    for (int q = f; q <= l; ++q) any |= (st[q].active != 0);
     if (!any) return;
-    const bool act = row && ri.active != 0;
+    const int s0 = tp[r0], s1 = tp[r1];
+    const int j = r0 + static_cast<int>(threadIdx.x);
+    const int my = (MULTI && j < r1) ? col_inst[j] : 0;
     double c[3] = {0.0, 0.0, 0.0};
-    if (td.s1 - td.s0 > kStreamCap) {  // one long row
-        const T acc = long_row_sum<V, T>(td.s0, td.s1, tv, ti, y, lds);
-        if (threadIdx.x == 0 && act)
-            primal_update<T>(acc, in, j, ri.step, ri.k, it, flags, rho, x, xbar, xhat, xfull, c);
+    if (s1 - s0 > kStreamCap) {  // one long row
+        const T acc = long_row_sum<V, T>(s0, s1, tv, ti, y, lds);
+        if (threadIdx.x == 0 && (!MULTI || st[my].active != 0))
+            primal_epilogue<T>(acc, r0, st[my], it, flags, rho, dhat, lcs, ucs, x, xbar, xhat, xfull, c);
     } else {
-        stream_products<V, T>(td.s0, td.s1, tv, ti, y, lds);
+        stream_products<V, T>(s0, s1, tv, ti, y, lds);
         __syncthreads();
-        if (act) {
+        if (j < r1 && (!MULTI || st[my].active != 0)) {
             T acc = 0;
-            for (int q = pb - td.s0; q < pe - td.s0; ++q) acc += lds[q];
-            primal_update<T>(acc, in, j, ri.step, ri.k, it, flags, rho, x, xbar, xhat, xfull, c);
+            const int e = tp[j + 1] - s0;
+            for (int q = tp[j] - s0; q < e; ++q) acc += lds[q];
+            primal_epilogue<T>(acc, j, st[my], it, flags, rho, dhat, lcs, ucs, x, xbar, xhat, xfull, c);
         }
     }
     if (flags) {
         const int mask = ((flags & 1) ? 1 : 0) | ((flags & 2) ? 6 : 0);
-        write_tile_partials<3, MULTI>(c, ri.inst, f, l, tile, part, mask);
+        write_tile_partials<3, MULTI>(c, my, f, l, tile, part, mask);
     }
 }
 
 template <class V, class T, bool MULTI>
 __global__ __launch_bounds__(kBlock) void stream_dual_step(
-    int ntiles, const TileDesc* __restrict__ tdesc, const TileInst* __restrict__ tinst, const int* __restrict__ row_off,
-    const int* __restrict__ rp, const int* __restrict__ ci, const V* __restrict__ av, const T* __restrict__ xbar,
-    const T* __restrict__ yhat_t, const T* __restrict__ lrs, const T* __restrict__ urs, T* __restrict__ y,
-    const double* __restrict__ yhat, double* __restrict__ yfull, const InstState* __restrict__ st,
-    double* __restrict__ part, int it, int flags, double rho) {
+    int m, const int* __restrict__ tile_row, const int* __restrict__ rp, const int* __restrict__ ci,
+    const V* __restrict__ av, const T* __restrict__ xbar, const T* __restrict__ yhat_t, const T* __restrict__ lrs,
+    const T* __restrict__ urs, T* __restrict__ y, const double* __restrict__ yhat, double* __restrict__ yfull,
+    const InstState* __restrict__ st, const int* __restrict__ row_inst, double* __restrict__ part, int it, int flags,
+    double rho) {
     __shared__ T lds[kStreamCap];
-    const int tile = xcd_tile(ntiles);
-    if (tile >= ntiles) return;
-    const TileDesc td = tdesc[tile];
+    const int tile = blockIdx.x;
+    const int r0 = tile_row[tile], r1 = tile_row[tile + 1];
     int f = 0, l = 0;
     if (MULTI) {
-        const TileInst tq = tinst[tile];
-        f = tq.f;
-        l = tq.l;
+        f = row_inst[r0];
+        l = row_inst[r1 - 1];
     }
-    const int i = td.r0 + static_cast<int>(threadIdx.x);
-    const bool row = i < td.r1;
     bool any = false;
-    const RowInst ri = row_instance<false, MULTI>(i, f, l, row_off, st, any);
-    DualRow<T> in{};
-    int pb = 0, pe = 0;
-    if (row) {
-        pb = rp[i];
-        pe = rp[i + 1];
-        in = dual_load<T>(i, yhat_t, lrs, urs, y);
-    }
+    for (int q = f; q <= l; ++q) any |= (st[q].active != 0);
     if (!any) return;
-    const bool act = row && ri.active != 0;
+    const int s0 = rp[r0], s1 = rp[r1];
+    const int i = r0 + static_cast<int>(threadIdx.x);
+    const int my = (MULTI && i < r1) ? row_inst[i] : 0;
     double c[3] = {0.0, 0.0, 0.0};
-    if (td.s1 - td.s0 > kStreamCap) {
-        const T w = long_row_sum<V, T>(td.s0, td.s1, av, ci, xbar, lds);
-        if (threadIdx.x == 0 && act) dual_update<T>(w, in, i, ri.step, ri.k, it, flags, rho, y, yhat, yfull, c);
+    if (s1 - s0 > kStreamCap) {
+        const T w = long_row_sum<V, T>(s0, s1, av, ci, xbar, lds);
+        if (threadIdx.x == 0 && (!MULTI || st[my].active != 0))
+            dual_epilogue<T>(w, r0, st[my], it, flags, rho, yhat_t, lrs, urs, y, yhat, yfull, c);
     } else {
-        stream_products<V, T>(td.s0, td.s1, av, ci, xbar, lds);
+        stream_products<V, T>(s0, s1, av, ci, xbar, lds);
         __syncthreads();
-        if (act) {
+        if (i < r1 && (!MULTI || st[my].active != 0)) {
             T w = 0;
-            for (int q = pb - td.s0; q < pe - td.s0; ++q) w += lds[q];
-            dual_update<T>(w, in, i, ri.step, ri.k, it, flags, rho, y, yhat, yfull, c);
+            const int e = rp[i + 1] - s0;
+            for (int q = rp[i] - s0; q < e; ++q) w += lds[q];
+            dual_epilogue<T>(w, i, st[my], it, flags, rho, yhat_t, lrs, urs, y, yhat, yfull, c);
         }
     }
     if (flags) {
         const int mask = ((flags & 1) ? 1 : 0) | ((flags & 2) ? 6 : 0);
-        write_tile_partials<3, MULTI>(c, ri.inst, f, l, tile, part, mask);
+        write_tile_partials<3, MULTI>(c, my, f, l, tile, part, mask);
     }
 }
 
@@ -530,7 +414,7 @@ __global__ __launch_bounds__(kBlock) void split_partials(int rows, int S, const 
     if (r < rows) {
         const int* p = bp + static_cast<int64_t>(k) * (rows + 1);
         const int q1 = p[r + 1];
-        acc = lane_dot<G, V, T>(p[r], q1, lane, bval, bidx, vec);
+        for (int q = p[r] + lane; q < q1; q += G) acc += static_cast<T>(load_val(bval, q)) * vec[load_idx(bidx, q)];
     }
     acc = group_sum<G>(acc);
     if (r < rows && lane == 0) part[static_cast<int64_t>(k) * rows + r] = acc;
