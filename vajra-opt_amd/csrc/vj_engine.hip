@@ -1108,10 +1108,10 @@ int fast_benchmark(vj_ctx* ctx, int reps, vj_kernel_stat* out, int cap, int* wri
     // names carry the kernel variant the autotuner chose, as the profiler lists it
     const char* pt = E.prec == VJ_PREC_FP64 ? "double" : (E.prec == VJ_PREC_FP32 ? "float" : "bf16");
     char names[2][48];
-    if (E.SAt > 1) std::snprintf(names[0], 48, "split_partials+primal_finish<%s>", pt);
+    if (E.SAt > 1) std::snprintf(names[0], 48, "split_partials+primal_finish<%s,G=%d>", pt, E.GsAt);
     else if (E.streamAt) std::snprintf(names[0], 48, "stream_primal_step<%s>", pt);
     else std::snprintf(names[0], 48, "primal_step<%s,G=%d>", pt, E.GAt);
-    if (E.SA > 1) std::snprintf(names[1], 48, "split_partials+dual_finish<%s>", pt);
+    if (E.SA > 1) std::snprintf(names[1], 48, "split_partials+dual_finish<%s,G=%d>", pt, E.GsA);
     else if (E.streamA) std::snprintf(names[1], 48, "stream_dual_step<%s>", pt);
     else std::snprintf(names[1], 48, "dual_step<%s,G=%d>", pt, E.GA);
     const double us[2] = {up, ud};
