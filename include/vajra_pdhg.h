@@ -106,6 +106,14 @@ typedef struct vj_params {
     int col_splits_A;              /* column blocks of A for the XCD-split SpMV: 0 = by vector
                                       size vs L2, 1 = fused single pass, 2/4/8 = forced */
     int col_splits_At;             /* same for A^T (blocks of rows of A) */
+    /* RESTARTED: restart rule on the fixed-point residual r (r0 at the epoch start):
+     * restart if r <= sufficient*r0, or r <= necessary*r0 and r grew since the last check,
+     * or the epoch is longer than artificial * (total iterations). */
+    double restart_sufficient;     /* default 0.2 */
+    double restart_necessary;      /* default 0.8 */
+    double restart_artificial;     /* default 0.36 */
+    double primal_weight_smoothing;/* theta in log w <- theta log(|dy|/|dx|) + (1-theta) log w; default 0.5 */
+    int power_iterations;          /* RESTARTED: power-iteration steps for ||A||_2; default 64 */
 } vj_params;
 /* Fill defaults for a mode (reference defaults for VJ_MODE_REFERENCE). */
 void vj_params_default(vj_params* p, int mode);

@@ -50,6 +50,23 @@ def test_cfg0_to_1e6(engine, prec):
     print(prec, r.iterations, r.restart_count, r.iteration_time, k[:4])
 
 
+def test_cfg0_objective_against_highs(engine):
+    """The FP64 engine's objective on the 2,000 x 5,000 LP against the optimum SciPy-HiGHS
+    found for the same LP (the reference's own dual simplex fails on it: status 9, recorded
+    in the golden metadata). At rel-KKT 1e-8 the objective agrees to 1e-6 relative."""
+    import json
+    import os
+    meta = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "golden_meta.json")))
+    opt = meta["cfg0_highs_objective"]
+    lp = lp_cases.cfg0_lp()
+    print("[case] cfg0 fp64 objective vs HiGHS", flush=True)
+    r, x, rd, y = solve(engine, lp, tol=1e-8)
+    assert r.status == 1, r.as_dict()
+    check_point(lp, r, x, y, 1e-8)
+    got = float(np.dot(lp.c, x)) + lp.offset
+    assert abs(got - opt) <= 1e-6 * (1 + abs(opt)), (got, opt)
+
+
 @pytest.mark.parametrize("name", ["afiro", "crude_blend"])
 def test_mps_fixtures(engine, name):
     lp = lp_cases.mps_lp(name)

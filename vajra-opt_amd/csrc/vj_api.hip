@@ -187,6 +187,11 @@ void vj_params_default(vj_params* p, int mode) {
     p->lanes_At = 0;
     p->refine_rounds = 0;
     p->verbose = 0;
+    p->restart_sufficient = 0.2;
+    p->restart_necessary = 0.8;
+    p->restart_artificial = 0.36;
+    p->primal_weight_smoothing = 0.5;
+    p->power_iterations = 64;
 }
 
 int vj_ctx_create(int device, vj_ctx** out) {
@@ -240,6 +245,14 @@ int vj_params_set(vj_ctx* ctx, const vj_params* p) {
     auto splits_ok = [](int k) { return k == 0 || k == 1 || k == 2 || k == 4 || k == 8; };
     if (!splits_ok(p->col_splits_A) || !splits_ok(p->col_splits_At)) {
         set_error("column splits must be 0 (auto), 1, 2, 4 or 8");
+        return VJ_ERR_INVALID_ARGUMENT;
+    }
+    if (!(p->restart_sufficient > 0.0 && p->restart_sufficient < 1.0) ||
+        !(p->restart_necessary >= p->restart_sufficient && p->restart_necessary < 1.0) ||
+        !(p->restart_artificial > 0.0 && p->restart_artificial <= 1.0) ||
+        !(p->primal_weight_smoothing >= 0.0 && p->primal_weight_smoothing <= 1.0) || p->power_iterations < 1 ||
+        p->power_iterations > 10000) {
+        set_error("restart/primal-weight/power-iteration parameters out of range");
         return VJ_ERR_INVALID_ARGUMENT;
     }
     if (!lanes_ok(p->lanes_A) || !lanes_ok(p->lanes_At)) {

@@ -575,7 +575,7 @@ int setup(vj_ctx* ctx, FastEngine& E) {
     double* w = E.wtmp_m.as<double>();
     hipLaunchKernelGGL(init_unit_seg, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, cinst, P.d_col_off.as<int>(), v);
     VJ_LAUNCHED();
-    const int power_iters = 64;
+    const int power_iters = prm.power_iterations;
     for (int it = 0; it < power_iters; ++it) {
         VJ_TRY(spmv_f64(ctx, m, rp, ci, av, v, w, GA));
         VJ_TRY(spmv_f64(ctx, n, tp, ti, tv, w, u, GAt));
@@ -754,9 +754,10 @@ int enqueue_block(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     a.row_off = ctx->prob.d_row_off.as<int>();
     a.col_off = ctx->prob.d_col_off.as<int>();
     a.K = E.K;
-    a.restart_sufficient = 0.2;
-    a.restart_necessary = 0.8;
-    a.restart_artificial = 0.36;
+    a.restart_sufficient = ctx->params.restart_sufficient;
+    a.restart_necessary = ctx->params.restart_necessary;
+    a.restart_artificial = ctx->params.restart_artificial;
+    a.pw_theta = ctx->params.primal_weight_smoothing;
     a.hist = E.hist_cap > 0 ? E.hist.as<double>() : nullptr;
     a.hist_cap = E.hist_cap;
     hipLaunchKernelGGL(control_kernel, dim3(E.count), dim3(kBlock), 0, ctx->stream, E.st.as<InstState>(), a);

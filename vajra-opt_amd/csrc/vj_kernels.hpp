@@ -621,7 +621,7 @@ __global__ __launch_bounds__(kBlock) void kkt_finish(
 // fixed-point residual sqrt(omega |dx|^2 + |dy|^2 / omega), restart when
 // r <= 0.2 r0, or r <= 0.8 r0 and r > r_prev, or the epoch exceeds 0.36 of all
 // iterations. At a restart the primal weight is smoothed as in PDLP:
-// log omega <- 0.5 log(|Dy|/|Dx|) + 0.5 log omega.
+// log omega <- theta log(|Dy|/|Dx|) + (1 - theta) log omega (theta = 0.5 by default).
 struct ControlArgs {
     const double* px; int Rx;      // primal-step partials (cols), fixed tile height Rx
     const double* py; int Ry;      // dual-step partials (rows), fixed tile height Ry
@@ -633,7 +633,7 @@ struct ControlArgs {
     const int* row_off;
     const int* col_off;
     int K;
-    double restart_sufficient, restart_necessary, restart_artificial;
+    double restart_sufficient, restart_necessary, restart_artificial, pw_theta;
     double* hist;                  // per instance: [checks] rel_kkt history (may be null)
     int hist_cap;
 };
@@ -689,7 +689,7 @@ __global__ __launch_bounds__(kBlock) void control_kernel(InstState* __restrict__
             s.k = 0;
             const double dx = sqrt(xq2), dy = sqrt(yq2);
             if (dx > 1e-10 && dy > 1e-10) {
-                s.omega = exp(0.5 * log(dy / dx) + 0.5 * log(s.omega));
+                s.omega = exp(a.pw_theta * log(dy / dx) + (1.0 - a.pw_theta) * log(s.omega));
                 s.tau = s.eta / s.omega;
                 s.sigma = s.eta * s.omega;
             }

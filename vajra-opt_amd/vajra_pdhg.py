@@ -49,7 +49,9 @@ class Params(ctypes.Structure):
                 ("bound_objective_rescaling", _c_int), ("reflection", _c_dbl),
                 ("step_scale", _c_dbl), ("use_graph", _c_int), ("lanes_A", _c_int),
                 ("lanes_At", _c_int), ("refine_rounds", _c_int), ("verbose", _c_int),
-                ("col_splits_A", _c_int), ("col_splits_At", _c_int)]
+                ("col_splits_A", _c_int), ("col_splits_At", _c_int),
+                ("restart_sufficient", _c_dbl), ("restart_necessary", _c_dbl), ("restart_artificial", _c_dbl),
+                ("primal_weight_smoothing", _c_dbl), ("power_iterations", _c_int)]
 
 
 class Result(ctypes.Structure):
