@@ -57,9 +57,9 @@ struct GpuTimingDiagnostics {
 [[nodiscard]] bool is_gpu_available() noexcept;
 [[nodiscard]] GpuTimingDiagnostics get_last_gpu_timing() noexcept;
 
-// GPU PDHG; with no device it prints a notice and falls back to the CPU PDHG as the
-// reference does (gpu_context.cpp:65-71), unless Options "require_gpu" is true, in which case
-// it throws std::runtime_error. HIP failures throw std::runtime_error (pdhg_cuda.cu:20-28).
+// GPU PDHG. With no device it throws std::runtime_error, unless Options "allow_cpu_fallback"
+// is true: then it prints a notice and runs the CPU PDHG, as the reference always does
+// (gpu_context.cpp:65-71). HIP failures throw std::runtime_error (pdhg_cuda.cu:20-28).
 Solution solve_pdhg_gpu(const Model& model, const Options& options = Options());
 
 // Many independent LPs in one device pass (no reference counterpart): every instance keeps

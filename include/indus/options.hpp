@@ -8,8 +8,8 @@
 //
 // PDHG keys read on this path: "tolerance", "gap_tolerance", "check_frequency", "verbose"
 // (reference, pdhg.cpp:62-67) and the engine's own: "pdhg_mode" ("restarted" | "reference"),
-// "pdhg_precision" ("fp64" | "fp32" | "bf16"), "pdhg_graph" (bool), "require_gpu" (bool:
-// throw instead of the CPU fallback when no device is present).
+// "pdhg_precision" ("fp64" | "fp32" | "bf16"), "pdhg_graph" (bool), "allow_cpu_fallback"
+// (bool: with no device, run the CPU PDHG instead of throwing).
 #pragma once
 #include <cstdint>
 #include <string>

@@ -1,6 +1,6 @@
 // CPU PDHG (restates include/indus/pdhg.hpp:17-67 of the reference). This is the reference
-// iteration map on the host: the solver behind solve_pdhg_cpu and the CPU fallback of
-// solve_pdhg_gpu when no device is present.
+// iteration map on the host: the solver behind solve_pdhg_cpu (and behind solve_pdhg_gpu
+// when no device is present and the caller set "allow_cpu_fallback").
 #pragma once
 #include <cstdint>
 

@@ -3,7 +3,7 @@
 // the MPS reader, the writers, the verifier, the presolve handoff and the batch entry point.
 //
 //   test_indus_api cpu <data_dir>   CPU PDHG (algorithm "pdhg_cpu"); no device needed
-//   test_indus_api gpu <data_dir>   MI355X engine (algorithm "pdhg_cuda", require_gpu)
+//   test_indus_api gpu <data_dir>   MI355X engine (algorithm "pdhg_cuda")
 //
 //   test_indus_api dump <file.mps> <iters> <tol> <cpu|gpu-reference|gpu>
 //   test_indus_api presolve_scale <m> <n>
@@ -62,7 +62,6 @@ indus::Options pdhg_options(double tol = 0.0, int64_t iters = 20000) {
     if (g_gpu) {
         o.set("algorithm", "pdhg_cuda");
         o.use_gpu = true;
-        o.set("require_gpu", true);
     } else {
         o.set("algorithm", "pdhg_cpu");
     }
@@ -239,18 +238,17 @@ void test_device_and_fallback() {
     o.use_gpu = true;
     o.enable_presolve = false;
     o.iteration_limit = 5000;
-    const indus::Solution s = indus::solve(M, o);
-    CHECK(s.status == indus::SolveStatus::kOptimal);
-    CHECK_NEAR(s.objective_value, 3.0, 1e-3);
-    const auto t = indus::gpu::get_last_gpu_timing();
-    std::cout << "  kernel mode: " << t.kernel_mode << "\n";
     if (g_gpu) {
+        const indus::Solution s = indus::solve(M, o);
+        CHECK(s.status == indus::SolveStatus::kOptimal);
+        CHECK_NEAR(s.objective_value, 3.0, 1e-3);
+        const auto t = indus::gpu::get_last_gpu_timing();
+        std::cout << "  kernel mode: " << t.kernel_mode << "\n";
         CHECK(t.kernel_mode.find("restarted_halpern") == 0);
         CHECK(t.iterations == s.iterations);
         CHECK(t.rel_kkt <= 1e-6);
     } else {
-        CHECK(t.kernel_mode == "CPU reference engine");
-        o.set("require_gpu", true);
+        // no device: a device request fails loudly ...
         bool threw = false;
         try {
             (void)indus::solve(M, o);
@@ -258,6 +256,14 @@ void test_device_and_fallback() {
             threw = true;
         }
         CHECK(threw);
+        // ... unless the caller asks for the reference's CPU fallback (gpu_context.cpp:65-71)
+        o.set("allow_cpu_fallback", true);
+        o.set("tolerance", 1e-7);
+        const indus::Solution s = indus::solve(M, o);
+        CHECK(s.status == indus::SolveStatus::kOptimal);
+        CHECK_NEAR(s.objective_value, 3.0, 1e-3);
+        CHECK(s.algorithm_used == "pdhg_cpu");
+        CHECK(indus::gpu::get_last_gpu_timing().kernel_mode == "CPU reference engine");
     }
 }
 
@@ -516,7 +522,6 @@ int dump(int argc, char** argv) {
         std::printf("diag %lld %d %a %a %a\n", static_cast<long long>(d.iterations), d.restart_count, d.tau, d.sigma,
                     d.spectral_norm_estimate);
     } else {
-        o.set("require_gpu", true);
         if (how == "gpu-reference") o.set("pdhg_mode", "reference");
         s = indus::gpu::solve_pdhg_gpu(M, o);
     }

@@ -184,6 +184,7 @@ int main(int argc, char** argv) {
             indus::Options o;
             o.set("algorithm", algo);
             o.use_gpu = std::strcmp(algo, "pdhg_cpu") != 0;
+            o.set("allow_cpu_fallback", true);   // rows without a device say "CPU FALLBACK"
             o.enable_presolve = false;
             o.iteration_limit = lim;
             o.set("tolerance", 1e-4);

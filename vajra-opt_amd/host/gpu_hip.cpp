@@ -163,8 +163,11 @@ GpuTimingDiagnostics get_last_gpu_timing() noexcept {
 Solution solve_pdhg_gpu(const Model& model, const Options& options) {
     const GpuDeviceInfo info = probe_device();
     if (!info.available) {
-        if (options.get_bool("require_gpu", false)) throw std::runtime_error("solve_pdhg_gpu: no HIP device available");
-        std::cout << "[INDUS-GPU] no HIP device (" << info.device_name << "); falling back to the CPU PDHG.\n";
+        // no silent substitute for the device: the CPU PDHG runs only when the caller asks
+        if (!options.get_bool("allow_cpu_fallback", false))
+            throw std::runtime_error("solve_pdhg_gpu: no HIP device available (Options \"allow_cpu_fallback\" runs "
+                                     "the CPU PDHG instead)");
+        std::cout << "[INDUS-GPU] no HIP device (" << info.device_name << "); allow_cpu_fallback set: running the CPU PDHG.\n";
         GpuTimingDiagnostics t;
         t.device_name = "None (CPU fallback)";
         t.kernel_mode = "CPU reference engine";
@@ -232,7 +235,9 @@ std::vector<Solution> solve_pdhg_gpu_batch(const std::vector<Model>& models, con
     std::vector<Solution> out(models.size());
     if (models.empty()) return out;
     if (!is_gpu_available()) {
-        if (options.get_bool("require_gpu", false)) throw std::runtime_error("solve_pdhg_gpu_batch: no HIP device available");
+        if (!options.get_bool("allow_cpu_fallback", false))
+            throw std::runtime_error("solve_pdhg_gpu_batch: no HIP device available (Options \"allow_cpu_fallback\" "
+                                     "runs the CPU PDHG instead)");
         for (size_t q = 0; q < models.size(); ++q) out[q] = pdhg::solve_pdhg_cpu(models[q], options);
         return out;
     }

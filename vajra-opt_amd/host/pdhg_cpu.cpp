@@ -1,6 +1,7 @@
 // CPU PDHG: the reference iteration map on the host (semantics of
-// src/solvers/pdhg/pdhg.cpp:11-453 of the reference). Serves solve_pdhg_cpu and the CPU
-// fallback of solve_pdhg_gpu; the GPU engine's VJ_MODE_REFERENCE runs the same map.
+// src/solvers/pdhg/pdhg.cpp:11-453 of the reference). Serves solve_pdhg_cpu (and
+// solve_pdhg_gpu without a device when "allow_cpu_fallback" is set); the GPU engine's
+// VJ_MODE_REFERENCE runs the same map.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
