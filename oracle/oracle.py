@@ -192,14 +192,23 @@ def ref_solve(lp, algorithm="pdhg_cpu", mode=0, presolve=False, scaling=True, to
     return d
 
 
-def ref_read_mps(path):
-    dims = np.zeros(4, np.int32)
+def _ref_read(fn, path):
+    dims = np.zeros(6, np.int32)
     off = ctypes.c_double(0.0)
-    ref_lib().ref_read_mps(path.encode(), _p(dims), ctypes.byref(off), None, None, None, None, None, None, None, None)
-    m, n, nnz, mx = [int(v) for v in dims]
+    fn(path.encode(), _p(dims), ctypes.byref(off), None, None, None, None, None, None, None, None)
+    m, n, nnz, mx, ints, qnnz = [int(v) for v in dims]
     cp, ri, v = np.zeros(n + 1, np.int64), np.zeros(nnz, np.int32), np.zeros(nnz)
     c, lc, uc, lr, ur = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(m), np.zeros(m)
-    ref_lib().ref_read_mps(path.encode(), _p(dims), ctypes.byref(off), _p(cp), _p(ri), _p(v), _p(c), _p(lc), _p(uc),
-                           _p(lr), _p(ur))
+    fn(path.encode(), _p(dims), ctypes.byref(off), _p(cp), _p(ri), _p(v), _p(c), _p(lc), _p(uc), _p(lr), _p(ur))
     return dict(m=m, n=n, col_ptr=cp, row_idx=ri, vals=v, c=c, col_lower=lc, col_upper=uc, row_lower=lr,
-                row_upper=ur, maximize=bool(mx), offset=off.value)
+                row_upper=ur, maximize=bool(mx), offset=off.value, integers=ints, q_nnz=qnnz)
+
+
+def ref_read_mps(path):
+    """The reference's MPS reader (src/io/mps_reader.cpp), compiled from its sources."""
+    return _ref_read(ref_lib().ref_read_mps, path)
+
+
+def ref_read_lp(path):
+    """The reference's LP reader (src/io/lp_reader.cpp), compiled from its sources."""
+    return _ref_read(ref_lib().ref_read_lp, path)

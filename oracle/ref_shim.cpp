@@ -159,13 +159,21 @@ int ref_spmv(int m, int n, const int64_t* col_ptr, const int* row_idx, const dou
 }
 
 // Two-call MPS reader: first call with null arrays fills dims[0..3] = m, n, nnz, maximize.
-int ref_read_mps(const char* path, int* dims, double* offset, int64_t* col_ptr, int* row_idx,
-                 double* vals, double* c, double* lc, double* uc, double* lr, double* ur) {
-    indus::Model model = indus::io::read_mps(path);
+// Model read by the reference's own MPS (lp = 0) or LP (lp = 1) reader; two calls: dims first
+// (m, n, nnz, maximize, number of integer columns, Q nonzeros), then the arrays.
+static int ref_read_any(int lp, const char* path, int* dims, double* offset, int64_t* col_ptr, int* row_idx,
+                        double* vals, double* c, double* lc, double* uc, double* lr, double* ur) {
+    indus::Model model = lp ? indus::io::read_lp(path) : indus::io::read_mps(path);
     dims[0] = model.num_rows;
     dims[1] = model.num_cols;
     dims[2] = static_cast<int>(model.A.nnz());
     dims[3] = model.sense == indus::ObjSense::kMaximize ? 1 : 0;
+    if (dims[4] >= 0) {   // extended query: integer columns and Q nonzeros
+        int ints = 0;
+        for (auto t : model.col_type) ints += t == indus::VarType::kInteger;
+        dims[4] = ints;
+        dims[5] = static_cast<int>(model.Q.nnz());
+    }
     *offset = model.objective_offset;
     if (!col_ptr) return 0;
     const int m = model.num_rows, n = model.num_cols;
@@ -178,6 +186,16 @@ int ref_read_mps(const char* path, int* dims, double* offset, int64_t* col_ptr, 
     std::memcpy(lr, model.row_lower.data(), sizeof(double) * m);
     std::memcpy(ur, model.row_upper.data(), sizeof(double) * m);
     return 0;
+}
+
+extern "C" int ref_read_mps(const char* path, int* dims, double* offset, int64_t* col_ptr, int* row_idx,
+                            double* vals, double* c, double* lc, double* uc, double* lr, double* ur) {
+    return ref_read_any(0, path, dims, offset, col_ptr, row_idx, vals, c, lc, uc, lr, ur);
+}
+
+extern "C" int ref_read_lp(const char* path, int* dims, double* offset, int64_t* col_ptr, int* row_idx,
+                           double* vals, double* c, double* lc, double* uc, double* lr, double* ur) {
+    return ref_read_any(1, path, dims, offset, col_ptr, row_idx, vals, c, lc, uc, lr, ur);
 }
 
 // verify_solution on a caller-given point; out: 0 passed, 1 max_primal_violation,

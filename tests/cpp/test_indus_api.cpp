@@ -7,6 +7,7 @@
 //
 //   test_indus_api dump <file.mps> <iters> <tol> <cpu|gpu-reference|gpu>
 //   test_indus_api presolve_scale <m> <n>
+//   test_indus_api dump_model <file> <mps|lp>
 //
 // data_dir holds afiro.mps and crude_blend.mps. Exit status 0 iff every check passed.
 #include <chrono>
@@ -604,8 +605,41 @@ int presolve_scale(int argc, char** argv) {
     return ok ? 0 : 1;
 }
 
+// dump_model <file> <mps|lp>: the Model a reader builds, in hexadecimal floating point, for
+// comparison with the reference's own readers.
+int dump_model(int argc, char** argv) {
+    if (argc < 4) return 2;
+    const std::string kind = argv[3];
+    const indus::Model M = kind == "lp" ? indus::io::read_lp(argv[2]) : indus::io::read_mps(argv[2]);
+    int ints = 0;
+    for (auto t : M.col_type) ints += t == indus::VarType::kInteger;
+    std::printf("dims %d %d %lld %d %d %lld %a\n", M.num_rows, M.num_cols, static_cast<long long>(M.A.nnz()),
+                M.sense == indus::ObjSense::kMaximize ? 1 : 0, ints, static_cast<long long>(M.Q.nnz()),
+                M.objective_offset);
+    auto ints_line = [](const char* tag, const auto& v) {
+        std::printf("%s", tag);
+        for (auto a : v) std::printf(" %lld", static_cast<long long>(a));
+        std::printf("\n");
+    };
+    auto dbl_line = [](const char* tag, const std::vector<double>& v) {
+        std::printf("%s", tag);
+        for (double a : v) std::printf(" %a", a);
+        std::printf("\n");
+    };
+    ints_line("col_ptr", M.A.col_ptr);
+    ints_line("row_idx", M.A.row_idx);
+    dbl_line("vals", M.A.values);
+    dbl_line("c", M.c);
+    dbl_line("col_lower", M.col_lower);
+    dbl_line("col_upper", M.col_upper);
+    dbl_line("row_lower", M.row_lower);
+    dbl_line("row_upper", M.row_upper);
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc >= 2 && std::string(argv[1]) == "dump") return dump(argc, argv);
+    if (argc >= 2 && std::string(argv[1]) == "dump_model") return dump_model(argc, argv);
     if (argc >= 2 && std::string(argv[1]) == "presolve_scale") return presolve_scale(argc, argv);
     if (argc < 3 || (std::string(argv[1]) != "cpu" && std::string(argv[1]) != "gpu")) {
         std::cerr << "usage: test_indus_api cpu|gpu <data_dir>\n";
