@@ -99,8 +99,11 @@ typedef struct vj_params {
     double reflection;             /* RESTARTED: reflection rho in ((1+rho)T - rho I), [0,1] */
     double step_scale;             /* RESTARTED: eta = step_scale / ||A||_2 */
     int use_graph;                 /* RESTARTED: replay the check block as a hipGraph */
-    int lanes_A;                   /* lanes per row of A  (1..64, power of 2; 0 = autotune) */
-    int lanes_At;                  /* lanes per row of A^T (1..64, power of 2; 0 = autotune) */
+    int lanes_A;                   /* SpMV kernel for rows of A: 0 = autotune; 2..32 = lanes per row
+                                      (lane groups, or lanes per block segment when split);
+                                      -1 = tile-streamed; -2 = tile-streamed reading interleaved
+                                      (index, value) records (FP32; other precisions as -1) */
+    int lanes_At;                  /* same for rows of A^T */
     int refine_rounds;             /* FP32/BF16: max FP64 refinement rounds (0 = 64) */
     int verbose;
     int col_splits_A;              /* column blocks of A for the XCD-split SpMV: 0 = by vector

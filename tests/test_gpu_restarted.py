@@ -114,3 +114,61 @@ def test_column_split_spmv(engine, prec, sa, sat):
     check_point(lp, r1, x1, y1, 1e-6)
     r2, x2, _, y2 = solve(engine, lp, precision=prec, col_splits_A=sa, col_splits_At=sat)
     assert r1.iterations == r2.iterations and np.array_equal(x1, x2) and np.array_equal(y1, y2)
+
+
+def long_row_lp(seed=7, m=10000, n=24000, k=4):
+    """Feasible, bounded LP (the random construction: b = A x0, c = A^T y0 + s0) whose row 0
+    is dense over all n columns and whose column 0 is dense over all m rows, so that tiles of
+    one row longer than the 4096-entry LDS capacity occur in A and A^T, also per column block
+    of the split layouts (n/4 = 6000 and m/2 = 5000 entries)."""
+    rng = np.random.default_rng(seed)
+    cols = [np.union1d(rng.choice(m, k, replace=False), [0]) for _ in range(n)]
+    cols[0] = np.arange(m)
+    col_ptr = np.zeros(n + 1, np.int64)
+    col_ptr[1:] = np.cumsum([len(c) for c in cols])
+    row_idx = np.concatenate(cols).astype(np.int32)
+    vals = rng.standard_normal(len(row_idx))
+    lp = vp.LP(m, n, col_ptr, row_idx, vals, np.zeros(n), np.zeros(n), np.full(n, 10.0), np.zeros(m), np.zeros(m))
+    x0 = rng.uniform(0, 1, n)
+    y0 = rng.standard_normal(m)
+    A = (lp.col_ptr, lp.row_idx, lp.vals)
+    b = np.zeros(m)
+    c = rng.uniform(0, 1, n)
+    for j in range(n):
+        s, e = A[0][j], A[0][j + 1]
+        b[A[1][s:e]] += A[2][s:e] * x0[j]
+        c[j] += np.dot(A[2][s:e], y0[A[1][s:e]])
+    lp.row_lower[:] = b
+    lp.row_upper[:] = b
+    lp.c[:] = c
+    return lp
+
+
+_LONG = {}
+
+
+@pytest.mark.parametrize("prec,lanes,sa,sat", [
+    ("fp32", -2, 1, 1), ("fp32", -1, 1, 1), ("fp32", -2, 4, 2), ("fp32", -1, 4, 2),
+    ("fp64", -1, 4, 2), ("fp64", -1, 1, 1), ("bf16", -2, 2, 2)])
+def test_tile_stream_variants(engine, prec, lanes, sa, sat):
+    """Each tile-streamed kernel (fused, column-split, with separate index/value arrays or
+    with FP32 (index, value) records), forced on an LP with rows and columns longer than the
+    LDS tile: solves to rel-KKT 1e-6 (oracle-checked), the objective agrees with the
+    autotuned FP64 solve, and a rerun is bit-identical."""
+    lp = _LONG.get("lp")
+    if lp is None:
+        lp = _LONG["lp"] = long_row_lp()
+    print(f"[case] tile-stream {prec} lanes={lanes} SA={sa} SAt={sat}", flush=True)
+    kw = dict(precision=prec, lanes_A=lanes, lanes_At=lanes, col_splits_A=sa, col_splits_At=sat)
+    r1, x1, _, y1 = solve(engine, lp, **kw)
+    assert (r1.col_splits_A, r1.col_splits_At) == (sa, sat)
+    assert r1.status == 1, r1.as_dict()
+    check_point(lp, r1, x1, y1, 1e-6)
+    r2, x2, _, y2 = solve(engine, lp, **kw)
+    assert r1.iterations == r2.iterations and np.array_equal(x1, x2) and np.array_equal(y1, y2)
+    if "obj" not in _LONG:
+        r0, x0, _, _ = solve(engine, lp, tol=1e-8)
+        assert r0.status == 1
+        _LONG["obj"] = float(np.dot(lp.c, x0))
+    o0, o1 = _LONG["obj"], float(np.dot(lp.c, x1))
+    assert abs(o1 - o0) <= 2e-5 * (1 + abs(o0)), (o1, o0)

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <initializer_list>
 #include <memory>
@@ -165,6 +166,13 @@ struct FastEngine {
     // of A (gathers y); S = 1 means the fused single-pass kernels are used
     int SA = 1, SAt = 1, GsA = 2, GsAt = 2;
     DevBuf bpA, idxA, valA, psumA, bpAt, idxAt, valAt, psumAt, permA, permAt;
+    // row tiles of the split layouts for split_stream_partials (Gs = 0 selects that kernel)
+    DevBuf tileSA, tileSAt;
+    int ntSA = 0, ntSAt = 0;
+    // FP32: interleaved (index, value) records of A, A^T and of the split layouts, read by
+    // the tile-streamed kernels when the autotuner picks them (recA / recAt)
+    DevBuf recA_buf, recAt_buf, recSA_buf, recSAt_buf;
+    bool recA = false, recAt = false;
     int builtSA = 0, builtSAt = 0;   // split layout currently held in bp/idx/perm
     // tile-streamed kernels: host-built row tiles of A (y side) and A^T (x side) and, per
     // instance, the first/last tile touching it
@@ -194,6 +202,21 @@ int partial_slots(int rows, int count) { return tiles_for(rows, 32) + count + 1;
 // ---- launch helpers ---------------------------------------------------------------
 int require(std::initializer_list<const void*> ptrs, const char* where);
 
+// Call go(entries) with the entry accessor the chosen variant reads: FP32 records when
+// `rec` is set (FP32 values only), else the separate index / value arrays.
+template <class V, class F>
+int with_entries(bool rec, const DevBuf& recs, const int* idx, const V* val, const char* where, F go) {
+    if constexpr (std::is_same<V, float>::value) {
+        if (rec) {
+            VJ_TRY(require({recs.p}, where));
+            go(RecEntries{recs.as<vj_i32x2>()});
+            return VJ_OK;
+        }
+    }
+    go(SepEntries<V>{idx, val});
+    return VJ_OK;
+}
+
 template <class V, class T>
 int launch_primal(vj_ctx* ctx, FastEngine& E, const V* tv, T* x, T* xbar, const T* y, int it, int flags) {
     Problem& P = ctx->prob;
@@ -202,9 +225,19 @@ int launch_primal(vj_ctx* ctx, FastEngine& E, const V* tv, T* x, T* xbar, const 
                     E.part_x.p, E.multi ? P.col_inst.p : E.st.p}, "primal_step"));
     if (E.SAt > 1) {
         VJ_TRY(require({E.bpAt.p, E.idxAt.p, E.valAt.p, E.psumAt.p}, "split primal_step"));
-        VJ_DISPATCH_G(E.GsAt, hipLaunchKernelGGL((split_partials<V, T, GG>), dim3(E.SAt * tiles_for(E.n, GG)), dim3(kBlock),
-            0, ctx->stream, E.n, E.SAt, E.bpAt.as<int>(), E.idxAt.as<int>(), E.valAt.as<V>(), y, E.psumAt.as<T>(),
-            E.st.as<InstState>()));
+        if (E.GsAt == 0) {
+            VJ_TRY(require({E.tileSAt.p}, "split_stream_partials"));
+            VJ_TRY(with_entries<V>(E.recAt, E.recSAt_buf, E.idxAt.as<int>(), E.valAt.as<V>(), "split records",
+                                   [&](auto ent) {
+                hipLaunchKernelGGL((split_stream_partials<decltype(ent), T>), dim3(E.SAt * E.ntSAt), dim3(kBlock), 0,
+                                   ctx->stream, E.n, E.SAt, E.tileSAt.as<int>(), E.bpAt.as<int>(), ent, y,
+                                   E.psumAt.as<T>(), E.st.as<InstState>());
+            }));
+        } else {
+            VJ_DISPATCH_G(E.GsAt, hipLaunchKernelGGL((split_partials<V, T, GG>), dim3(E.SAt * tiles_for(E.n, GG)),
+                dim3(kBlock), 0, ctx->stream, E.n, E.SAt, E.bpAt.as<int>(), E.idxAt.as<int>(), E.valAt.as<V>(), y,
+                E.psumAt.as<T>(), E.st.as<InstState>()));
+        }
         hipLaunchKernelGGL((primal_finish<T>), dim3(grid_for(E.n, kBlock)), dim3(kBlock), 0, ctx->stream, E.n, E.SAt,
                            E.psumAt.as<T>(), E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), x, xbar, E.xhat.as<double>(),
                            E.xfull.as<double>(), E.st.as<InstState>(), E.part_x.as<double>(), it, flags, E.rho);
@@ -213,16 +246,19 @@ int launch_primal(vj_ctx* ctx, FastEngine& E, const V* tv, T* x, T* xbar, const 
     }
     if (E.streamAt) {
         VJ_TRY(require({E.tileAt.p}, "stream primal_step"));
-        if (E.multi)
-            hipLaunchKernelGGL((stream_primal_step<V, T, true>), dim3(E.ntAt), dim3(kBlock), 0, ctx->stream, E.n,
-                               E.tileAt.as<int>(), P.tp.as<int>(), P.ti.as<int>(), tv, y, E.dhat.as<T>(), E.lcs.as<T>(),
-                               E.ucs.as<T>(), x, xbar, E.xhat.as<double>(), E.xfull.as<double>(), E.st.as<InstState>(),
-                               P.col_inst.as<int>(), E.part_x.as<double>(), it, flags, E.rho);
-        else
-            hipLaunchKernelGGL((stream_primal_step<V, T, false>), dim3(E.ntAt), dim3(kBlock), 0, ctx->stream, E.n,
-                               E.tileAt.as<int>(), P.tp.as<int>(), P.ti.as<int>(), tv, y, E.dhat.as<T>(), E.lcs.as<T>(),
-                               E.ucs.as<T>(), x, xbar, E.xhat.as<double>(), E.xfull.as<double>(), E.st.as<InstState>(),
-                               P.col_inst.as<int>(), E.part_x.as<double>(), it, flags, E.rho);
+        VJ_TRY(with_entries<V>(E.recAt, E.recAt_buf, P.ti.as<int>(), tv, "stream records", [&](auto ent) {
+            using Ent = decltype(ent);
+            if (E.multi)
+                hipLaunchKernelGGL((stream_primal_step<Ent, T, true>), dim3(E.ntAt), dim3(kBlock), 0, ctx->stream, E.n,
+                                   E.tileAt.as<int>(), P.tp.as<int>(), ent, y, E.dhat.as<T>(), E.lcs.as<T>(),
+                                   E.ucs.as<T>(), x, xbar, E.xhat.as<double>(), E.xfull.as<double>(),
+                                   E.st.as<InstState>(), P.col_inst.as<int>(), E.part_x.as<double>(), it, flags, E.rho);
+            else
+                hipLaunchKernelGGL((stream_primal_step<Ent, T, false>), dim3(E.ntAt), dim3(kBlock), 0, ctx->stream, E.n,
+                                   E.tileAt.as<int>(), P.tp.as<int>(), ent, y, E.dhat.as<T>(), E.lcs.as<T>(),
+                                   E.ucs.as<T>(), x, xbar, E.xhat.as<double>(), E.xfull.as<double>(),
+                                   E.st.as<InstState>(), P.col_inst.as<int>(), E.part_x.as<double>(), it, flags, E.rho);
+        }));
         VJ_LAUNCHED();
         return VJ_OK;
     }
@@ -266,9 +302,19 @@ int launch_dual(vj_ctx* ctx, FastEngine& E, const V* av, const T* xbar, T* y, in
                     E.multi ? P.row_inst.p : E.st.p}, "dual_step"));
     if (E.SA > 1) {
         VJ_TRY(require({E.bpA.p, E.idxA.p, E.valA.p, E.psumA.p}, "split dual_step"));
-        VJ_DISPATCH_G(E.GsA, hipLaunchKernelGGL((split_partials<V, T, GG>), dim3(E.SA * tiles_for(E.m, GG)), dim3(kBlock),
-            0, ctx->stream, E.m, E.SA, E.bpA.as<int>(), E.idxA.as<int>(), E.valA.as<V>(), xbar, E.psumA.as<T>(),
-            E.st.as<InstState>()));
+        if (E.GsA == 0) {
+            VJ_TRY(require({E.tileSA.p}, "split_stream_partials"));
+            VJ_TRY(with_entries<V>(E.recA, E.recSA_buf, E.idxA.as<int>(), E.valA.as<V>(), "split records",
+                                   [&](auto ent) {
+                hipLaunchKernelGGL((split_stream_partials<decltype(ent), T>), dim3(E.SA * E.ntSA), dim3(kBlock), 0,
+                                   ctx->stream, E.m, E.SA, E.tileSA.as<int>(), E.bpA.as<int>(), ent, xbar,
+                                   E.psumA.as<T>(), E.st.as<InstState>());
+            }));
+        } else {
+            VJ_DISPATCH_G(E.GsA, hipLaunchKernelGGL((split_partials<V, T, GG>), dim3(E.SA * tiles_for(E.m, GG)),
+                dim3(kBlock), 0, ctx->stream, E.m, E.SA, E.bpA.as<int>(), E.idxA.as<int>(), E.valA.as<V>(), xbar,
+                E.psumA.as<T>(), E.st.as<InstState>()));
+        }
         hipLaunchKernelGGL((dual_finish<T>), dim3(grid_for(E.m, kBlock)), dim3(kBlock), 0, ctx->stream, E.m, E.SA,
                            E.psumA.as<T>(), yht, E.lrs.as<T>(), E.urs.as<T>(), y, E.yhat.as<double>(),
                            E.yfull.as<double>(), E.st.as<InstState>(), E.part_y.as<double>(), it, flags, E.rho);
@@ -277,16 +323,19 @@ int launch_dual(vj_ctx* ctx, FastEngine& E, const V* av, const T* xbar, T* y, in
     }
     if (E.streamA) {
         VJ_TRY(require({E.tileA.p}, "stream dual_step"));
-        if (E.multi)
-            hipLaunchKernelGGL((stream_dual_step<V, T, true>), dim3(E.ntA), dim3(kBlock), 0, ctx->stream, E.m,
-                               E.tileA.as<int>(), P.rp.as<int>(), P.ci.as<int>(), av, xbar, yht, E.lrs.as<T>(),
-                               E.urs.as<T>(), y, E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(),
-                               P.row_inst.as<int>(), E.part_y.as<double>(), it, flags, E.rho);
-        else
-            hipLaunchKernelGGL((stream_dual_step<V, T, false>), dim3(E.ntA), dim3(kBlock), 0, ctx->stream, E.m,
-                               E.tileA.as<int>(), P.rp.as<int>(), P.ci.as<int>(), av, xbar, yht, E.lrs.as<T>(),
-                               E.urs.as<T>(), y, E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(),
-                               P.row_inst.as<int>(), E.part_y.as<double>(), it, flags, E.rho);
+        VJ_TRY(with_entries<V>(E.recA, E.recA_buf, P.ci.as<int>(), av, "stream records", [&](auto ent) {
+            using Ent = decltype(ent);
+            if (E.multi)
+                hipLaunchKernelGGL((stream_dual_step<Ent, T, true>), dim3(E.ntA), dim3(kBlock), 0, ctx->stream, E.m,
+                                   E.tileA.as<int>(), P.rp.as<int>(), ent, xbar, yht, E.lrs.as<T>(), E.urs.as<T>(), y,
+                                   E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(),
+                                   P.row_inst.as<int>(), E.part_y.as<double>(), it, flags, E.rho);
+            else
+                hipLaunchKernelGGL((stream_dual_step<Ent, T, false>), dim3(E.ntA), dim3(kBlock), 0, ctx->stream, E.m,
+                                   E.tileA.as<int>(), P.rp.as<int>(), ent, xbar, yht, E.lrs.as<T>(), E.urs.as<T>(), y,
+                                   E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(),
+                                   P.row_inst.as<int>(), E.part_y.as<double>(), it, flags, E.rho);
+        }));
         VJ_LAUNCHED();
         return VJ_OK;
     }
@@ -438,6 +487,28 @@ void build_stream_tiles(int rows, const std::vector<int64_t>& rp, const std::vec
     for (int q = 0; q < count; ++q) {
         first[q] = tile_of(off[q]);
         last[q] = tile_of(off[q + 1] - 1);
+    }
+}
+
+// Row tiles of a split layout (S blocks, absolute pointers bp[k*(rows+1) + r]) for the
+// tile-streamed split kernel: at most kSplitRows rows and at most kStreamCap entries in
+// every block; a row longer than kStreamCap in some block is a tile of its own.
+void build_split_tiles(int rows, int S, const std::vector<int32_t>& bp, std::vector<int32_t>& tile_row) {
+    tile_row.assign(1, 0);
+    auto fits = [&](int r, int e) {
+        for (int k = 0; k < S; ++k) {
+            const int32_t* b = bp.data() + static_cast<size_t>(k) * (rows + 1);
+            if (b[e] - b[r] > kStreamCap) return false;
+        }
+        return true;
+    };
+    int r = 0;
+    while (r < rows) {
+        int e = r + 1;
+        if (fits(r, e))
+            while (e < rows && e - r < kSplitRows && fits(r, e + 1)) ++e;
+        tile_row.push_back(e);
+        r = e;
     }
 }
 
@@ -601,6 +672,12 @@ int setup(vj_ctx* ctx, FastEngine& E) {
         VJ_TRY(E.tv_lo.alloc(nnz * 4));
         hipLaunchKernelGGL(convert_f32, dim3(grid_for(nnz, kBlock)), dim3(kBlock), 0, s, nnz, av, E.av_lo.as<float>());
         hipLaunchKernelGGL(convert_f32, dim3(grid_for(nnz, kBlock)), dim3(kBlock), 0, s, nnz, tv, E.tv_lo.as<float>());
+        VJ_TRY(E.recA_buf.alloc(nnz * 8));
+        VJ_TRY(E.recAt_buf.alloc(nnz * 8));
+        hipLaunchKernelGGL(pack_records, dim3(grid_for(nnz, kBlock)), dim3(kBlock), 0, s, nnz, ci, E.av_lo.as<float>(),
+                           E.recA_buf.as<vj_i32x2>());
+        hipLaunchKernelGGL(pack_records, dim3(grid_for(nnz, kBlock)), dim3(kBlock), 0, s, nnz, ti, E.tv_lo.as<float>(),
+                           E.recAt_buf.as<vj_i32x2>());
         VJ_LAUNCHED();
     } else if (E.prec == VJ_PREC_BF16) {
         VJ_TRY(E.av_lo.alloc(nnz * 2));
@@ -650,12 +727,17 @@ int setup(vj_ctx* ctx, FastEngine& E) {
 // are gathered on the device from the scaled FP64 matrix in the working precision.
 int setup_one_split(vj_ctx* ctx, FastEngine& E, int S, int rows, int cols, const std::vector<int64_t>& rp,
                     const std::vector<int32_t>& ci, const double* vals_dev, int& built, DevBuf& bp, DevBuf& idx,
-                    DevBuf& perm, DevBuf& val, DevBuf& psum) {
+                    DevBuf& perm, DevBuf& val, DevBuf& psum, DevBuf& tiles, int& ntiles, DevBuf& rec) {
     hipStream_t s = ctx->stream;
     const int64_t nnz = rp[rows];
     if (built != S) {
         std::vector<int32_t> hbp, hidx, hperm;
         build_split(rows, cols, rp, ci, S, hbp, hidx, hperm);
+        std::vector<int32_t> htile;
+        build_split_tiles(rows, S, hbp, htile);
+        ntiles = static_cast<int>(htile.size()) - 1;
+        VJ_TRY(tiles.alloc(htile.size() * 4));
+        VJ_HIP(hipMemcpy(tiles.p, htile.data(), htile.size() * 4, hipMemcpyHostToDevice));
         VJ_TRY(bp.alloc(hbp.size() * 4));
         VJ_TRY(idx.alloc(static_cast<size_t>(nnz) * 4));
         VJ_TRY(perm.alloc(static_cast<size_t>(nnz) * 4));
@@ -670,8 +752,12 @@ int setup_one_split(vj_ctx* ctx, FastEngine& E, int S, int rows, int cols, const
     const int g = grid_for(nnz, kBlock);
     if (E.prec == VJ_PREC_FP64)
         hipLaunchKernelGGL(gather_split_vals<double>, dim3(g), dim3(kBlock), 0, s, nnz, perm.as<int>(), vals_dev, val.as<double>());
-    else if (E.prec == VJ_PREC_FP32)
+    else if (E.prec == VJ_PREC_FP32) {
         hipLaunchKernelGGL(gather_split_vals<float>, dim3(g), dim3(kBlock), 0, s, nnz, perm.as<int>(), vals_dev, val.as<float>());
+        VJ_TRY(rec.alloc(static_cast<size_t>(nnz) * 8));
+        hipLaunchKernelGGL(pack_records, dim3(g), dim3(kBlock), 0, s, nnz, idx.as<int>(), val.as<float>(),
+                           rec.as<vj_i32x2>());
+    }
     else
         hipLaunchKernelGGL(gather_split_vals<bf16_t>, dim3(g), dim3(kBlock), 0, s, nnz, perm.as<int>(), vals_dev, val.as<bf16_t>());
     VJ_LAUNCHED();
@@ -687,10 +773,10 @@ int setup_splits(vj_ctx* ctx, FastEngine& E) {
     E.SAt = prm.col_splits_At ? prm.col_splits_At : choose_splits(E.m);
     if (E.SA > 1)
         VJ_TRY(setup_one_split(ctx, E, E.SA, E.m, E.n, P.h_rp, P.h_ci, E.av_s.as<double>(), E.builtSA, E.bpA, E.idxA,
-                               E.permA, E.valA, E.psumA));
+                               E.permA, E.valA, E.psumA, E.tileSA, E.ntSA, E.recSA_buf));
     if (E.SAt > 1)
         VJ_TRY(setup_one_split(ctx, E, E.SAt, E.n, E.m, P.h_tp, P.h_ti, E.tv_s.as<double>(), E.builtSAt, E.bpAt,
-                               E.idxAt, E.permAt, E.valAt, E.psumAt));
+                               E.idxAt, E.permAt, E.valAt, E.psumAt, E.tileSAt, E.ntSAt, E.recSAt_buf));
     // split partial kernels see ~nnz/(rows*S) entries per row: few lanes per row
     E.GsA = (static_cast<double>(E.nnz) / std::max(1, E.m) / E.SA) > 3.0 ? 4 : 2;
     E.GsAt = (static_cast<double>(E.nnz) / std::max(1, E.n) / E.SAt) > 3.0 ? 4 : 2;
@@ -811,6 +897,7 @@ int time_kernels(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv, int reps,
 struct StepCfg {
     int S, G;
     bool stream;
+    bool rec;   // FP32 (index, value) records in the tile-streamed kernels
 };
 
 template <class V, class T>
@@ -821,30 +908,37 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     const bool tuneA = prm.lanes_A == 0, tuneAt = prm.lanes_At == 0;
     const int SmA = E.SA, SmAt = E.SAt;   // split width prepared by setup_splits
     // forced: the caller's col_splits (0 = free choice, 1 = unsplit only, >1 = split only)
+    const bool f32 = E.prec == VJ_PREC_FP32;
     auto base = [&](bool tune, int lanes, double avg, int Sm, int forced) {
         std::vector<StepCfg> c;
         if (!tune) {
-            c.push_back({Sm, lanes, false});
+            if (lanes < 0) c.push_back({Sm, Sm > 1 ? 0 : choose_lanes(avg), Sm == 1, lanes == -2 && f32});
+            else c.push_back({Sm, lanes, false, false});
             return c;
         }
         if (forced <= 1 || Sm == 1) {   // unsplit candidates
-            c.push_back({1, choose_lanes(avg), true});
-            for (int g : {4, 8, 16, 32}) c.push_back({1, g, false});
+            c.push_back({1, choose_lanes(avg), true, false});
+            if (f32) c.push_back({1, choose_lanes(avg), true, true});
+            for (int g : {4, 8, 16, 32}) c.push_back({1, g, false, false});
         }
-        if (Sm > 1)
-            for (int g : {1, 2, 4}) c.push_back({Sm, g, false});
+        if (Sm > 1) {
+            for (int g : {1, 2, 4}) c.push_back({Sm, g, false, false});
+            c.push_back({Sm, 0, false, false});   // tile-streamed split
+            if (f32) c.push_back({Sm, 0, false, true});
+        }
         return c;
     };
     std::vector<StepCfg> cA = base(tuneA, prm.lanes_A, avgA, SmA, prm.col_splits_A);
     std::vector<StepCfg> cAt = base(tuneAt, prm.lanes_At, avgAt, SmAt, prm.col_splits_At);
-    auto apply = [](const StepCfg& c, int& S, int& G, int& Gs, bool& stream) {
+    auto apply = [](const StepCfg& c, int& S, int& G, int& Gs, bool& stream, bool& rec) {
         S = c.S;
         stream = c.S == 1 && c.stream;
+        rec = c.rec;
         if (c.S > 1) Gs = c.G;
         else G = std::max(c.G, 2);
     };
-    apply(cA[0], E.SA, E.GA, E.GsA, E.streamA);
-    apply(cAt[0], E.SAt, E.GAt, E.GsAt, E.streamAt);
+    apply(cA[0], E.SA, E.GA, E.GsA, E.streamA, E.recA);
+    apply(cAt[0], E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt);
     if (E.nnz < 200000 || (cA.size() == 1 && cAt.size() == 1))
         return VJ_OK;  // launch-bound sizes: the first candidate is as good as timing
     const size_t rounds = std::max(cA.size(), cAt.size());
@@ -853,15 +947,18 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     for (size_t r = 0; r < rounds; ++r) {
         const StepCfg& a = cA[std::min(r, cA.size() - 1)];
         const StepCfg& b = cAt[std::min(r, cAt.size() - 1)];
-        apply(a, E.SA, E.GA, E.GsA, E.streamA);
-        apply(b, E.SAt, E.GAt, E.GsAt, E.streamAt);
+        apply(a, E.SA, E.GA, E.GsA, E.streamA, E.recA);
+        apply(b, E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt);
         double up = 0, ud = 0;
         VJ_TRY((time_kernels<V, T>(ctx, E, av, tv, 3, &up, &ud)));
+        if (prm.verbose >= 2)
+            std::fprintf(stderr, "[autotune] A: S=%d G=%d stream=%d rec=%d %.1f us | A^T: S=%d G=%d stream=%d rec=%d %.1f us\n",
+                         a.S, a.G, a.stream ? 1 : 0, a.rec ? 1 : 0, ud, b.S, b.G, b.stream ? 1 : 0, b.rec ? 1 : 0, up);
         if (ud < bestA) { bestA = ud; wA = a; }
         if (up < bestAt) { bestAt = up; wAt = b; }
     }
-    apply(wA, E.SA, E.GA, E.GsA, E.streamA);
-    apply(wAt, E.SAt, E.GAt, E.GsAt, E.streamAt);
+    apply(wA, E.SA, E.GA, E.GsA, E.streamA, E.recA);
+    apply(wAt, E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt);
     return VJ_OK;
 }
 
@@ -1110,11 +1207,15 @@ int fast_benchmark(vj_ctx* ctx, int reps, vj_kernel_stat* out, int cap, int* wri
     // names carry the kernel variant the autotuner chose, as the profiler lists it
     const char* pt = E.prec == VJ_PREC_FP64 ? "double" : (E.prec == VJ_PREC_FP32 ? "float" : "bf16");
     char names[2][48];
-    if (E.SAt > 1) std::snprintf(names[0], 48, "split_partials+primal_finish<%s,G=%d>", pt, E.GsAt);
-    else if (E.streamAt) std::snprintf(names[0], 48, "stream_primal_step<%s>", pt);
+    const char* ra = E.recA ? ",rec" : "";
+    const char* rt = E.recAt ? ",rec" : "";
+    if (E.SAt > 1 && E.GsAt == 0) std::snprintf(names[0], 48, "split_stream_partials+primal_finish<%s%s>", pt, rt);
+    else if (E.SAt > 1) std::snprintf(names[0], 48, "split_partials+primal_finish<%s,G=%d>", pt, E.GsAt);
+    else if (E.streamAt) std::snprintf(names[0], 48, "stream_primal_step<%s%s>", pt, rt);
     else std::snprintf(names[0], 48, "primal_step<%s,G=%d>", pt, E.GAt);
-    if (E.SA > 1) std::snprintf(names[1], 48, "split_partials+dual_finish<%s,G=%d>", pt, E.GsA);
-    else if (E.streamA) std::snprintf(names[1], 48, "stream_dual_step<%s>", pt);
+    if (E.SA > 1 && E.GsA == 0) std::snprintf(names[1], 48, "split_stream_partials+dual_finish<%s%s>", pt, ra);
+    else if (E.SA > 1) std::snprintf(names[1], 48, "split_partials+dual_finish<%s,G=%d>", pt, E.GsA);
+    else if (E.streamA) std::snprintf(names[1], 48, "stream_dual_step<%s%s>", pt, ra);
     else std::snprintf(names[1], 48, "dual_step<%s,G=%d>", pt, E.GA);
     const double us[2] = {up, ud};
     const double bytes[2] = {b_primal, b_dual};

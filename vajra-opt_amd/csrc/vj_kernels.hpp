@@ -134,20 +134,30 @@ __global__ __launch_bounds__(kBlock) void seg_finish(const double* __restrict__ 
 
 // Primal epilogue for column j given acc = (A^T y)_j: projection, extrapolation, reflected
 // Halpern step, residual contributions c[] at measured iterations.
+// Row-owned operands of the primal update, loaded separately so that the tile-streamed
+// kernels can issue them before their gathers (the loads then overlap the SpMV).
 template <class T>
-__device__ __forceinline__ void primal_epilogue(T acc, int j, const InstState& s, int it, int flags, double rho,
-                                                const T* __restrict__ dhat, const T* __restrict__ lcs,
-                                                const T* __restrict__ ucs, T* __restrict__ x, T* __restrict__ xbar,
-                                                const double* __restrict__ xhat, double* __restrict__ xfull,
-                                                double (&c)[3]) {
+struct PrimalOps {
+    T xk, dh, lo, hi;
+};
+template <class T>
+__device__ __forceinline__ PrimalOps<T> primal_ops(int j, const T* __restrict__ x, const T* __restrict__ dhat,
+                                                   const T* __restrict__ lcs, const T* __restrict__ ucs) {
+    return PrimalOps<T>{x[j], dhat[j], lcs[j], ucs[j]};
+}
+
+template <class T>
+__device__ __forceinline__ void primal_update(T acc, const PrimalOps<T>& o, int j, const InstState& s, int it,
+                                              int flags, double rho, T* __restrict__ x, T* __restrict__ xbar,
+                                              const double* __restrict__ xhat, double* __restrict__ xfull,
+                                              double (&c)[3]) {
     const T tau = static_cast<T>(s.tau);
     const double kk = static_cast<double>(s.k + it);
     const T lam = static_cast<T>((kk + 1.0) / (kk + 2.0));
-    const T xk = x[j];
-    T xt = xk - tau * (dhat[j] - acc);
-    const T lo = lcs[j], hi = ucs[j];
-    xt = xt < lo ? lo : xt;
-    xt = xt > hi ? hi : xt;
+    const T xk = o.xk;
+    T xt = xk - tau * (o.dh - acc);
+    xt = xt < o.lo ? o.lo : xt;
+    xt = xt > o.hi ? o.hi : xt;
     xbar[j] = static_cast<T>(2) * xt - xk;
     x[j] = lam * ((static_cast<T>(1) + static_cast<T>(rho)) * xt - static_cast<T>(rho) * xk);
     const double d = static_cast<double>(xt) - static_cast<double>(xk);
@@ -159,26 +169,41 @@ __device__ __forceinline__ void primal_epilogue(T acc, int j, const InstState& s
     }
 }
 
+template <class T>
+__device__ __forceinline__ void primal_epilogue(T acc, int j, const InstState& s, int it, int flags, double rho,
+                                                const T* __restrict__ dhat, const T* __restrict__ lcs,
+                                                const T* __restrict__ ucs, T* __restrict__ x, T* __restrict__ xbar,
+                                                const double* __restrict__ xhat, double* __restrict__ xfull,
+                                                double (&c)[3]) {
+    primal_update<T>(acc, primal_ops<T>(j, x, dhat, lcs, ucs), j, s, it, flags, rho, x, xbar, xhat, xfull, c);
+}
+
 // Dual epilogue for row i given w = (A x_bar)_i: prox of the row-bound support function
 // written per active side, y+ = y + sigma*(proj_[l,u](w - y/sigma) - w) in full
 // coordinates, then the reflected Halpern step.
 template <class T>
-__device__ __forceinline__ void dual_epilogue(T w, int i, const InstState& s, int it, int flags, double rho,
-                                              const T* __restrict__ yhat_t, const T* __restrict__ lrs,
-                                              const T* __restrict__ urs, T* __restrict__ y,
-                                              const double* __restrict__ yhat, double* __restrict__ yfull,
-                                              double (&c)[3]) {
+struct DualOps {
+    T yk, yh, lo, hi;
+};
+template <class T>
+__device__ __forceinline__ DualOps<T> dual_ops(int i, const T* __restrict__ y, const T* __restrict__ yhat_t,
+                                               const T* __restrict__ lrs, const T* __restrict__ urs) {
+    return DualOps<T>{y[i], yhat_t[i], lrs[i], urs[i]};
+}
+
+template <class T>
+__device__ __forceinline__ void dual_update(T w, const DualOps<T>& o, int i, const InstState& s, int it, int flags,
+                                            double rho, T* __restrict__ y, const double* __restrict__ yhat,
+                                            double* __restrict__ yfull, double (&c)[3]) {
     const T sig = static_cast<T>(s.sigma);
     const double kk = static_cast<double>(s.k + it);
     const T lam = static_cast<T>((kk + 1.0) / (kk + 2.0));
-    const T yk = y[i];
-    const T yh = yhat_t[i];
-    const T lo = lrs[i], hi = urs[i];
-    const T sfull = yh + yk;
+    const T yk = o.yk;
+    const T sfull = o.yh + yk;
     T yt;
-    if (sig * (w - lo) < sfull) yt = yk + sig * (lo - w);
-    else if (sig * (w - hi) > sfull) yt = yk + sig * (hi - w);
-    else yt = -yh;
+    if (sig * (w - o.lo) < sfull) yt = yk + sig * (o.lo - w);
+    else if (sig * (w - o.hi) > sfull) yt = yk + sig * (o.hi - w);
+    else yt = -o.yh;
     y[i] = lam * ((static_cast<T>(1) + static_cast<T>(rho)) * yt - static_cast<T>(rho) * yk);
     const double d = static_cast<double>(yt) - static_cast<double>(yk);
     if (flags & 1) c[0] = d * d;
@@ -187,6 +212,15 @@ __device__ __forceinline__ void dual_epilogue(T w, int i, const InstState& s, in
         c[2] = static_cast<double>(yt) * static_cast<double>(yt);
         yfull[i] = yhat[i] + static_cast<double>(yt);
     }
+}
+
+template <class T>
+__device__ __forceinline__ void dual_epilogue(T w, int i, const InstState& s, int it, int flags, double rho,
+                                              const T* __restrict__ yhat_t, const T* __restrict__ lrs,
+                                              const T* __restrict__ urs, T* __restrict__ y,
+                                              const double* __restrict__ yhat, double* __restrict__ yfull,
+                                              double (&c)[3]) {
+    dual_update<T>(w, dual_ops<T>(i, y, yhat_t, lrs, urs), i, s, it, flags, rho, y, yhat, yfull, c);
 }
 
 template <class V, class T, int G, bool MULTI>
@@ -271,32 +305,62 @@ __global__ __launch_bounds__(kBlock) void dual_step(
 // epilogue. A single-row tile longer than kStreamCap is reduced by the whole workgroup.
 constexpr int kStreamCap = 4096;
 
-template <class V, class T>
-__device__ __forceinline__ void stream_products(int s0, int s1, const V* __restrict__ val, const int* __restrict__ idx,
-                                                const T* __restrict__ vec, T* __restrict__ lds) {
+// Entry streams of a CSR matrix as the tile-streamed kernels read them: separate index and
+// value arrays, or (FP32 values) interleaved 8-byte (index, value) records, one load per
+// entry instead of two.
+template <class V>
+struct SepEntries {
+    const int* __restrict__ idx;
+    const V* __restrict__ val;
+    template <class T>
+    __device__ __forceinline__ void load(int64_t k, int& c, T& a) const {
+        c = load_idx(idx, k);
+        a = static_cast<T>(load_val(val, k));
+    }
+};
+typedef int vj_i32x2 __attribute__((ext_vector_type(2)));
+struct RecEntries {
+    const vj_i32x2* __restrict__ rec;
+    template <class T>
+    __device__ __forceinline__ void load(int64_t k, int& c, T& a) const {
+        const vj_i32x2 r = __builtin_nontemporal_load(rec + k);
+        c = r.x;
+        a = static_cast<T>(__int_as_float(r.y));
+    }
+};
+
+template <class Ent, class T>
+__device__ __forceinline__ void stream_products(int s0, int s1, const Ent& ent, const T* __restrict__ vec,
+                                                T* __restrict__ lds) {
     constexpr int U = 4;
     int p = s0 + static_cast<int>(threadIdx.x);
     for (; p + (U - 1) * kBlock < s1; p += U * kBlock) {
         int c[U];
         T a[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            c[u] = load_idx(idx, p + u * kBlock);
-            a[u] = static_cast<T>(load_val(val, p + u * kBlock));
-        }
+        for (int u = 0; u < U; ++u) ent.load(p + u * kBlock, c[u], a[u]);
 #pragma unroll
         for (int u = 0; u < U; ++u) lds[p + u * kBlock - s0] = a[u] * vec[c[u]];
     }
-    for (; p < s1; p += kBlock) lds[p - s0] = static_cast<T>(load_val(val, p)) * vec[load_idx(idx, p)];
+    for (; p < s1; p += kBlock) {
+        int c;
+        T a;
+        ent.load(p, c, a);
+        lds[p - s0] = a * vec[c];
+    }
 }
 
 // Whole-workgroup sum of one long row (fixed order: strided lane sums, then a tree).
-template <class V, class T>
-__device__ __forceinline__ T long_row_sum(int s0, int s1, const V* __restrict__ val, const int* __restrict__ idx,
-                                          const T* __restrict__ vec, T* __restrict__ lds) {
+template <class Ent, class T>
+__device__ __forceinline__ T long_row_sum(int s0, int s1, const Ent& ent, const T* __restrict__ vec,
+                                          T* __restrict__ lds) {
     T acc = 0;
-    for (int p = s0 + static_cast<int>(threadIdx.x); p < s1; p += kBlock)
-        acc += static_cast<T>(load_val(val, p)) * vec[load_idx(idx, p)];
+    for (int p = s0 + static_cast<int>(threadIdx.x); p < s1; p += kBlock) {
+        int c;
+        T a;
+        ent.load(p, c, a);
+        acc += a * vec[c];
+    }
     lds[threadIdx.x] = acc;
     __syncthreads();
     for (int w = kBlock / 2; w > 0; w >>= 1) {
@@ -308,10 +372,9 @@ __device__ __forceinline__ T long_row_sum(int s0, int s1, const V* __restrict__ 
     return r;
 }
 
-template <class V, class T, bool MULTI>
+template <class Ent, class T, bool MULTI>
 __global__ __launch_bounds__(kBlock) void stream_primal_step(
-    int n, const int* __restrict__ tile_row, const int* __restrict__ tp, const int* __restrict__ ti,
-    const V* __restrict__ tv, const T* __restrict__ y, const T* __restrict__ dhat, const T* __restrict__ lcs,
+    int n, const int* __restrict__ tile_row, const int* __restrict__ tp, Ent ent, const T* __restrict__ y, const T* __restrict__ dhat, const T* __restrict__ lcs,
     const T* __restrict__ ucs, T* __restrict__ x, T* __restrict__ xbar, const double* __restrict__ xhat,
     double* __restrict__ xfull, const InstState* __restrict__ st, const int* __restrict__ col_inst,
     double* __restrict__ part, int it, int flags, double rho) {
@@ -331,17 +394,25 @@ __global__ __launch_bounds__(kBlock) void stream_primal_step(
     const int my = (MULTI && j < r1) ? col_inst[j] : 0;
     double c[3] = {0.0, 0.0, 0.0};
     if (s1 - s0 > kStreamCap) {  // one long row
-        const T acc = long_row_sum<V, T>(s0, s1, tv, ti, y, lds);
+        const T acc = long_row_sum<Ent, T>(s0, s1, ent, y, lds);
         if (threadIdx.x == 0 && (!MULTI || st[my].active != 0))
             primal_epilogue<T>(acc, r0, st[my], it, flags, rho, dhat, lcs, ucs, x, xbar, xhat, xfull, c);
     } else {
-        stream_products<V, T>(s0, s1, tv, ti, y, lds);
+        // the row's operands and segment bounds are loaded before the gathers
+        const bool mine = j < r1 && (!MULTI || st[my].active != 0);
+        PrimalOps<T> o{};
+        int q0 = 0, q1 = 0;
+        if (mine) {
+            o = primal_ops<T>(j, x, dhat, lcs, ucs);
+            q0 = tp[j] - s0;
+            q1 = tp[j + 1] - s0;
+        }
+        stream_products<Ent, T>(s0, s1, ent, y, lds);
         __syncthreads();
-        if (j < r1 && (!MULTI || st[my].active != 0)) {
+        if (mine) {
             T acc = 0;
-            const int e = tp[j + 1] - s0;
-            for (int q = tp[j] - s0; q < e; ++q) acc += lds[q];
-            primal_epilogue<T>(acc, j, st[my], it, flags, rho, dhat, lcs, ucs, x, xbar, xhat, xfull, c);
+            for (int q = q0; q < q1; ++q) acc += lds[q];
+            primal_update<T>(acc, o, j, st[my], it, flags, rho, x, xbar, xhat, xfull, c);
         }
     }
     if (flags) {
@@ -350,10 +421,9 @@ __global__ __launch_bounds__(kBlock) void stream_primal_step(
     }
 }
 
-template <class V, class T, bool MULTI>
+template <class Ent, class T, bool MULTI>
 __global__ __launch_bounds__(kBlock) void stream_dual_step(
-    int m, const int* __restrict__ tile_row, const int* __restrict__ rp, const int* __restrict__ ci,
-    const V* __restrict__ av, const T* __restrict__ xbar, const T* __restrict__ yhat_t, const T* __restrict__ lrs,
+    int m, const int* __restrict__ tile_row, const int* __restrict__ rp, Ent ent, const T* __restrict__ xbar, const T* __restrict__ yhat_t, const T* __restrict__ lrs,
     const T* __restrict__ urs, T* __restrict__ y, const double* __restrict__ yhat, double* __restrict__ yfull,
     const InstState* __restrict__ st, const int* __restrict__ row_inst, double* __restrict__ part, int it, int flags,
     double rho) {
@@ -373,17 +443,25 @@ __global__ __launch_bounds__(kBlock) void stream_dual_step(
     const int my = (MULTI && i < r1) ? row_inst[i] : 0;
     double c[3] = {0.0, 0.0, 0.0};
     if (s1 - s0 > kStreamCap) {
-        const T w = long_row_sum<V, T>(s0, s1, av, ci, xbar, lds);
+        const T w = long_row_sum<Ent, T>(s0, s1, ent, xbar, lds);
         if (threadIdx.x == 0 && (!MULTI || st[my].active != 0))
             dual_epilogue<T>(w, r0, st[my], it, flags, rho, yhat_t, lrs, urs, y, yhat, yfull, c);
     } else {
-        stream_products<V, T>(s0, s1, av, ci, xbar, lds);
+        // the row's operands and segment bounds are loaded before the gathers
+        const bool mine = i < r1 && (!MULTI || st[my].active != 0);
+        DualOps<T> o{};
+        int q0 = 0, q1 = 0;
+        if (mine) {
+            o = dual_ops<T>(i, y, yhat_t, lrs, urs);
+            q0 = rp[i] - s0;
+            q1 = rp[i + 1] - s0;
+        }
+        stream_products<Ent, T>(s0, s1, ent, xbar, lds);
         __syncthreads();
-        if (i < r1 && (!MULTI || st[my].active != 0)) {
+        if (mine) {
             T w = 0;
-            const int e = rp[i + 1] - s0;
-            for (int q = rp[i] - s0; q < e; ++q) w += lds[q];
-            dual_epilogue<T>(w, i, st[my], it, flags, rho, yhat_t, lrs, urs, y, yhat, yfull, c);
+            for (int q = q0; q < q1; ++q) w += lds[q];
+            dual_update<T>(w, o, i, st[my], it, flags, rho, y, yhat, yfull, c);
         }
     }
     if (flags) {
@@ -418,6 +496,54 @@ __global__ __launch_bounds__(kBlock) void split_partials(int rows, int S, const 
     }
     acc = group_sum<G>(acc);
     if (r < rows && lane == 0) part[static_cast<int64_t>(k) * rows + r] = acc;
+}
+
+// Tile-streamed column-split partials. Workgroup w handles block k = w % S (the same XCD
+// mapping as split_partials) and row tile t = w / S. The host builds tiles common to all
+// blocks: at most kSplitRows rows, and at most kStreamCap entries in every block; a single
+// row longer than that in some block is a tile of its own, reduced by the whole workgroup.
+// The block's span of values and indices is streamed with coalesced loads independent of
+// the row structure, the products go to LDS, and each thread adds up to kSplitRows/256 rows
+// of the tile in index order (deterministic).
+constexpr int kSplitRows = 1024;
+
+template <class Ent, class T>
+__global__ __launch_bounds__(kBlock) void split_stream_partials(int rows, int S, const int* __restrict__ tile_row,
+                                                                const int* __restrict__ bp, Ent ent,
+                                                                const T* __restrict__ vec, T* __restrict__ part,
+                                                                const InstState* __restrict__ st) {
+    __shared__ T lds[kStreamCap];
+    if (st[0].active == 0) return;
+    const int k = static_cast<int>(blockIdx.x) % S;
+    const int t = static_cast<int>(blockIdx.x) / S;
+    const int r0 = tile_row[t], r1 = tile_row[t + 1];
+    const int* p = bp + static_cast<int64_t>(k) * (rows + 1);
+    T* out = part + static_cast<int64_t>(k) * rows;
+    const int s0 = p[r0], s1 = p[r1];
+    if (s1 - s0 > kStreamCap) {
+        const T acc = long_row_sum<Ent, T>(s0, s1, ent, vec, lds);
+        if (threadIdx.x == 0) out[r0] = acc;
+        return;
+    }
+    constexpr int RPT = kSplitRows / kBlock;
+    int lo[RPT], hi[RPT];
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) {
+        const int r = r0 + static_cast<int>(threadIdx.x) + u * kBlock;
+        lo[u] = r < r1 ? p[r] - s0 : 0;
+        hi[u] = r < r1 ? p[r + 1] - s0 : 0;
+    }
+    stream_products<Ent, T>(s0, s1, ent, vec, lds);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) {
+        const int r = r0 + static_cast<int>(threadIdx.x) + u * kBlock;
+        if (r < r1) {
+            T acc = 0;
+            for (int q = lo[u]; q < hi[u]; ++q) acc += lds[q];
+            out[r] = acc;
+        }
+    }
 }
 
 template <class T>
@@ -481,6 +607,17 @@ __global__ void gather_split_vals(int64_t nnz, const int* __restrict__ perm, con
 // FP64 KKT passes (the check, and the audit on original data when the unscale arrays
 // are null). Row pass: ax = A x; q0 = sum r_p^2 (original units), q1 = row part of the
 // dual objective, q2 = row-dual sign violations^2 (original units).
+// Interleave an index array and an FP32 value array into 8-byte (index, value) records.
+__global__ void pack_records(int64_t nnz, const int* __restrict__ idx, const float* __restrict__ val,
+                             vj_i32x2* __restrict__ rec) {
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= nnz) return;
+    vj_i32x2 r;
+    r.x = idx[q];
+    r.y = __float_as_int(val[q]);
+    rec[q] = r;
+}
+
 template <int G, bool MULTI>
 __global__ __launch_bounds__(kBlock) void kkt_rows(
     int m, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ av,
