@@ -168,7 +168,8 @@ struct FastEngine {
     DevBuf bpA, idxA, valA, psumA, bpAt, idxAt, valAt, psumAt, permA, permAt;
     // row tiles of the split layouts for split_stream_partials (Gs = 0 selects that kernel)
     DevBuf tileSA, tileSAt;
-    int ntSA = 0, ntSAt = 0;
+    int ntSA = 0, ntSAt = 0, tcapSA = 0, tcapSAt = 0;   // tile count; LDS entries the tiles fit
+    std::vector<int32_t> h_bpA, h_bpAt;                  // host copies of the block pointers
     // FP32: interleaved (index, value) records of A, A^T and of the split layouts, read by
     // the tile-streamed kernels when the autotuner picks them (recA / recAt)
     DevBuf recA_buf, recAt_buf, recSA_buf, recSAt_buf;
@@ -491,14 +492,14 @@ void build_stream_tiles(int rows, const std::vector<int64_t>& rp, const std::vec
 }
 
 // Row tiles of a split layout (S blocks, absolute pointers bp[k*(rows+1) + r]) for the
-// tile-streamed split kernel: at most kSplitRows rows and at most kStreamCap entries in
-// every block; a row longer than kStreamCap in some block is a tile of its own.
-void build_split_tiles(int rows, int S, const std::vector<int32_t>& bp, std::vector<int32_t>& tile_row) {
+// tile-streamed split kernel: at most kSplitRows rows and at most `cap` entries in every
+// block; a row longer than `cap` in some block is a tile of its own.
+void build_split_tiles(int rows, int S, int cap, const std::vector<int32_t>& bp, std::vector<int32_t>& tile_row) {
     tile_row.assign(1, 0);
     auto fits = [&](int r, int e) {
         for (int k = 0; k < S; ++k) {
             const int32_t* b = bp.data() + static_cast<size_t>(k) * (rows + 1);
-            if (b[e] - b[r] > kStreamCap) return false;
+            if (b[e] - b[r] > cap) return false;
         }
         return true;
     };
@@ -727,17 +728,14 @@ int setup(vj_ctx* ctx, FastEngine& E) {
 // are gathered on the device from the scaled FP64 matrix in the working precision.
 int setup_one_split(vj_ctx* ctx, FastEngine& E, int S, int rows, int cols, const std::vector<int64_t>& rp,
                     const std::vector<int32_t>& ci, const double* vals_dev, int& built, DevBuf& bp, DevBuf& idx,
-                    DevBuf& perm, DevBuf& val, DevBuf& psum, DevBuf& tiles, int& ntiles, DevBuf& rec) {
+                    DevBuf& perm, DevBuf& val, DevBuf& psum, DevBuf& tiles, int& ntiles, DevBuf& rec,
+                    std::vector<int32_t>& hbp, int& tiles_cap) {
     hipStream_t s = ctx->stream;
     const int64_t nnz = rp[rows];
     if (built != S) {
-        std::vector<int32_t> hbp, hidx, hperm;
+        std::vector<int32_t> hidx, hperm;
         build_split(rows, cols, rp, ci, S, hbp, hidx, hperm);
-        std::vector<int32_t> htile;
-        build_split_tiles(rows, S, hbp, htile);
-        ntiles = static_cast<int>(htile.size()) - 1;
-        VJ_TRY(tiles.alloc(htile.size() * 4));
-        VJ_HIP(hipMemcpy(tiles.p, htile.data(), htile.size() * 4, hipMemcpyHostToDevice));
+        tiles_cap = 0;
         VJ_TRY(bp.alloc(hbp.size() * 4));
         VJ_TRY(idx.alloc(static_cast<size_t>(nnz) * 4));
         VJ_TRY(perm.alloc(static_cast<size_t>(nnz) * 4));
@@ -745,6 +743,16 @@ int setup_one_split(vj_ctx* ctx, FastEngine& E, int S, int rows, int cols, const
         VJ_HIP(hipMemcpy(idx.p, hidx.data(), static_cast<size_t>(nnz) * 4, hipMemcpyHostToDevice));
         VJ_HIP(hipMemcpy(perm.p, hperm.data(), static_cast<size_t>(nnz) * 4, hipMemcpyHostToDevice));
         built = S;
+    }
+    // tiles depend on the LDS capacity in entries of the working precision
+    const int cap = E.prec == VJ_PREC_FP64 ? split_cap<double>() : split_cap<float>();
+    if (tiles_cap != cap) {
+        std::vector<int32_t> htile;
+        build_split_tiles(rows, S, cap, hbp, htile);
+        ntiles = static_cast<int>(htile.size()) - 1;
+        VJ_TRY(tiles.alloc(htile.size() * 4));
+        VJ_HIP(hipMemcpy(tiles.p, htile.data(), htile.size() * 4, hipMemcpyHostToDevice));
+        tiles_cap = cap;
     }
     const size_t vsz = E.prec == VJ_PREC_FP64 ? 8 : (E.prec == VJ_PREC_FP32 ? 4 : 2);
     VJ_TRY(val.alloc(static_cast<size_t>(nnz) * vsz));
@@ -773,10 +781,12 @@ int setup_splits(vj_ctx* ctx, FastEngine& E) {
     E.SAt = prm.col_splits_At ? prm.col_splits_At : choose_splits(E.m);
     if (E.SA > 1)
         VJ_TRY(setup_one_split(ctx, E, E.SA, E.m, E.n, P.h_rp, P.h_ci, E.av_s.as<double>(), E.builtSA, E.bpA, E.idxA,
-                               E.permA, E.valA, E.psumA, E.tileSA, E.ntSA, E.recSA_buf));
+                               E.permA, E.valA, E.psumA, E.tileSA, E.ntSA, E.recSA_buf, E.h_bpA,
+                               E.tcapSA));
     if (E.SAt > 1)
         VJ_TRY(setup_one_split(ctx, E, E.SAt, E.n, E.m, P.h_tp, P.h_ti, E.tv_s.as<double>(), E.builtSAt, E.bpAt,
-                               E.idxAt, E.permAt, E.valAt, E.psumAt, E.tileSAt, E.ntSAt, E.recSAt_buf));
+                               E.idxAt, E.permAt, E.valAt, E.psumAt, E.tileSAt, E.ntSAt, E.recSAt_buf,
+                               E.h_bpAt, E.tcapSAt));
     // split partial kernels see ~nnz/(rows*S) entries per row: few lanes per row
     E.GsA = (static_cast<double>(E.nnz) / std::max(1, E.m) / E.SA) > 3.0 ? 4 : 2;
     E.GsAt = (static_cast<double>(E.nnz) / std::max(1, E.n) / E.SAt) > 3.0 ? 4 : 2;

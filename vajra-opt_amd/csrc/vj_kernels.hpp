@@ -500,19 +500,28 @@ __global__ __launch_bounds__(kBlock) void split_partials(int rows, int S, const 
 
 // Tile-streamed column-split partials. Workgroup w handles block k = w % S (the same XCD
 // mapping as split_partials) and row tile t = w / S. The host builds tiles common to all
-// blocks: at most kSplitRows rows, and at most kStreamCap entries in every block; a single
-// row longer than that in some block is a tile of its own, reduced by the whole workgroup.
+// blocks: at most kSplitRows rows, and at most split_cap<T>() entries in every block; a
+// single row longer than that in some block is a tile of its own, reduced by the whole
+// workgroup.
 // The block's span of values and indices is streamed with coalesced loads independent of
 // the row structure, the products go to LDS, and each thread adds up to kSplitRows/256 rows
 // of the tile in index order (deterministic).
 constexpr int kSplitRows = 1024;
+#ifndef VJ_SPLIT_LDS_BYTES
+#define VJ_SPLIT_LDS_BYTES 16384
+#endif
+// LDS product buffer of one split tile: 16 KB (4096 FP32 / 2048 FP64 entries), so that
+// eight workgroups of four waves fit a CU in either precision.
+constexpr int kSplitLdsBytes = VJ_SPLIT_LDS_BYTES;
+template <class T>
+constexpr int split_cap() { return kSplitLdsBytes / static_cast<int>(sizeof(T)); }
 
 template <class Ent, class T>
 __global__ __launch_bounds__(kBlock) void split_stream_partials(int rows, int S, const int* __restrict__ tile_row,
                                                                 const int* __restrict__ bp, Ent ent,
                                                                 const T* __restrict__ vec, T* __restrict__ part,
                                                                 const InstState* __restrict__ st) {
-    __shared__ T lds[kStreamCap];
+    __shared__ T lds[split_cap<T>()];
     if (st[0].active == 0) return;
     const int k = static_cast<int>(blockIdx.x) % S;
     const int t = static_cast<int>(blockIdx.x) / S;
@@ -520,7 +529,7 @@ __global__ __launch_bounds__(kBlock) void split_stream_partials(int rows, int S,
     const int* p = bp + static_cast<int64_t>(k) * (rows + 1);
     T* out = part + static_cast<int64_t>(k) * rows;
     const int s0 = p[r0], s1 = p[r1];
-    if (s1 - s0 > kStreamCap) {
+    if (s1 - s0 > split_cap<T>()) {
         const T acc = long_row_sum<Ent, T>(s0, s1, ent, vec, lds);
         if (threadIdx.x == 0) out[r0] = acc;
         return;
