@@ -2,11 +2,13 @@
 // of the reference: reversible reductions pushed on a LIFO stack with index maps between the
 // reduced and the original model, and a postsolve that rebuilds primal and dual values).
 //
-// Reductions applied here: empty rows, empty columns, fixed columns, singleton rows (turned
-// into column bounds) and redundant rows (activity bounds inside the row bounds). The
-// reference's forcing-row, free-column-singleton and doubleton-row reductions are not
-// applied (their record types are kept so stacks stay interchangeable). Every pass is O(nnz)
-// over CSR and CSC arrays.
+// All eight reductions of the reference are applied, in its order within a pass (empty rows,
+// empty columns, fixed columns, singleton rows, redundant and forcing rows, free column
+// singletons, doubleton equality rows), to a fixed point or max_passes passes, with its tests
+// and tolerances; the reduced model, the stack and the postsolved solution are those of the
+// reference (src/presolve/presolve.cpp:10-976). Kept reference behaviours: an empty column's
+// bound is chosen by its raw cost, also for maximisation; a row touched by a doubleton
+// substitution is never a doubleton candidate again in the same call.
 //
 // Dual convention: duals are those of the canonical minimisation (c' = sense * c), with
 // y_i > 0 pricing the row's lower bound and d = c' - A^T y.
@@ -89,6 +91,8 @@ public:
     static PresolveResult apply(const Model& original_model, int max_passes = 10);
     static Solution postsolve(const Solution& reduced_solution, const PresolveStack& stack,
                               const Model& original_model);
+    // col_dual = reduced costs of the row duals on the original model (tol is unused)
+    static void refine_dual_solution(const Model& original_model, Solution& solution, double tol = 1e-6);
 };
 
 }  // namespace indus::presolve

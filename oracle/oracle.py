@@ -212,3 +212,13 @@ def ref_read_mps(path):
 def ref_read_lp(path):
     """The reference's LP reader (src/io/lp_reader.cpp), compiled from its sources."""
     return _ref_read(ref_lib().ref_read_lp, path)
+
+
+def ref_presolve_dump(path, out_path):
+    """The reference's PresolveEngine::apply on a model file read by its own reader, then its
+    postsolve of the synthetic reduced solution, written as the canonical text dump of
+    tests/cpp/presolve_dump.hpp (src/presolve/presolve.cpp:10-976)."""
+    lp = 1 if str(path).endswith(".lp") else 0
+    rc = ref_lib().ref_presolve_dump(str(path).encode(), _c_int(lp), str(out_path).encode())
+    if rc != 0:
+        raise OSError(f"ref_presolve_dump failed on {path}")

@@ -10,7 +10,10 @@
 //   indus::la::RuizScaler::compute_and_scale  src/linalg/scaling.cpp:7
 //   indus::la::SparseMatrixCSC::multiply / multiply_transpose  src/linalg/sparse.cpp:196,219
 //   indus::io::read_mps                   src/io/mps_reader.cpp:48
+//   indus::presolve::PresolveEngine::apply / postsolve  src/presolve/presolve.cpp:10,649
 //   indus::verifier::verify_solution      src/engine/verifier.cpp:25
+#include "../tests/cpp/presolve_dump.hpp"
+#include "indus/presolve.hpp"
 #include "indus/model.hpp"
 #include "indus/pdhg.hpp"
 #include "indus/io.hpp"
@@ -236,6 +239,22 @@ int ref_quality(int m, int n, const int64_t* col_ptr, const int* row_idx, const 
     out[2] = sol.quality.max_complementarity_violation;
     out[3] = sol.quality.is_primal_feasible; out[4] = sol.quality.is_dual_feasible;
     if (col_dual_out) std::memcpy(col_dual_out, sol.col_dual.data(), sizeof(double) * n);
+    return 0;
+}
+
+// Presolve of a model file read by the reference's reader, then the postsolve of the
+// synthetic reduced solution, both written as the canonical dump of presolve_dump.hpp.
+int ref_presolve_dump(const char* path, int lp, const char* out_path) {
+    indus::Model model = lp ? indus::io::read_lp(path) : indus::io::read_mps(path);
+    auto res = indus::presolve::PresolveEngine::apply(model);
+    FILE* f = std::fopen(out_path, "w");
+    if (!f) return 1;
+    presolve_dump::dump_result(f, res);
+    if (!res.is_infeasible && !res.is_unbounded) {
+        auto red = presolve_dump::synthetic_reduced<indus::Solution>(res.presolved_model, indus::SolveStatus::kOptimal);
+        presolve_dump::dump_solution(f, indus::presolve::PresolveEngine::postsolve(red, res.stack, model));
+    }
+    std::fclose(f);
     return 0;
 }
 

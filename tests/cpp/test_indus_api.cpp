@@ -28,6 +28,7 @@
 #include "indus/pdhg.hpp"
 #include "indus/presolve.hpp"
 #include "indus/verifier.hpp"
+#include "presolve_dump.hpp"
 #include "vajra_lpgen.h"
 
 namespace {
@@ -159,9 +160,12 @@ void test_presolve_handoff() {
     const indus::Model M = make(2, 3, {-1, -1, 2}, {0, 0, 5}, {1e20, 1e20, 5}, {-1e20, -1e20}, {4, 10},
                                 {{0, 0, 1.0}, {0, 1, 1.0}, {1, 1, 1.0}});
     const auto pr = indus::presolve::PresolveEngine::apply(M);
-    CHECK(pr.num_fixed_cols == 1);
-    CHECK(pr.total_reductions() >= 2);
-    CHECK(pr.presolved_model.num_cols == 2);
+    // x3 (fixed at 5) has no entries: the empty-column step, which runs before the
+    // fixed-column step, takes it; the row x2 <= 10 becomes a bound (presolve.cpp:109-245)
+    CHECK(pr.num_empty_cols == 1 && pr.num_fixed_cols == 0);
+    CHECK(pr.num_singleton_rows == 1);
+    CHECK(pr.total_reductions() == 2);
+    CHECK(pr.presolved_model.num_cols == 2 && pr.presolved_model.num_rows == 1);
     indus::Options o = pdhg_options();
     o.enable_presolve = true;
     const indus::Solution s = indus::solve(M, o);
@@ -637,8 +641,27 @@ int dump_model(int argc, char** argv) {
     return 0;
 }
 
+// presolve_dump <model file> <mps|lp> <out>: this build's presolve and postsolve, written as
+// the canonical dump the oracle's shim writes for the reference (tests/test_presolve.py)
+int presolve_dump_mode(int argc, char** argv) {
+    if (argc < 5) return 2;
+    const std::string kind = argv[3];
+    indus::Model model = kind == "lp" ? indus::io::read_lp(argv[2]) : indus::io::read_mps(argv[2]);
+    auto res = indus::presolve::PresolveEngine::apply(model);
+    FILE* f = std::fopen(argv[4], "w");
+    if (!f) return 1;
+    presolve_dump::dump_result(f, res);
+    if (!res.is_infeasible && !res.is_unbounded) {
+        auto red = presolve_dump::synthetic_reduced<indus::Solution>(res.presolved_model, indus::SolveStatus::kOptimal);
+        presolve_dump::dump_solution(f, indus::presolve::PresolveEngine::postsolve(red, res.stack, model));
+    }
+    std::fclose(f);
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc >= 2 && std::string(argv[1]) == "dump") return dump(argc, argv);
+    if (argc >= 2 && std::string(argv[1]) == "presolve_dump") return presolve_dump_mode(argc, argv);
     if (argc >= 2 && std::string(argv[1]) == "dump_model") return dump_model(argc, argv);
     if (argc >= 2 && std::string(argv[1]) == "presolve_scale") return presolve_scale(argc, argv);
     if (argc < 3 || (std::string(argv[1]) != "cpu" && std::string(argv[1]) != "gpu")) {

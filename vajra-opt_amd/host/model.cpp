@@ -221,9 +221,10 @@ Solution solve(const Model& model, const Options& options) {
             inner.enable_presolve = false;
             Solution red;
             if (pre.presolved_model.num_cols == 0 || pre.presolved_model.num_rows == 0) {
-                red = pre.presolved_model.num_cols == 0 ? solve_trivial_zero_cols(pre.presolved_model)
-                                                        : solve_trivial_zero_rows(pre.presolved_model);
-                if (red.status == SolveStatus::kOptimal) red.status_message = "Solved to optimality by presolve reductions";
+                // presolve left nothing to solve: the reduced answer is its offset alone
+                red.status = SolveStatus::kOptimal;
+                red.status_message = "Solved to optimality by presolve reductions";
+                red.objective_value = pre.presolved_model.objective_offset;
             } else {
                 red = solve(pre.presolved_model, inner);
             }
