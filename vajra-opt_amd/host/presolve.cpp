@@ -12,6 +12,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <memory>
 #include <string>
 #include <unordered_map>
 
@@ -21,7 +23,49 @@ namespace indus::presolve {
 
 namespace {
 
-using Entries = std::unordered_map<int, double>;
+// Bump allocator for the hash maps of one presolve call: nodes and bucket arrays come from
+// large blocks and are released together when the call ends (an erased node is not reused).
+// Only where memory comes from changes; the maps' visiting order does not depend on it.
+class Arena {
+public:
+    void* get(size_t bytes, size_t align) {
+        size_t pad = (align - reinterpret_cast<uintptr_t>(cur_) % align) % align;
+        if (pad + bytes > left_) {
+            const size_t sz = std::max<size_t>(bytes + align, size_t(4) << 20);
+            blocks_.emplace_back(new char[sz]);
+            cur_ = blocks_.back().get();
+            left_ = sz;
+            pad = (align - reinterpret_cast<uintptr_t>(cur_) % align) % align;
+        }
+        char* p = cur_ + pad;
+        cur_ = p + bytes;
+        left_ -= pad + bytes;
+        return p;
+    }
+
+private:
+    std::vector<std::unique_ptr<char[]>> blocks_;
+    char* cur_ = nullptr;
+    size_t left_ = 0;
+};
+
+template <class T>
+struct ArenaAlloc {
+    using value_type = T;
+    Arena* arena;
+    explicit ArenaAlloc(Arena* a) : arena(a) {}
+    template <class U>
+    ArenaAlloc(const ArenaAlloc<U>& o) : arena(o.arena) {}
+    T* allocate(size_t k) { return static_cast<T*>(arena->get(k * sizeof(T), alignof(T))); }
+    void deallocate(T*, size_t) noexcept {}
+    template <class U>
+    bool operator==(const ArenaAlloc<U>& o) const { return arena == o.arena; }
+    template <class U>
+    bool operator!=(const ArenaAlloc<U>& o) const { return arena != o.arena; }
+};
+
+using Entries = std::unordered_map<int, double, std::hash<int>, std::equal_to<int>,
+                                   ArenaAlloc<std::pair<const int, double>>>;
 constexpr double kInf = 1e20;
 constexpr double kInfTest = 1e19;   // bounds at or beyond +-1e19 count as infinite
 
@@ -30,20 +74,31 @@ public:
     Reducer(const Model& m, PresolveResult& r)
         : M(m), res(r), m_(m.num_rows), n_(m.num_cols), row_on(m_, true), col_on(n_, true),
           row_touched(m_, false), rl(m.row_lower), ru(m.row_upper), cl(m.col_lower), cu(m.col_upper),
-          cost(m.c), offset(m.objective_offset), rows(m_), cols(n_) {}
+          cost(m.c), offset(m.objective_offset), row_dirty(m_, 1) {
+        // each map built empty with the allocator alone, as a default-constructed map
+        const ArenaAlloc<std::pair<const int, double>> alloc(&arena);
+        rows.reserve(m_);
+        cols.reserve(n_);
+        for (int i = 0; i < m_; ++i) rows.emplace_back(alloc);
+        for (int j = 0; j < n_; ++j) cols.emplace_back(alloc);
+    }
 
     // false: presolve proved infeasibility or unboundedness (res carries the message)
     bool run(int max_passes) {
         if (!bounds_consistent()) return false;
+        // Each row map receives its columns in ascending order and each column map its rows
+        // in the stored order, the insertion sequences of one column sweep (presolve.cpp:
+        // 65-77); filling map after map keeps each map's nodes together in the arena.
         for (int j = 0; j < n_; ++j) {
             const auto rr = M.A.col_rows(j);
             const auto vv = M.A.col_vals(j);
-            for (size_t k = 0; k < rr.size(); ++k) {
-                if (std::fabs(vv[k]) <= tol::kZeroDrop) continue;
-                rows[rr[k]][j] = vv[k];
-                cols[j][rr[k]] = vv[k];
-            }
+            for (size_t k = 0; k < rr.size(); ++k)
+                if (std::fabs(vv[k]) > tol::kZeroDrop) cols[j][rr[k]] = vv[k];
         }
+        const la::SparseMatrixCSR R = M.A.to_csr();
+        for (int i = 0; i < m_; ++i)
+            for (int64_t k = R.row_ptr[i]; k < R.row_ptr[i + 1]; ++k)
+                if (std::fabs(R.values[k]) > tol::kZeroDrop) rows[i][R.col_idx[k]] = R.values[k];
         for (int pass = 0; pass < max_passes; ++pass) {
             int done = 0;
             if (!empty_rows(done) || !empty_cols(done) || !fixed_cols(done) || !singleton_rows(done) ||
@@ -62,7 +117,16 @@ private:
     std::vector<bool> row_on, col_on, row_touched;
     std::vector<double> rl, ru, cl, cu, cost;
     double offset;
+    // Rows whose activity test could decide differently than at their last evaluation: a
+    // row's entries, its bounds or a bound of one of its columns changed since. A clean row
+    // is skipped by the activity step, which would reach the same "no reduction" again.
+    std::vector<char> row_dirty;
+    Arena arena;   // declared before the maps: they are destroyed first
     std::vector<Entries> rows, cols;
+
+    void dirty_rows_of(int j) {
+        for (const auto& e : cols[j]) row_dirty[e.first] = 1;
+    }
 
     std::string rname(int i) const {
         if (static_cast<size_t>(i) < M.row_names.size() && !M.row_names[i].empty()) return M.row_names[i];
@@ -168,6 +232,7 @@ private:
                 if (rl[i] > -kInfTest) rl[i] -= a * v;
                 if (ru[i] < kInfTest) ru[i] -= a * v;
                 rows[i].erase(j);
+                row_dirty[i] = 1;
                 if (rl[i] > ru[i] + tol::kZeroDrop)
                     return infeasible("Fixed variable " + cname(j) + " created crossed bounds on row " + rname(i));
             }
@@ -200,6 +265,7 @@ private:
             row_on[i] = false;
             rows[i].clear();
             cols[j].erase(i);
+            dirty_rows_of(j);
             ReductionRecord r;
             r.type = ReductionType::kSingletonRow;
             r.row_idx = i;
@@ -223,7 +289,8 @@ private:
     // that attains it). presolve.cpp:247-346.
     bool activity_rows(int& done) {
         for (int i = 0; i < m_; ++i) {
-            if (!row_on[i] || rows[i].size() <= 1) continue;
+            if (!row_on[i] || rows[i].size() <= 1 || !row_dirty[i]) continue;
+            row_dirty[i] = 0;
             double amin = 0.0, amax = 0.0;
             bool min_inf = false, max_inf = false;
             for (const auto& e : rows[i]) {
@@ -278,7 +345,10 @@ private:
             res.stack.push(std::move(r));
             ++res.num_forcing_rows;
             ++done;
-            for (const auto& f : fix) cl[f.first] = cu[f.first] = f.second;
+            for (const auto& f : fix) {
+                cl[f.first] = cu[f.first] = f.second;
+                dirty_rows_of(f.first);
+            }
         }
         return true;
     }
@@ -394,6 +464,7 @@ private:
                 const int q = o.first;
                 const double aq = o.second;
                 row_touched[q] = true;
+                row_dirty[q] = 1;
                 const double shift = aq * (b / ae);
                 if (rl[q] > -kInfTest) rl[q] -= shift;
                 if (ru[q] < kInfTest) ru[q] -= shift;
@@ -414,6 +485,7 @@ private:
             cols[je].clear();
             rows[i].clear();
             cols[jk].erase(i);
+            dirty_rows_of(jk);   // x_keep's bounds were tightened
             res.stack.push(std::move(r));
             ++res.num_doubleton_rows;
             ++done;
@@ -441,25 +513,35 @@ private:
         P.objective_offset = offset;
         P.num_rows = static_cast<int>(S.reduced_to_orig_row.size());
         P.num_cols = static_cast<int>(S.reduced_to_orig_col.size());
-        std::vector<la::Triplet> trips;
+        // the reduced matrix, column by column with rows ascending: what assembling the
+        // surviving entries as triplets gives (no duplicates occur, |a| <= 1e-11 dropped)
+        P.A.resize(P.num_rows, P.num_cols);
+        std::vector<std::pair<int, double>> colbuf;
         for (int jr = 0; jr < P.num_cols; ++jr) {
             const int j = S.reduced_to_orig_col[jr];
             P.c.push_back(cost[j]);
             P.col_lower.push_back(cl[j]);
             P.col_upper.push_back(cu[j]);
             P.col_names.push_back(cname(j));
+            colbuf.clear();
             for (const auto& e : cols[j]) {
                 const int ir = S.orig_to_reduced_row[e.first];
-                if (ir >= 0 && std::fabs(e.second) > tol::kZeroDrop) trips.push_back({ir, jr, e.second});
+                if (ir >= 0 && std::fabs(e.second) > tol::kZeroDrop) colbuf.emplace_back(ir, e.second);
+            }
+            std::sort(colbuf.begin(), colbuf.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+            P.A.col_ptr[jr] = static_cast<int64_t>(P.A.row_idx.size());
+            for (const auto& e : colbuf) {
+                P.A.row_idx.push_back(e.first);
+                P.A.values.push_back(e.second);
             }
         }
+        P.A.col_ptr[P.num_cols] = static_cast<int64_t>(P.A.row_idx.size());
         for (int ir = 0; ir < P.num_rows; ++ir) {
             const int i = S.reduced_to_orig_row[ir];
             P.row_lower.push_back(rl[i]);
             P.row_upper.push_back(ru[i]);
             P.row_names.push_back(rname(i));
         }
-        P.A.set_from_triplets(P.num_rows, P.num_cols, trips, true);
         res.presolved_model = std::move(P);
     }
 };
