@@ -659,8 +659,78 @@ int presolve_dump_mode(int argc, char** argv) {
     return 0;
 }
 
+// api_solve <m> <n> <fp64|fp32|bf16> <tol> <presolve 0|1> [scaled 0|1]: indus::solve on the
+// device, end to end through the C++ API, on the random LP family of the benchmark (6
+// nonzeros per column; with `scaled`, 10% of the columns scaled by 10^U[-3,3]) with planted
+// presolve reductions (every 50th column fixed at its feasible value, 1000 empty rows, 1000
+// singleton rows). Prints the status, the objective, the presolve sizes, the device and host
+// times and the verifier. The engine's own time limit is 200 s.
+int api_solve(int argc, char** argv) {
+    if (argc < 7) return 2;
+    const int m = std::atoi(argv[2]), n = std::atoi(argv[3]), k = 6, extra = 1000;
+    const std::string prec = argv[4];
+    const double tol = std::atof(argv[5]);
+    const bool pre = std::atoi(argv[6]) != 0;
+    const bool scaled = argc > 7 && std::atoi(argv[7]) != 0;
+    const int64_t nnz = static_cast<int64_t>(n) * k;
+    std::vector<int64_t> cp(n + 1);
+    std::vector<int32_t> ri(nnz);
+    std::vector<double> v(nnz), c(n), lc(n), uc(n), lr(m), ur(m), x0(n), y0(m);
+    if (vj_gen_random_lp(0x5EED2026ULL + 1, m, n, k, scaled ? 0.1 : 0.0, scaled ? 3.0 : 0.0, cp.data(), ri.data(), v.data(), c.data(), lc.data(),
+                         uc.data(), lr.data(), ur.data(), x0.data(), y0.data()) != 0)
+        return 3;
+    std::vector<indus::la::Triplet> t;
+    t.reserve(static_cast<size_t>(nnz) + extra);
+    for (int j = 0; j < n; ++j)
+        for (int64_t p = cp[j]; p < cp[j + 1]; ++p) t.push_back({ri[p], j, v[p]});
+    indus::Model M;
+    M.name = "api_scale";
+    M.num_rows = m + 2 * extra;
+    M.num_cols = n;
+    M.c = c;
+    M.col_lower = lc;
+    M.col_upper = uc;
+    M.row_lower = lr;
+    M.row_upper = ur;
+    for (int e = 0; e < extra; ++e) {
+        M.row_lower.push_back(-1.0);
+        M.row_upper.push_back(1.0);
+    }
+    for (int e = 0; e < extra; ++e) {
+        t.push_back({m + extra + e, (e * 7919) % n, 1.0});
+        M.row_lower.push_back(-1e20);
+        M.row_upper.push_back(20.0);
+    }
+    for (int j = 0; j < n; j += 50) M.col_lower[j] = M.col_upper[j] = x0[j];
+    M.A.set_from_triplets(M.num_rows, n, t);
+    indus::Options o;
+    o.set("algorithm", "pdhg_cuda");
+    o.use_gpu = true;
+    o.enable_presolve = pre;
+    o.set("tolerance", tol);
+    o.set("pdhg_precision", prec.c_str());
+    o.iteration_limit = 1000000;
+    o.time_limit = 200.0;
+    const auto t0 = std::chrono::steady_clock::now();
+    const indus::Solution s = indus::solve(M, o);
+    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const auto g = indus::gpu::get_last_gpu_timing();
+    const auto ver = indus::verifier::verify_solution(M, s, 1e-6);
+    std::printf("api_solve m=%d n=%d nnz=%lld scaled=%d precision=%s tol=%g presolve=%d status=%d objective=%.17g "
+                "reduced=%dx%d reductions=%d iterations=%lld wall_s=%.3f device_total_s=%.3f device_iter_s=%.3f "
+                "host_s=%.3f rel_kkt=%.3e verified=%d primal_viol=%.3e dual_viol=%.3e\n",
+                M.num_rows, n, static_cast<long long>(M.A.nnz()), scaled ? 1 : 0, prec.c_str(), tol, pre ? 1 : 0,
+                static_cast<int>(s.status), s.objective_value, s.presolve_num_rows, s.presolve_num_cols,
+                s.presolve_total_reductions, static_cast<long long>(s.iterations), wall, g.total_time_sec,
+                g.iteration_time_sec, wall - g.total_time_sec, g.rel_kkt, ver.passed ? 1 : 0, ver.max_primal_violation,
+                ver.max_dual_violation);
+    std::printf("status_message %s\n", s.status_message.c_str());
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc >= 2 && std::string(argv[1]) == "dump") return dump(argc, argv);
+    if (argc >= 2 && std::string(argv[1]) == "api_solve") return api_solve(argc, argv);
     if (argc >= 2 && std::string(argv[1]) == "presolve_dump") return presolve_dump_mode(argc, argv);
     if (argc >= 2 && std::string(argv[1]) == "dump_model") return dump_model(argc, argv);
     if (argc >= 2 && std::string(argv[1]) == "presolve_scale") return presolve_scale(argc, argv);

@@ -200,3 +200,30 @@ def test_presolve_handoff_at_scale():
     assert r.returncode == 0
     fields = dict(kv.split("=") for kv in r.stdout.split() if "=" in kv)
     assert float(fields["apply_s"]) < 30.0 and float(fields["postsolve_s"]) < 30.0
+
+
+def _api_solve(m, n, prec, tol, pre, scaled=False):
+    r = subprocess.run([_driver(), "api_solve", str(m), str(n), prec, repr(tol), str(int(pre)), str(int(scaled))],
+                       capture_output=True, text=True, timeout=600)
+    print(r.stdout)
+    assert r.returncode == 0, r.stderr[-2000:]
+    line = next(l for l in r.stdout.splitlines() if l.startswith("api_solve"))
+    return dict(kv.split("=") for kv in line.split()[1:])
+
+
+@pytest.mark.gpu
+def test_api_solve_with_presolve_at_scale():
+    """indus::solve end to end on the device (default presolve on) for a 102,000 x 200,000
+    random LP (6 nonzeros per column) with planted reductions: presolve removes them, the
+    device solves the reduced LP, postsolve rebuilds the full point, the status guard
+    (1e-7 primal audit) and the verifier accept it, and the objective equals the solve
+    without presolve to 1e-6."""
+    print("[case] api_solve 100000 x 200000 fp64 1e-9 presolve on/off", flush=True)
+    a = _api_solve(100000, 200000, "fp64", 1e-9, True)
+    b = _api_solve(100000, 200000, "fp64", 1e-9, False)
+    assert int(a["status"]) == 1 and int(b["status"]) == 1, (a, b)
+    assert int(a["verified"]) == 1 and int(b["verified"]) == 1
+    assert int(a["reductions"]) >= 4000 + 2000
+    assert int(a["reduced"].split("x")[1]) < 200000
+    oa, ob = float(a["objective"]), float(b["objective"])
+    assert abs(oa - ob) <= 1e-6 * (1 + abs(ob)), (oa, ob)
