@@ -35,8 +35,9 @@ def log(*a):
 
 
 def solve_timed(vp, eng, prec, tol, max_iters, time_limit):
+    # verbose=1: the engine prints a progress line to stderr every 10 s of a long solve
     p = vp.default_params(vp.MODE_RESTARTED, precision=prec, tolerance=tol, max_iterations=max_iters,
-                          time_limit=time_limit)
+                          time_limit=time_limit, verbose=1)
     t0 = time.perf_counter()
     r = eng.solve(p)
     wall = time.perf_counter() - t0
@@ -203,7 +204,7 @@ def bench_batch(vp, kind, args):
     eng = vp.Engine(0)
     eng.upload_batch(lps)
     p = vp.default_params(vp.MODE_RESTARTED, precision=args.precision, tolerance=args.tol,
-                          max_iterations=args.max_iters, time_limit=args.time_limit)
+                          max_iterations=args.max_iters, time_limit=args.time_limit, verbose=1)
     t0 = time.perf_counter()
     res = eng.solve_batch(p)
     t_batch = time.perf_counter() - t0

@@ -1023,6 +1023,7 @@ int run_typed(vj_ctx* ctx, FastEngine& E, vj_result* out, clk::time_point t_star
     std::vector<double> conv_time(cnt, -1.0);
     int64_t transfers = 0;
     bool all_done = false, timed_out = false;
+    auto t_report = clk::now();   // verbose >= 1: a progress line every 10 s of a long solve
     for (int64_t b = 0;; ++b) {
         if (use_graph) VJ_HIP(hipGraphLaunch(E.gexec, s));
         else VJ_TRY((enqueue_block<V, T>(ctx, E, av, tv)));
@@ -1042,6 +1043,18 @@ int run_typed(vj_ctx* ctx, FastEngine& E, vj_result* out, clk::time_point t_star
             }
             if (active == 0) { all_done = true; break; }
             if (secs(t_start, clk::now()) > prm.time_limit) { timed_out = true; break; }
+            if (prm.verbose >= 1 && secs(t_report, clk::now()) >= 10.0) {
+                t_report = clk::now();
+                int64_t it = 0;
+                double worst = 0.0;
+                for (int q = 0; q < cnt; ++q) {
+                    it = std::max<int64_t>(it, hs[q].iters);
+                    if (hs[q].active) worst = std::max(worst, hs[q].rel_kkt);
+                }
+                std::fprintf(stderr, "[vj] %.1f s: %lld iterations, worst live rel-KKT %.3e, %d/%d live, restarts %d\n",
+                             t_now, static_cast<long long>(it), worst, active, cnt, hs[0].restarts);
+                std::fflush(stderr);
+            }
         }
     }
     VJ_HIP(hipStreamSynchronize(s));
