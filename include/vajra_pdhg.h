@@ -114,7 +114,7 @@ typedef struct vj_params {
      * or the epoch is longer than artificial * (total iterations). */
     double restart_sufficient;     /* default 0.2 */
     double restart_necessary;      /* default 0.8 */
-    double restart_artificial;     /* default 0.36 */
+    double restart_artificial;     /* default 0.5 (PDLP: 0.36) */
     double primal_weight_smoothing;/* theta in log w <- theta log(|dy|/|dx|) + (1-theta) log w; default 0.5 */
     int power_iterations;          /* RESTARTED: power-iteration steps for ||A||_2; default 64 */
 } vj_params;

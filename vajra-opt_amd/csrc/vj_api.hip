@@ -189,7 +189,7 @@ void vj_params_default(vj_params* p, int mode) {
     p->verbose = 0;
     p->restart_sufficient = 0.2;
     p->restart_necessary = 0.8;
-    p->restart_artificial = 0.36;
+    p->restart_artificial = 0.5;
     p->primal_weight_smoothing = 0.5;
     p->power_iterations = 64;
 }

@@ -765,8 +765,8 @@ __global__ __launch_bounds__(kBlock) void kkt_finish(
 // Control: one workgroup per instance, run after the check passes of every block.
 // Restart rule (reflected restarted Halpern PDHG, Lu & Yang): with r the weighted
 // fixed-point residual sqrt(omega |dx|^2 + |dy|^2 / omega), restart when
-// r <= 0.2 r0, or r <= 0.8 r0 and r > r_prev, or the epoch exceeds 0.36 of all
-// iterations. At a restart the primal weight is smoothed as in PDLP:
+// r <= 0.2 r0, or r <= 0.8 r0 and r > r_prev, or the epoch exceeds a fraction (default
+// 0.5) of all iterations. At a restart the primal weight is smoothed as in PDLP:
 // log omega <- theta log(|Dy|/|Dx|) + (1 - theta) log omega (theta = 0.5 by default).
 struct ControlArgs {
     const double* px; int Rx;      // primal-step partials (cols), fixed tile height Rx
