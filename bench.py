@@ -96,7 +96,7 @@ def main():
         extra["fp64"] = dict(status=vp.STATUS[r64.status], iterations=r64.iterations, restarts=r64.restart_count,
                              rel_kkt=r64.rel_kkt, objective=r64.primal_objective, time_to_target_s=r64.time_to_target,
                              setup_s=r64.setup_time, iteration_s=r64.iteration_time, solve_wall_s=wall64,
-                             lanes=[r64.lanes_A, r64.lanes_At], kernels=k64)
+                             lanes=[r64.lanes_A, r64.lanes_At], check_interval=r64.check_interval, kernels=k64)
 
     # ---- headline: FP32 iterates + FP64 refinement ----------------------------------------
     for w in range(args.warmup):
@@ -122,6 +122,7 @@ def main():
                           restarts=res.restart_count, rel_kkt_device=res.rel_kkt, rel_kkt_oracle=oracle_kkt,
                           objective=res.primal_objective, time_to_target_s=res.time_to_target,
                           setup_s=res.setup_time, iteration_s=res.iteration_time, lanes=[res.lanes_A, res.lanes_At],
+                          check_interval=res.check_interval,
                           kernels=kmix)
     if r64 is not None and r64.status == 1 and res.status == 1:
         extra["speedup_mixed_vs_fp64"] = extra["fp64"]["solve_wall_s"] / value

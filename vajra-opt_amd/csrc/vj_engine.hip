@@ -152,6 +152,7 @@ struct FastEngine {
     int64_t nnz = 0;
     bool multi = false;
     int GA = 8, GAt = 4, K = 64;
+    int K_tuned = 0;   // iterations per check picked by the autotuner (0: not timed)
     double rho = 1.0;
     DevBuf av_s, tv_s, av_lo, tv_lo;
     DevBuf Rsc, Csc, rstep, cstep, rnorm, cnorm;
@@ -949,6 +950,7 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     };
     apply(cA[0], E.SA, E.GA, E.GsA, E.streamA, E.recA);
     apply(cAt[0], E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt);
+    E.K_tuned = 0;
     if (E.nnz < 200000 || (cA.size() == 1 && cAt.size() == 1))
         return VJ_OK;  // launch-bound sizes: the first candidate is as good as timing
     const size_t rounds = std::max(cA.size(), cAt.size());
@@ -969,6 +971,34 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     }
     apply(wA, E.SA, E.GA, E.GsA, E.streamA, E.recA);
     apply(wAt, E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt);
+    // The FP64 check passes run lane-group kernels with E.GA / E.GAt lanes per row. A split
+    // winner leaves them at the last candidate timed: give them the width that suits the rows.
+    if (E.SA > 1) E.GA = std::max(choose_lanes(avgA), 2);
+    if (E.SAt > 1) E.GAt = std::max(choose_lanes(avgAt), 2);
+    // Iterations per check: the smallest power of two in [32, 256] at which the check (two
+    // FP64 KKT passes, their reduction, control and restart kernels) costs at most 2% of the
+    // iterations between checks. The KKT passes are timed on the current iterates; their
+    // outputs are recomputed by the first real check.
+    if (prm.check_frequency == 0) {
+        hipEvent_t e0, e1;
+        VJ_HIP(hipEventCreate(&e0));
+        VJ_HIP(hipEventCreate(&e1));
+        VJ_HIP(hipEventRecord(e0, ctx->stream));
+        for (int r = 0; r < 3; ++r)
+            VJ_TRY(launch_kkt(ctx, &E, E.GA, E.GAt, E.multi, E.xfull.as<double>(), E.yfull.as<double>(), true, 1,
+                              E.part_kr.as<double>(), E.part_kc.as<double>(), E.kkt8.as<double>()));
+        VJ_HIP(hipEventRecord(e1, ctx->stream));
+        VJ_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        VJ_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        const double t_check = 1000.0 * ms / 3 + 20.0;   // + control and restart kernels
+        const double t_iter = bestA + bestAt;
+        int K = 32;
+        while (K < 256 && t_check / K > 0.02 * t_iter) K *= 2;
+        E.K_tuned = K;
+    }
     return VJ_OK;
 }
 
@@ -990,7 +1020,7 @@ int run_typed(vj_ctx* ctx, FastEngine& E, vj_result* out, clk::time_point t_star
     VJ_TRY((autotune<V, T>(ctx, E, av, tv)));
     // after autotuning the scratch runs left the real iterates untouched; recentre anyway
     VJ_TRY(launch_restart<T>(ctx, E, 1));
-    E.K = prm.check_frequency > 0 ? prm.check_frequency : 64;
+    E.K = prm.check_frequency > 0 ? prm.check_frequency : (E.K_tuned > 0 ? E.K_tuned : 64);
     VJ_HIP(hipStreamSynchronize(s));
     const auto t_iter0 = clk::now();
 
