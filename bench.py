@@ -211,7 +211,8 @@ def bench_batch(vp, kind, args):
     t_batch = time.perf_counter() - t0
     ok = sum(1 for r in res if r.status == 1)
     worst = max(r.rel_kkt for r in res)
-    log(f"batch {kind}: {ok}/{cnt} optimal in {t_batch:.2f}s (max iters {max(r.iterations for r in res)}), worst rel_kkt {worst:.2e}")
+    log(f"batch {kind}: {ok}/{cnt} optimal in {t_batch:.2f}s (max iters {max(r.iterations for r in res)}), "
+        f"worst rel_kkt {worst:.2e}, check interval {res[0].check_interval}")
     eng.close()
     # sequential plain FP64 single solves on a sample, same tolerance
     sample = lps[:: max(1, cnt // args.seq_sample)][: args.seq_sample]
@@ -229,7 +230,8 @@ def bench_batch(vp, kind, args):
     seq_lps = len(sample) / seq_t
     batch_lps = cnt / t_batch
     log(f"batch {kind}: {batch_lps:.1f} LPs/s batched vs {seq_lps:.1f} LPs/s sequential fp64 ({seq_ok}/{len(sample)} optimal)")
-    return dict(count=cnt, sum_nnz=nnz, optimal=ok, worst_rel_kkt=worst, batch_s=t_batch, batch_lps_per_s=batch_lps,
+    return dict(count=cnt, sum_nnz=nnz, optimal=ok, worst_rel_kkt=worst, check_interval=res[0].check_interval,
+                batch_s=t_batch, batch_lps_per_s=batch_lps,
                 sequential_sample=len(sample), sequential_optimal=seq_ok, sequential_lps_per_s=seq_lps,
                 speedup=batch_lps / seq_lps, precision=args.precision)
 
