@@ -718,12 +718,12 @@ int api_solve(int argc, char** argv) {
     const auto ver = indus::verifier::verify_solution(M, s, 1e-6);
     std::printf("api_solve m=%d n=%d nnz=%lld scaled=%d precision=%s tol=%g presolve=%d status=%d objective=%.17g "
                 "reduced=%dx%d reductions=%d iterations=%lld wall_s=%.3f device_total_s=%.3f device_iter_s=%.3f "
-                "host_s=%.3f rel_kkt=%.3e verified=%d primal_viol=%.3e dual_viol=%.3e\n",
+                "host_s=%.3f rel_kkt=%.3e verified=%d primal_viol=%.3e dual_viol=%.3e check_interval=%d\n",
                 M.num_rows, n, static_cast<long long>(M.A.nnz()), scaled ? 1 : 0, prec.c_str(), tol, pre ? 1 : 0,
                 static_cast<int>(s.status), s.objective_value, s.presolve_num_rows, s.presolve_num_cols,
                 s.presolve_total_reductions, static_cast<long long>(s.iterations), wall, g.total_time_sec,
                 g.iteration_time_sec, wall - g.total_time_sec, g.rel_kkt, ver.passed ? 1 : 0, ver.max_primal_violation,
-                ver.max_dual_violation);
+                ver.max_dual_violation, g.check_interval);
     std::printf("status_message %s\n", s.status_message.c_str());
     return 0;
 }

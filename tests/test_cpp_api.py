@@ -227,3 +227,6 @@ def test_api_solve_with_presolve_at_scale():
     assert int(a["reduced"].split("x")[1]) < 200000
     oa, ob = float(a["objective"]), float(b["objective"])
     assert abs(oa - ob) <= 1e-6 * (1 + abs(ob)), (oa, ob)
+    # 1.2M nonzeros: the autotuner times the check and picks a power of two in [32, 256]
+    k = int(a["check_interval"])
+    assert k in (32, 64, 128, 256), k
