@@ -28,6 +28,17 @@ namespace {
 using clk = std::chrono::steady_clock;
 double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
 
+// A HIP event owned by a scope, destroyed on every return path (error returns included).
+struct ScopedEvent {
+    hipEvent_t e = nullptr;
+    ScopedEvent() = default;
+    ScopedEvent(const ScopedEvent&) = delete;
+    ScopedEvent& operator=(const ScopedEvent&) = delete;
+    ~ScopedEvent() {
+        if (e) (void)hipEventDestroy(e);
+    }
+};
+
 #define VJ_DISPATCH_G(G_, ...)                         \
     switch (G_) {                                      \
         case 1: { constexpr int GG = 1; __VA_ARGS__; } break;   \
@@ -877,10 +888,11 @@ int time_kernels(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv, int reps,
     VJ_HIP(hipMemcpyAsync(xs.p, E.x.p, Tn, hipMemcpyDeviceToDevice, s));
     VJ_HIP(hipMemcpyAsync(xbs.p, E.xbar.p, Tn, hipMemcpyDeviceToDevice, s));
     VJ_HIP(hipMemcpyAsync(ys.p, E.y.p, Tm, hipMemcpyDeviceToDevice, s));
-    hipEvent_t e0, e1, e2;
-    VJ_HIP(hipEventCreate(&e0));
-    VJ_HIP(hipEventCreate(&e1));
-    VJ_HIP(hipEventCreate(&e2));
+    ScopedEvent se0, se1, se2;
+    VJ_HIP(hipEventCreate(&se0.e));
+    VJ_HIP(hipEventCreate(&se1.e));
+    VJ_HIP(hipEventCreate(&se2.e));
+    hipEvent_t e0 = se0.e, e1 = se1.e, e2 = se2.e;
     for (int w = 0; w < 3; ++w) {   // warm-up (clocks, caches)
         VJ_TRY(launch_primal<V, T>(ctx, E, tv, xs.as<T>(), xbs.as<T>(), ys.as<T>(), 1, 0));
         VJ_TRY(launch_dual<V, T>(ctx, E, av, xbs.as<T>(), ys.as<T>(), 1, 0));
@@ -894,9 +906,6 @@ int time_kernels(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv, int reps,
     float ms1 = 0, ms2 = 0;
     VJ_HIP(hipEventElapsedTime(&ms1, e0, e1));
     VJ_HIP(hipEventElapsedTime(&ms2, e1, e2));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipEventDestroy(e2);
     *us_primal = 1000.0 * ms1 / reps;
     *us_dual = 1000.0 * ms2 / reps;
     return VJ_OK;
@@ -980,9 +989,10 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     // iterations between checks. The KKT passes are timed on the current iterates; their
     // outputs are recomputed by the first real check.
     if (prm.check_frequency == 0) {
-        hipEvent_t e0, e1;
-        VJ_HIP(hipEventCreate(&e0));
-        VJ_HIP(hipEventCreate(&e1));
+        ScopedEvent se0, se1;
+        VJ_HIP(hipEventCreate(&se0.e));
+        VJ_HIP(hipEventCreate(&se1.e));
+        hipEvent_t e0 = se0.e, e1 = se1.e;
         VJ_HIP(hipEventRecord(e0, ctx->stream));
         for (int r = 0; r < 3; ++r)
             VJ_TRY(launch_kkt(ctx, &E, E.GA, E.GAt, E.multi, E.xfull.as<double>(), E.yfull.as<double>(), true, 1,
@@ -991,8 +1001,6 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
         VJ_HIP(hipEventSynchronize(e1));
         float ms = 0.0f;
         VJ_HIP(hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         const double t_check = 1000.0 * ms / 3 + 20.0;   // + control and restart kernels
         const double t_iter = bestA + bestAt;
         int K = 32;
@@ -1042,13 +1050,16 @@ int run_typed(vj_ctx* ctx, FastEngine& E, vj_result* out, clk::time_point t_star
             return rc;
         }
         VJ_HIP(ec);
-        VJ_HIP(hipGraphInstantiate(&E.gexec, g, nullptr, nullptr, 0));
-        (void)hipGraphDestroy(g);
+        const hipError_t ei = hipGraphInstantiate(&E.gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);   // the executable graph keeps what it needs
+        VJ_HIP(ei);
     }
-    hipEvent_t ev[2], ev0;
-    VJ_HIP(hipEventCreate(&ev[0]));
-    VJ_HIP(hipEventCreate(&ev[1]));
-    VJ_HIP(hipEventCreate(&ev0));
+    ScopedEvent sev[2], sev0;
+    VJ_HIP(hipEventCreate(&sev[0].e));
+    VJ_HIP(hipEventCreate(&sev[1].e));
+    VJ_HIP(hipEventCreate(&sev0.e));
+    const hipEvent_t ev[2] = {sev[0].e, sev[1].e};
+    const hipEvent_t ev0 = sev0.e;
     VJ_HIP(hipEventRecord(ev0, s));
     std::vector<double> conv_time(cnt, -1.0);
     int64_t transfers = 0;
@@ -1088,9 +1099,6 @@ int run_typed(vj_ctx* ctx, FastEngine& E, vj_result* out, clk::time_point t_star
         }
     }
     VJ_HIP(hipStreamSynchronize(s));
-    (void)hipEventDestroy(ev[0]);
-    (void)hipEventDestroy(ev[1]);
-    (void)hipEventDestroy(ev0);
     const auto t_iter1 = clk::now();
     (void)all_done;
 
