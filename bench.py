@@ -110,7 +110,7 @@ def main():
             f"rel_kkt={res.rel_kkt:.3e} setup={res.setup_time:.3f}s iter={res.iteration_time:.3f}s wall={wall:.3f}s")
     kmix = eng.benchmark_kernels(reps=30)
     x, rd, y = eng.download()
-    # rel-KKT of the returned point, recomputed by the oracle on the host (checker only)
+    # rel-KKT of the returned point, recomputed by the oracle on the host (for comparison only)
     oracle_kkt = None
     try:
         import oracle as O

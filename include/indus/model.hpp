@@ -30,7 +30,7 @@ public:
     std::vector<VarType> col_type;
     std::vector<std::string> row_names;
     std::vector<std::string> col_names;
-    la::SparseMatrixCSC Q;  // quadratic objectives are outside this build's path
+    la::SparseMatrixCSC Q;  // quadratic objectives are outside this library's PDHG LP path
 
     [[nodiscard]] bool has_quadratic_objective() const noexcept { return Q.nnz() > 0; }
     [[nodiscard]] bool has_integers() const noexcept;
@@ -91,7 +91,7 @@ public:
 // The dispatcher seam (model.cpp:376-667) for the PDHG path: validation, trivial models,
 // presolve handoff, PDHG dispatch (GPU for "pdhg_cuda" / "pdhg_hip" / use_gpu, CPU for
 // "pdhg" / "pdhg_cpu"), quality recomputation and the status guard. Other algorithms
-// (simplex, MILP, QP) are not part of this build and return kUnsupported.
+// (simplex, MILP, QP) are not part of this library and return kUnsupported.
 Solution solve(const Model& model, const Options& options = Options());
 
 }  // namespace indus

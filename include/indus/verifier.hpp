@@ -1,6 +1,6 @@
 // In-memory KKT verifier (restates include/indus/verifier.hpp:9-49 and the LP branch of
 // src/engine/verifier.cpp:25-330 of the reference). File-based verification (.sol parsing)
-// is outside this build's path.
+// is outside this library's PDHG path.
 #pragma once
 #include <string>
 #include <vector>

@@ -1,7 +1,7 @@
 """TEST INFRASTRUCTURE -- ctypes binding of the CPU oracle (oracle/_build/liboracle.so) and,
 when it has been built, of the reference library compiled from its own sources
 (oracle/_ref/libindus_ref.so). Only tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg import this module, and only as the checker.
+cpu_baseline leg import this module, and only to compare against.
 """
 from __future__ import annotations
 

@@ -1,6 +1,6 @@
 /* TEST INFRASTRUCTURE -- NOT PART OF THE PRODUCT.
  *
- * CPU restatement of the reference's restarted-PDHG LP path, used only as the checker
+ * CPU restatement of the reference's restarted-PDHG LP path, used only for comparison
  * by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg. The product
  * (libvajra_hip.so / libindus_hip.so) never links, loads or calls this code.
  *
@@ -44,7 +44,7 @@ typedef struct {
     double offset;
 } or_lp;
 
-/* PdhgOptions (include/indus/pdhg.hpp:17-28) plus checker-only controls. */
+/* PdhgOptions (include/indus/pdhg.hpp:17-28) plus test-only controls. */
 typedef struct {
     int64_t max_iterations;
     double time_limit;
@@ -55,10 +55,10 @@ typedef struct {
     int restart_stagnation_limit;
     double step_size_safety;
     int enable_scaling;
-    /* checker-only: shadow rel-KKT monitor target at each check (0 = off); it reads the
+    /* test-only: shadow rel-KKT monitor target at each check (0 = off); it reads the
      * current iterate and never changes the trajectory. */
     double kkt_target;
-    /* checker-only: >0 stops after this iteration, before finalisation, and returns the
+    /* test-only: >0 stops after this iteration, before finalisation, and returns the
      * current scaled iterates in traj_*. */
     int64_t stop_at_iter;
 } or_pdhg_opts;

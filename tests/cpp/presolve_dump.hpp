@@ -1,8 +1,8 @@
 // TEST INFRASTRUCTURE -- a canonical text dump of a presolve result and of the postsolve of a
 // fixed synthetic reduced solution. Written against the field names the reference's
-// indus::presolve / indus::Model / indus::Solution types share with this build's, it is
+// indus::presolve / indus::Model / indus::Solution types share with this repository's, it is
 // compiled twice: into the oracle's shim (reference types, oracle/ref_shim.cpp) and into the
-// C++ test driver (this build's types, tests/cpp/test_indus_api.cpp). Equal dumps mean the
+// C++ test driver (this repository's types, tests/cpp/test_indus_api.cpp). Equal dumps mean the
 // same reductions, in the same order, with bit-identical numbers (doubles printed with %a).
 #pragma once
 #include <cstdio>

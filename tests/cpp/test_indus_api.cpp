@@ -641,7 +641,7 @@ int dump_model(int argc, char** argv) {
     return 0;
 }
 
-// presolve_dump <model file> <mps|lp> <out>: this build's presolve and postsolve, written as
+// presolve_dump <model file> <mps|lp> <out>: the C++ API's presolve and postsolve, written as
 // the canonical dump the oracle's shim writes for the reference (tests/test_presolve.py)
 int presolve_dump_mode(int argc, char** argv) {
     if (argc < 5) return 2;

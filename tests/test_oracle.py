@@ -1,4 +1,4 @@
-"""CPU tests of the oracle (oracle/pdhg_oracle.c), the checker every GPU parity test relies on.
+"""CPU tests of the oracle (oracle/pdhg_oracle.c), the comparison every GPU parity test relies on.
 
 Pins, in order of strength:
   * golden vectors written by the reference compiled from its own sources

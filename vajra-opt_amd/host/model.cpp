@@ -183,7 +183,7 @@ Solution solve(const Model& model, const Options& options) {
     if (model.classify() != ProblemClass::kLp || (!wants_gpu(options) && !wants_cpu_pdhg(options))) {
         Solution sol;
         sol.status = SolveStatus::kUnsupported;
-        sol.status_message = "this build provides the PDHG LP path (algorithm pdhg / pdhg_cpu / pdhg_cuda / pdhg_hip)";
+        sol.status_message = "this library provides the PDHG LP path only (algorithm pdhg / pdhg_cpu / pdhg_cuda / pdhg_hip)";
         sol.solve_time_seconds = since(t0);
         return sol;
     }
