@@ -257,7 +257,20 @@ def cpu_baseline(lp, args, r64):
                       f"iteration) + {iters} iterations, {wall:.1f}s wall; value = setup + per-iteration time x the "
                       f"FP64 GPU engine's iteration count ({gpu_iters}); the reference iteration map itself needs "
                       f"more iterations (it did not reach rel-KKT 1e-6 within 1e6 iterations on the 2,000 x 5,000 LP)",
-            "setup_s": r["setup_time"], "s_per_iteration": per_it, "host_nproc": os.cpu_count()}
+            "setup_s": r["setup_time"], "s_per_iteration": per_it, "host_nproc": os.cpu_count(),
+            "host_cpu": host_cpu_model()}
+
+
+def host_cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), for the CPU baseline's record."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 if __name__ == "__main__":
