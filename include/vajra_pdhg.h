@@ -102,7 +102,13 @@ typedef struct vj_params {
     int lanes_A;                   /* SpMV kernel for rows of A: 0 = autotune; 2..32 = lanes per row
                                       (lane groups, or lanes per block segment when split);
                                       -1 = tile-streamed; -2 = tile-streamed reading interleaved
-                                      (index, value) records (FP32; other precisions as -1) */
+                                      (index, value) records (FP32; other precisions as -1);
+                                      -3 = sliced ELL (slices of 64 rows, single LP; where no
+                                      ELL copy exists, as -1); -4 = sliced ELL reading (index,
+                                      value) records (FP32; other precisions as -3; no ELL copy:
+                                      as -2). The result reports the kernel that ran with the
+                                      same codes; the autotuner tries ELL where its padding is
+                                      at most 25% of the nonzeros */
     int lanes_At;                  /* same for rows of A^T */
     int refine_rounds;             /* FP32/BF16: max FP64 refinement rounds (0 = 64) */
     int verbose;

@@ -186,6 +186,14 @@ struct FastEngine {
     // the tile-streamed kernels when the autotuner picks them (recA / recAt)
     DevBuf recA_buf, recAt_buf, recSA_buf, recSAt_buf;
     bool recA = false, recAt = false;
+    // sliced-ELL copies (single LP): per-slice entry offsets and widths, entries in slice
+    // order, their CSR positions (-1 = padding), values and FP32 records; ellA / ellAt select
+    // the ELL step kernels; ellPad* = ELL entries / nonzeros (0 while not measured)
+    DevBuf ellOffA, ellWA, ellIdxA, ellPermA, ellValA, ellRecA;
+    DevBuf ellOffAt, ellWAt, ellIdxAt, ellPermAt, ellValAt, ellRecAt;
+    int64_t ellLenA = 0, ellLenAt = 0;
+    bool ellBuiltA = false, ellBuiltAt = false, ellA = false, ellAt = false;
+    double ellPadA = 0.0, ellPadAt = 0.0;
     int builtSA = 0, builtSAt = 0;   // split layout currently held in bp/idx/perm
     // tile-streamed kernels: host-built row tiles of A (y side) and A^T (x side) and, per
     // instance, the first/last tile touching it
@@ -254,6 +262,18 @@ int launch_primal(vj_ctx* ctx, FastEngine& E, const V* tv, T* x, T* xbar, const 
         hipLaunchKernelGGL((primal_finish<T>), dim3(grid_for(E.n, kBlock)), dim3(kBlock), 0, ctx->stream, E.n, E.SAt,
                            E.psumAt.as<T>(), E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), x, xbar, E.xhat.as<double>(),
                            E.xfull.as<double>(), E.st.as<InstState>(), E.part_x.as<double>(), it, flags, E.rho);
+        VJ_LAUNCHED();
+        return VJ_OK;
+    }
+    if (E.ellAt) {
+        VJ_TRY(require({E.ellOffAt.p, E.ellWAt.p, E.ellIdxAt.p, E.ellValAt.p}, "ell primal_step"));
+        VJ_TRY(with_entries<V>(E.recAt, E.ellRecAt, E.ellIdxAt.as<int>(), E.ellValAt.as<V>(), "ell records",
+                               [&](auto ent) {
+            hipLaunchKernelGGL((ell_primal_step<decltype(ent), T>), dim3(grid_for(E.n, kBlock)), dim3(kBlock), 0,
+                               ctx->stream, E.n, E.ellOffAt.as<int>(), E.ellWAt.as<int>(), ent, y, E.dhat.as<T>(),
+                               E.lcs.as<T>(), E.ucs.as<T>(), x, xbar, E.xhat.as<double>(), E.xfull.as<double>(),
+                               E.st.as<InstState>(), E.part_x.as<double>(), it, flags, E.rho);
+        }));
         VJ_LAUNCHED();
         return VJ_OK;
     }
@@ -331,6 +351,18 @@ int launch_dual(vj_ctx* ctx, FastEngine& E, const V* av, const T* xbar, T* y, in
         hipLaunchKernelGGL((dual_finish<T>), dim3(grid_for(E.m, kBlock)), dim3(kBlock), 0, ctx->stream, E.m, E.SA,
                            E.psumA.as<T>(), yht, E.lrs.as<T>(), E.urs.as<T>(), y, E.yhat.as<double>(),
                            E.yfull.as<double>(), E.st.as<InstState>(), E.part_y.as<double>(), it, flags, E.rho);
+        VJ_LAUNCHED();
+        return VJ_OK;
+    }
+    if (E.ellA) {
+        VJ_TRY(require({E.ellOffA.p, E.ellWA.p, E.ellIdxA.p, E.ellValA.p}, "ell dual_step"));
+        VJ_TRY(with_entries<V>(E.recA, E.ellRecA, E.ellIdxA.as<int>(), E.ellValA.as<V>(), "ell records",
+                               [&](auto ent) {
+            hipLaunchKernelGGL((ell_dual_step<decltype(ent), T>), dim3(grid_for(E.m, kBlock)), dim3(kBlock), 0,
+                               ctx->stream, E.m, E.ellOffA.as<int>(), E.ellWA.as<int>(), ent, xbar, yht,
+                               E.lrs.as<T>(), E.urs.as<T>(), y, E.yhat.as<double>(), E.yfull.as<double>(),
+                               E.st.as<InstState>(), E.part_y.as<double>(), it, flags, E.rho);
+        }));
         VJ_LAUNCHED();
         return VJ_OK;
     }
@@ -523,6 +555,56 @@ void build_split_tiles(int rows, int S, int cap, const std::vector<int32_t>& bp,
         tile_row.push_back(e);
         r = e;
     }
+}
+
+// Sliced-ELL layout of a CSR (rows x cols), slices of kEllSlice rows: per slice the offset
+// of its first entry and its width (its longest row); entry q of row r sits at
+// off[s] + kEllSlice*q + (r mod kEllSlice). perm holds each entry's CSR position, or -1 for
+// padding, whose index repeats the row's first column (column 0 for an empty row). Returns
+// false when the layout would not fit 32-bit offsets.
+bool build_ell(int rows, const std::vector<int64_t>& rp, const std::vector<int32_t>& ci, std::vector<int32_t>& off,
+               std::vector<int32_t>& width, std::vector<int32_t>& idx, std::vector<int32_t>& perm) {
+    const int slices = (rows + kEllSlice - 1) / kEllSlice;
+    off.assign(slices, 0);
+    width.assign(slices, 0);
+    int64_t total = 0;
+    for (int sl = 0; sl < slices; ++sl) {
+        int64_t w = 0;
+        for (int r = sl * kEllSlice; r < std::min(rows, (sl + 1) * kEllSlice); ++r) w = std::max(w, rp[r + 1] - rp[r]);
+        if (total + kEllSlice * w > 2147483647LL) return false;
+        off[sl] = static_cast<int32_t>(total);
+        width[sl] = static_cast<int32_t>(w);
+        total += kEllSlice * w;
+    }
+    idx.assign(static_cast<size_t>(total), 0);
+    perm.assign(static_cast<size_t>(total), -1);
+    for (int sl = 0; sl < slices; ++sl)
+        for (int lane = 0; lane < kEllSlice; ++lane) {
+            const int r = sl * kEllSlice + lane;
+            const int64_t len = r < rows ? rp[r + 1] - rp[r] : 0;
+            const int32_t first = len > 0 ? ci[rp[r]] : 0;
+            for (int q = 0; q < width[sl]; ++q) {
+                const size_t e = static_cast<size_t>(off[sl]) + static_cast<size_t>(q) * kEllSlice + lane;
+                if (q < len) {
+                    idx[e] = ci[rp[r] + q];
+                    perm[e] = static_cast<int32_t>(rp[r] + q);
+                } else {
+                    idx[e] = first;
+                }
+            }
+        }
+    return true;
+}
+
+// ELL entries / nonzeros of a CSR in slices of kEllSlice rows (the padding the layout adds).
+double ell_padding(int rows, const std::vector<int64_t>& rp) {
+    int64_t total = 0;
+    for (int sl = 0; sl * kEllSlice < rows; ++sl) {
+        int64_t w = 0;
+        for (int r = sl * kEllSlice; r < std::min(rows, (sl + 1) * kEllSlice); ++r) w = std::max(w, rp[r + 1] - rp[r]);
+        total += kEllSlice * w;
+    }
+    return rp[rows] > 0 ? static_cast<double>(total) / static_cast<double>(rp[rows]) : 0.0;
 }
 
 // Column blocks of at most 512K entries of the gathered vector (2 MB in FP32, 4 MB in FP64):
@@ -805,6 +887,65 @@ int setup_splits(vj_ctx* ctx, FastEngine& E) {
     return VJ_OK;
 }
 
+// Sliced-ELL copies for a single LP: the layout is built on the host once per upload for a
+// matrix whose padding stays within a quarter of its nonzeros (or whose ELL kernel the
+// caller forces); the values are gathered per solve from the scaled FP64 matrix.
+constexpr double kEllMaxPadding = 1.25;
+
+int setup_one_ell(vj_ctx* ctx, FastEngine& E, int rows, const std::vector<int64_t>& rp,
+                  const std::vector<int32_t>& ci, const double* vals_dev, bool forced, bool& built, double& pad,
+                  int64_t& len, DevBuf& off, DevBuf& width, DevBuf& idx, DevBuf& perm, DevBuf& val, DevBuf& rec) {
+    hipStream_t s = ctx->stream;
+    if (!built) {
+        pad = ell_padding(rows, rp);
+        if (pad > kEllMaxPadding && !forced) return VJ_OK;
+        std::vector<int32_t> hoff, hw, hidx, hperm;
+        if (!build_ell(rows, rp, ci, hoff, hw, hidx, hperm)) {
+            pad = 0.0;   // too large for 32-bit offsets: no ELL copy
+            return VJ_OK;
+        }
+        len = static_cast<int64_t>(hidx.size());
+        VJ_TRY(off.alloc(hoff.size() * 4));
+        VJ_TRY(width.alloc(hw.size() * 4));
+        VJ_TRY(idx.alloc(std::max<size_t>(hidx.size(), 1) * 4));
+        VJ_TRY(perm.alloc(std::max<size_t>(hperm.size(), 1) * 4));
+        VJ_HIP(hipMemcpy(off.p, hoff.data(), hoff.size() * 4, hipMemcpyHostToDevice));
+        VJ_HIP(hipMemcpy(width.p, hw.data(), hw.size() * 4, hipMemcpyHostToDevice));
+        if (len > 0) {
+            VJ_HIP(hipMemcpy(idx.p, hidx.data(), hidx.size() * 4, hipMemcpyHostToDevice));
+            VJ_HIP(hipMemcpy(perm.p, hperm.data(), hperm.size() * 4, hipMemcpyHostToDevice));
+        }
+        built = true;
+    }
+    if (len == 0) return VJ_OK;
+    const size_t vsz = E.prec == VJ_PREC_FP64 ? 8 : (E.prec == VJ_PREC_FP32 ? 4 : 2);
+    VJ_TRY(val.alloc(static_cast<size_t>(len) * vsz));
+    const int g = grid_for(len, kBlock);
+    if (E.prec == VJ_PREC_FP64)
+        hipLaunchKernelGGL(gather_split_vals<double>, dim3(g), dim3(kBlock), 0, s, len, perm.as<int>(), vals_dev, val.as<double>());
+    else if (E.prec == VJ_PREC_FP32) {
+        hipLaunchKernelGGL(gather_split_vals<float>, dim3(g), dim3(kBlock), 0, s, len, perm.as<int>(), vals_dev, val.as<float>());
+        VJ_TRY(rec.alloc(static_cast<size_t>(len) * 8));
+        hipLaunchKernelGGL(pack_records, dim3(g), dim3(kBlock), 0, s, len, idx.as<int>(), val.as<float>(),
+                           rec.as<vj_i32x2>());
+    } else {
+        hipLaunchKernelGGL(gather_split_vals<bf16_t>, dim3(g), dim3(kBlock), 0, s, len, perm.as<int>(), vals_dev, val.as<bf16_t>());
+    }
+    VJ_LAUNCHED();
+    return VJ_OK;
+}
+
+int setup_ell(vj_ctx* ctx, FastEngine& E) {
+    if (E.count != 1) return VJ_OK;
+    Problem& P = ctx->prob;
+    const vj_params& prm = ctx->params;
+    VJ_TRY(setup_one_ell(ctx, E, E.m, P.h_rp, P.h_ci, E.av_s.as<double>(), prm.lanes_A <= -3, E.ellBuiltA, E.ellPadA,
+                         E.ellLenA, E.ellOffA, E.ellWA, E.ellIdxA, E.ellPermA, E.ellValA, E.ellRecA));
+    VJ_TRY(setup_one_ell(ctx, E, E.n, P.h_tp, P.h_ti, E.tv_s.as<double>(), prm.lanes_At <= -3, E.ellBuiltAt, E.ellPadAt,
+                         E.ellLenAt, E.ellOffAt, E.ellWAt, E.ellIdxAt, E.ellPermAt, E.ellValAt, E.ellRecAt));
+    return VJ_OK;
+}
+
 template <class T>
 int launch_restart(vj_ctx* ctx, FastEngine& E, int force) {
     Problem& P = ctx->prob;
@@ -851,10 +992,10 @@ int enqueue_block(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
                       E.part_kr.as<double>(), E.part_kc.as<double>(), E.kkt8.as<double>()));
     ControlArgs a;
     a.px = E.part_x.as<double>();
-    a.Rx = E.SAt > 1 ? kBlock : kBlock / E.GAt;   // tile height of the kernel writing the partials
+    a.Rx = (E.SAt > 1 || E.ellAt) ? kBlock : kBlock / E.GAt;   // tile height of the kernel writing the partials
     a.py = E.part_y.as<double>();
-    a.Ry = E.SA > 1 ? kBlock : kBlock / E.GA;
-    const bool sx = E.SAt == 1 && E.streamAt, sy = E.SA == 1 && E.streamA;
+    a.Ry = (E.SA > 1 || E.ellA) ? kBlock : kBlock / E.GA;
+    const bool sx = E.SAt == 1 && E.streamAt && !E.ellAt, sy = E.SA == 1 && E.streamA && !E.ellA;
     a.xt_first = sx ? E.tfAt.as<int>() : nullptr;
     a.xt_last = sx ? E.tlAt.as<int>() : nullptr;
     a.yt_first = sy ? E.tfA.as<int>() : nullptr;
@@ -918,6 +1059,7 @@ struct StepCfg {
     int S, G;
     bool stream;
     bool rec;   // FP32 (index, value) records in the tile-streamed kernels
+    bool ell;   // sliced-ELL kernel (S = 1)
 };
 
 template <class V, class T>
@@ -929,36 +1071,43 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     const int SmA = E.SA, SmAt = E.SAt;   // split width prepared by setup_splits
     // forced: the caller's col_splits (0 = free choice, 1 = unsplit only, >1 = split only)
     const bool f32 = E.prec == VJ_PREC_FP32;
-    auto base = [&](bool tune, int lanes, double avg, int Sm, int forced) {
+    // ell: the sliced-ELL copy exists (single LP, little padding, or forced by lanes -3 / -4)
+    auto base = [&](bool tune, int lanes, double avg, int Sm, int forced, bool ell) {
         std::vector<StepCfg> c;
         if (!tune) {
-            if (lanes < 0) c.push_back({Sm, Sm > 1 ? 0 : choose_lanes(avg), Sm == 1, lanes == -2 && f32});
-            else c.push_back({Sm, lanes, false, false});
+            if (lanes <= -3 && ell) c.push_back({1, choose_lanes(avg), false, lanes == -4 && f32, true});
+            else if (lanes < 0) c.push_back({Sm, Sm > 1 ? 0 : choose_lanes(avg), Sm == 1, (lanes == -2 || lanes == -4) && f32, false});
+            else c.push_back({Sm, lanes, false, false, false});
             return c;
         }
         if (forced <= 1 || Sm == 1) {   // unsplit candidates
-            c.push_back({1, choose_lanes(avg), true, false});
-            if (f32) c.push_back({1, choose_lanes(avg), true, true});
-            for (int g : {4, 8, 16, 32}) c.push_back({1, g, false, false});
+            c.push_back({1, choose_lanes(avg), true, false, false});
+            if (f32) c.push_back({1, choose_lanes(avg), true, true, false});
+            for (int g : {4, 8, 16, 32}) c.push_back({1, g, false, false, false});
+            if (ell) {
+                c.push_back({1, choose_lanes(avg), false, false, true});
+                if (f32) c.push_back({1, choose_lanes(avg), false, true, true});
+            }
         }
         if (Sm > 1) {
-            for (int g : {1, 2, 4}) c.push_back({Sm, g, false, false});
-            c.push_back({Sm, 0, false, false});   // tile-streamed split
-            if (f32) c.push_back({Sm, 0, false, true});
+            for (int g : {1, 2, 4}) c.push_back({Sm, g, false, false, false});
+            c.push_back({Sm, 0, false, false, false});   // tile-streamed split
+            if (f32) c.push_back({Sm, 0, false, true, false});
         }
         return c;
     };
-    std::vector<StepCfg> cA = base(tuneA, prm.lanes_A, avgA, SmA, prm.col_splits_A);
-    std::vector<StepCfg> cAt = base(tuneAt, prm.lanes_At, avgAt, SmAt, prm.col_splits_At);
-    auto apply = [](const StepCfg& c, int& S, int& G, int& Gs, bool& stream, bool& rec) {
+    std::vector<StepCfg> cA = base(tuneA, prm.lanes_A, avgA, SmA, prm.col_splits_A, E.ellLenA > 0);
+    std::vector<StepCfg> cAt = base(tuneAt, prm.lanes_At, avgAt, SmAt, prm.col_splits_At, E.ellLenAt > 0);
+    auto apply = [](const StepCfg& c, int& S, int& G, int& Gs, bool& stream, bool& rec, bool& ell) {
         S = c.S;
         stream = c.S == 1 && c.stream;
         rec = c.rec;
+        ell = c.ell;
         if (c.S > 1) Gs = c.G;
         else G = std::max(c.G, 2);
     };
-    apply(cA[0], E.SA, E.GA, E.GsA, E.streamA, E.recA);
-    apply(cAt[0], E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt);
+    apply(cA[0], E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA);
+    apply(cAt[0], E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt);
     E.K_tuned = 0;
     if (E.nnz < 200000 || (cA.size() == 1 && cAt.size() == 1))
         return VJ_OK;  // launch-bound sizes: the first candidate is as good as timing
@@ -968,22 +1117,23 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     for (size_t r = 0; r < rounds; ++r) {
         const StepCfg& a = cA[std::min(r, cA.size() - 1)];
         const StepCfg& b = cAt[std::min(r, cAt.size() - 1)];
-        apply(a, E.SA, E.GA, E.GsA, E.streamA, E.recA);
-        apply(b, E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt);
+        apply(a, E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA);
+        apply(b, E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt);
         double up = 0, ud = 0;
         VJ_TRY((time_kernels<V, T>(ctx, E, av, tv, 3, &up, &ud)));
         if (prm.verbose >= 2)
-            std::fprintf(stderr, "[autotune] A: S=%d G=%d stream=%d rec=%d %.1f us | A^T: S=%d G=%d stream=%d rec=%d %.1f us\n",
-                         a.S, a.G, a.stream ? 1 : 0, a.rec ? 1 : 0, ud, b.S, b.G, b.stream ? 1 : 0, b.rec ? 1 : 0, up);
+            std::fprintf(stderr, "[autotune] A: S=%d G=%d stream=%d rec=%d ell=%d %.1f us | A^T: S=%d G=%d stream=%d rec=%d ell=%d %.1f us\n",
+                         a.S, a.G, a.stream ? 1 : 0, a.rec ? 1 : 0, a.ell ? 1 : 0, ud, b.S, b.G, b.stream ? 1 : 0,
+                         b.rec ? 1 : 0, b.ell ? 1 : 0, up);
         if (ud < bestA) { bestA = ud; wA = a; }
         if (up < bestAt) { bestAt = up; wAt = b; }
     }
-    apply(wA, E.SA, E.GA, E.GsA, E.streamA, E.recA);
-    apply(wAt, E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt);
+    apply(wA, E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA);
+    apply(wAt, E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt);
     // The FP64 check passes run lane-group kernels with E.GA / E.GAt lanes per row. A split
     // winner leaves them at the last candidate timed: give them the width that suits the rows.
-    if (E.SA > 1) E.GA = std::max(choose_lanes(avgA), 2);
-    if (E.SAt > 1) E.GAt = std::max(choose_lanes(avgAt), 2);
+    if (E.SA > 1 || E.ellA) E.GA = std::max(choose_lanes(avgA), 2);
+    if (E.SAt > 1 || E.ellAt) E.GAt = std::max(choose_lanes(avgAt), 2);
     // Iterations per check: the smallest power of two in [32, 256] at which the check (two
     // FP64 KKT passes, their reduction, control and restart kernels) costs at most 2% of the
     // iterations between checks. The KKT passes are timed on the current iterates; their
@@ -1024,6 +1174,7 @@ int run_typed(vj_ctx* ctx, FastEngine& E, vj_result* out, clk::time_point t_star
         tv = reinterpret_cast<const V*>(E.tv_lo.p);
     }
     VJ_TRY(setup_splits(ctx, E));
+    VJ_TRY(setup_ell(ctx, E));
     VJ_TRY(launch_restart<T>(ctx, E, 1));  // initial centring
     VJ_TRY((autotune<V, T>(ctx, E, av, tv)));
     // after autotuning the scratch runs left the real iterates untouched; recentre anyway
@@ -1146,8 +1297,9 @@ int run_typed(vj_ctx* ctx, FastEngine& E, vj_result* out, clk::time_point t_star
         r.time_to_target = conv_time[q];
         r.host_device_transfers = transfers;
         r.refinements = E.prec == VJ_PREC_FP64 ? 0 : z.restarts;
-        r.lanes_A = E.SA > 1 ? E.GsA : (E.streamA ? 0 : E.GA);      // 0: tile-streamed kernel
-        r.lanes_At = E.SAt > 1 ? E.GsAt : (E.streamAt ? 0 : E.GAt);
+        // 0: tile-streamed kernel, -3 / -4: sliced ELL (separate arrays / FP32 records)
+        r.lanes_A = E.ellA ? (E.recA ? -4 : -3) : (E.SA > 1 ? E.GsA : (E.streamA ? 0 : E.GA));
+        r.lanes_At = E.ellAt ? (E.recAt ? -4 : -3) : (E.SAt > 1 ? E.GsAt : (E.streamAt ? 0 : E.GAt));
         r.col_splits_A = E.SA;
         r.col_splits_At = E.SAt;
         r.check_interval = E.K;
@@ -1272,10 +1424,12 @@ int fast_benchmark(vj_ctx* ctx, int reps, vj_kernel_stat* out, int cap, int* wri
     const char* rt = E.recAt ? ",rec" : "";
     if (E.SAt > 1 && E.GsAt == 0) std::snprintf(names[0], 48, "split_stream_partials+primal_finish<%s%s>", pt, rt);
     else if (E.SAt > 1) std::snprintf(names[0], 48, "split_partials+primal_finish<%s,G=%d>", pt, E.GsAt);
+    else if (E.ellAt) std::snprintf(names[0], 48, "ell_primal_step<%s%s>", pt, rt);
     else if (E.streamAt) std::snprintf(names[0], 48, "stream_primal_step<%s%s>", pt, rt);
     else std::snprintf(names[0], 48, "primal_step<%s,G=%d>", pt, E.GAt);
     if (E.SA > 1 && E.GsA == 0) std::snprintf(names[1], 48, "split_stream_partials+dual_finish<%s%s>", pt, ra);
     else if (E.SA > 1) std::snprintf(names[1], 48, "split_partials+dual_finish<%s,G=%d>", pt, E.GsA);
+    else if (E.ellA) std::snprintf(names[1], 48, "ell_dual_step<%s%s>", pt, ra);
     else if (E.streamA) std::snprintf(names[1], 48, "stream_dual_step<%s%s>", pt, ra);
     else std::snprintf(names[1], 48, "dual_step<%s,G=%d>", pt, E.GA);
     const double us[2] = {up, ud};

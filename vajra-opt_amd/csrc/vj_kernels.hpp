@@ -591,13 +591,15 @@ __global__ __launch_bounds__(kBlock) void dual_finish(
     }
 }
 
-// Gather of the scaled matrix into split order: out[q] = in[perm[q]], converted to V.
+// Gather of the scaled matrix into split or sliced-ELL order: out[q] = in[perm[q]] (0 where
+// perm[q] < 0), converted to V.
 template <class V>
 __global__ void gather_split_vals(int64_t nnz, const int* __restrict__ perm, const double* __restrict__ in,
                                   V* __restrict__ out) {
     const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (q >= nnz) return;
-    const double v = in[perm[q]];
+    const int src = perm[q];
+    const double v = src >= 0 ? in[src] : 0.0;   // a negative entry marks sliced-ELL padding
     if constexpr (sizeof(V) == 8) {
         out[q] = v;
     } else if constexpr (sizeof(V) == 4) {
@@ -613,6 +615,80 @@ __global__ void gather_split_vals(int64_t nnz, const int* __restrict__ perm, con
 // FP64 KKT passes (the check, and the audit on original data when the unscale arrays
 // are null). Row pass: ax = A x; q0 = sum r_p^2 (original units), q1 = row part of the
 // dual objective, q2 = row-dual sign violations^2 (original units).
+// ---------------------------------------------------------------------------------
+// Sliced-ELL fused steps (single LP). Rows come in slices of 64, one per wavefront; a slice
+// stores its rows' entries column-major, padded to its longest row with zero-valued entries
+// that point at the row's first column. Lane r of the wavefront reads entry q of its row at
+// off[s] + 64 q + r: one coalesced load per q, no row pointers, no LDS. The row sum runs in
+// index order. The autotuner offers it when padding adds at most a quarter to the entries,
+// as when every row has the same length.
+constexpr int kEllSlice = 64;
+
+template <class Ent, class T>
+__device__ __forceinline__ T ell_row_sum(int r, const int* __restrict__ off, const int* __restrict__ width,
+                                         const Ent& ent, const T* __restrict__ vec) {
+    const int sl = r / kEllSlice;
+    const int64_t base = static_cast<int64_t>(off[sl]) + (r % kEllSlice);
+    const int w = width[sl];
+    T acc = 0;
+    int q = 0;
+    for (; q + 3 < w; q += 4) {
+        int c[4];
+        T a[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) ent.load(base + static_cast<int64_t>(q + u) * kEllSlice, c[u], a[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc += a[u] * vec[c[u]];
+    }
+    for (; q < w; ++q) {
+        int c;
+        T a;
+        ent.load(base + static_cast<int64_t>(q) * kEllSlice, c, a);
+        acc += a * vec[c];
+    }
+    return acc;
+}
+
+template <class Ent, class T>
+__global__ __launch_bounds__(kBlock) void ell_primal_step(
+    int n, const int* __restrict__ off, const int* __restrict__ width, Ent ent, const T* __restrict__ y,
+    const T* __restrict__ dhat, const T* __restrict__ lcs, const T* __restrict__ ucs, T* __restrict__ x,
+    T* __restrict__ xbar, const double* __restrict__ xhat, double* __restrict__ xfull,
+    const InstState* __restrict__ st, double* __restrict__ part, int it, int flags, double rho) {
+    if (st[0].active == 0) return;
+    const int j = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
+    double c[3] = {0.0, 0.0, 0.0};
+    if (j < n) {
+        const PrimalOps<T> o = primal_ops<T>(j, x, dhat, lcs, ucs);   // issued before the gathers
+        const T acc = ell_row_sum<Ent, T>(j, off, width, ent, y);
+        primal_update<T>(acc, o, j, st[0], it, flags, rho, x, xbar, xhat, xfull, c);
+    }
+    if (flags) {
+        const int mask = ((flags & 1) ? 1 : 0) | ((flags & 2) ? 6 : 0);
+        write_tile_partials<3, false>(c, 0, 0, 0, blockIdx.x, part, mask);
+    }
+}
+
+template <class Ent, class T>
+__global__ __launch_bounds__(kBlock) void ell_dual_step(
+    int m, const int* __restrict__ off, const int* __restrict__ width, Ent ent, const T* __restrict__ xbar,
+    const T* __restrict__ yhat_t, const T* __restrict__ lrs, const T* __restrict__ urs, T* __restrict__ y,
+    const double* __restrict__ yhat, double* __restrict__ yfull, const InstState* __restrict__ st,
+    double* __restrict__ part, int it, int flags, double rho) {
+    if (st[0].active == 0) return;
+    const int i = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
+    double c[3] = {0.0, 0.0, 0.0};
+    if (i < m) {
+        const DualOps<T> o = dual_ops<T>(i, y, yhat_t, lrs, urs);
+        const T w = ell_row_sum<Ent, T>(i, off, width, ent, xbar);
+        dual_update<T>(w, o, i, st[0], it, flags, rho, y, yhat, yfull, c);
+    }
+    if (flags) {
+        const int mask = ((flags & 1) ? 1 : 0) | ((flags & 2) ? 6 : 0);
+        write_tile_partials<3, false>(c, 0, 0, 0, blockIdx.x, part, mask);
+    }
+}
+
 // Interleave an index array and an FP32 value array into 8-byte (index, value) records.
 __global__ void pack_records(int64_t nnz, const int* __restrict__ idx, const float* __restrict__ val,
                              vj_i32x2* __restrict__ rec) {

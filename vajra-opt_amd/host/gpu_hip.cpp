@@ -55,7 +55,7 @@ std::string kernel_mode(const vj_params& p, const vj_result& r) {
     if (p.mode == VJ_MODE_REFERENCE) return "reference_map_exact_order";
     std::string s = "restarted_halpern_";
     s += p.precision == VJ_PREC_FP64 ? "fp64" : (p.precision == VJ_PREC_FP32 ? "fp32_refined" : "bf16_refined");
-    s += r.col_splits_A > 1 || r.col_splits_At > 1 ? "_xcd_split" : (r.lanes_A == 0 ? "_tile_stream" : "_lane_groups");
+    s += r.col_splits_A > 1 || r.col_splits_At > 1 ? "_xcd_split" : (r.lanes_A == 0 ? "_tile_stream" : (r.lanes_A < 0 ? "_sliced_ell" : "_lane_groups"));
     s += p.use_graph ? "_graph" : "";
     return s;
 }
