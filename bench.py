@@ -237,26 +237,37 @@ def bench_batch(vp, kind, args):
 
 
 def cpu_baseline(lp, args, r64):
+    """The reference's CPU PDHG timed on a bounded sample of the same LP, one thread: its own
+    `solve_pdhg_cpu` (src/solvers/pdhg/pdhg.cpp:448-453) compiled from its sources
+    (oracle/_ref, kind "reference") where that library was built, else the oracle's C
+    restatement of the same iteration map (kind "port")."""
     try:
         import oracle as O
     except Exception as e:
         log(f"cpu baseline skipped: {e}")
         return None
     iters = args.cpu_iters
-    o = O.default_opts(max_iterations=10**9, stop_at_iter=iters, check_frequency=100)
+    gpu_iters = r64.iterations if r64 is not None else None
     t0 = time.perf_counter()
-    r = O.pdhg_solve(lp, o)
+    if O.have_ref():
+        kind = "reference"
+        r = O.ref_solve(lp, algorithm="pdhg_cpu", mode=0, iteration_limit=iters, check_frequency=100)
+        what = (f"the reference's solve_pdhg_cpu (its sources compiled with g++ -O2, 1 thread) on the same 1M x 2M LP, "
+                f"iteration_limit {iters}, check_frequency 100")
+    else:
+        kind = "port"
+        r = O.pdhg_solve(lp, O.default_opts(max_iterations=10**9, stop_at_iter=iters, check_frequency=100))
+        what = f"the oracle's C restatement of the reference CPU PDHG map (1 thread) on the same 1M x 2M LP, {iters} iterations"
     wall = time.perf_counter() - t0
     per_it = r["iteration_time"] / iters
-    gpu_iters = r64.iterations if r64 is not None else None
     proj = per_it * gpu_iters + r["setup_time"] if gpu_iters else None
-    log(f"cpu: setup {r['setup_time']:.2f}s, {iters} iterations {r['iteration_time']:.2f}s ({per_it*1e3:.1f} ms/it)")
+    log(f"cpu ({kind}): setup {r['setup_time']:.2f}s, {iters} iterations {r['iteration_time']:.2f}s ({per_it*1e3:.1f} ms/it)")
     return {"value": round(proj, 2) if proj else None, "unit": "s",
-            "cores": 1, "kind": "port",
-            "sample": f"reference CPU PDHG map (oracle, 1 thread) on the same 1M x 2M LP: setup (Ruiz + power "
-                      f"iteration) + {iters} iterations, {wall:.1f}s wall; value = setup + per-iteration time x the "
-                      f"FP64 GPU engine's iteration count ({gpu_iters}); the reference iteration map itself needs "
-                      f"more iterations (it did not reach rel-KKT 1e-6 within 1e6 iterations on the 2,000 x 5,000 LP)",
+            "cores": 1, "kind": kind,
+            "sample": f"{what}: setup (Ruiz + power iteration) {r['setup_time']:.2f}s + {iters} iterations "
+                      f"{r['iteration_time']:.2f}s, {wall:.1f}s wall; value = setup + per-iteration time x the FP64 GPU "
+                      f"engine's iteration count ({gpu_iters}); the reference iteration map itself needs more "
+                      f"iterations (it did not reach rel-KKT 1e-6 within 1e6 iterations on the 2,000 x 5,000 LP)",
             "setup_s": r["setup_time"], "s_per_iteration": per_it, "host_nproc": os.cpu_count(),
             "host_cpu": host_cpu_model()}
 
