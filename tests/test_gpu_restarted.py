@@ -50,6 +50,22 @@ def test_cfg0_to_1e6(engine, prec):
     print(prec, r.iterations, r.restart_count, r.iteration_time, k[:4])
 
 
+def test_fp32_refinement_to_1e8(engine):
+    """FP32 iterates with FP64 refinement reach rel-KKT 1e-8 (oracle-checked) on the
+    2,000 x 5,000 LP, and the objective agrees with the FP64 engine's at 1e-8 to 1e-7
+    relative: the refinement restarts (on the FP64 rel-KKT of the checks) remove the error of
+    the rounded FP32 data."""
+    lp = lp_cases.cfg0_lp()
+    print("[case] cfg0 fp32 to 1e-8", flush=True)
+    r, x, rd, y = solve(engine, lp, tol=1e-8, precision="fp32")
+    assert r.status == 1, r.as_dict()
+    check_point(lp, r, x, y, 1e-8)
+    r64, x64, _, _ = solve(engine, lp, tol=1e-8)
+    assert r64.status == 1, r64.as_dict()
+    o32, o64 = float(np.dot(lp.c, x)), float(np.dot(lp.c, x64))
+    assert abs(o32 - o64) <= 1e-7 * (1 + abs(o64)), (o32, o64)
+
+
 def test_cfg0_objective_against_highs(engine):
     """The FP64 engine's objective on the 2,000 x 5,000 LP against the optimum SciPy-HiGHS
     found for the same LP (the reference's own dual simplex fails on it: status 9, recorded

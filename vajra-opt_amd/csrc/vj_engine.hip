@@ -892,6 +892,11 @@ int setup_splits(vj_ctx* ctx, FastEngine& E) {
 // caller forces); the values are gathered per solve from the scaled FP64 matrix.
 constexpr double kEllMaxPadding = 1.25;
 
+// FP32 iterates: a check whose FP64 rel-KKT exceeds this factor times the least rel-KKT of
+// the epoch triggers a restart, i.e. an FP64 refinement step. (BF16 iterates do not use it:
+// their rounding noise moves the rel-KKT by more than this factor from check to check.)
+constexpr double kRefineDrift = 2.0;
+
 int setup_one_ell(vj_ctx* ctx, FastEngine& E, int rows, const std::vector<int64_t>& rp,
                   const std::vector<int32_t>& ci, const double* vals_dev, bool forced, bool& built, double& pad,
                   int64_t& len, DevBuf& off, DevBuf& width, DevBuf& idx, DevBuf& perm, DevBuf& val, DevBuf& rec) {
@@ -1007,6 +1012,7 @@ int enqueue_block(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     a.restart_sufficient = ctx->params.restart_sufficient;
     a.restart_necessary = ctx->params.restart_necessary;
     a.restart_artificial = ctx->params.restart_artificial;
+    a.kkt_drift = E.prec == VJ_PREC_FP32 ? kRefineDrift : 0.0;
     a.pw_theta = ctx->params.primal_weight_smoothing;
     a.hist = E.hist_cap > 0 ? E.hist.as<double>() : nullptr;
     a.hist_cap = E.hist_cap;

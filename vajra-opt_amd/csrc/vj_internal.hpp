@@ -83,6 +83,7 @@ struct alignas(16) InstState {
     double norm_b, norm_c;             // original-space norms for the rel-KKT denominators
     double obj_scale, bound_scale;     // cs, bs of the rescaled problem
     double tol;
+    double kkt_min;                    // least rel-KKT at the checks of the current epoch (0: none yet)
     double time_to_target;             // filled by host
     int64_t k;                         // iterations since the last restart (epoch index)
     int64_t iters;                     // total iterations

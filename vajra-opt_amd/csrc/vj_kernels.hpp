@@ -839,7 +839,11 @@ __global__ __launch_bounds__(kBlock) void kkt_finish(
 // Restart rule (reflected restarted Halpern PDHG, Lu & Yang): with r the weighted
 // fixed-point residual sqrt(omega |dx|^2 + |dy|^2 / omega), restart when
 // r <= 0.2 r0, or r <= 0.8 r0 and r > r_prev, or the epoch exceeds a fraction (default
-// 0.5) of all iterations. At a restart the primal weight is smoothed as in PDLP:
+// 0.5) of all iterations. With FP32 iterates (kkt_drift > 0) it also restarts when the
+// FP64 rel-KKT of a check exceeds kkt_drift times the least rel-KKT of the epoch: the
+// working-precision iteration then converges towards the fixed point of its rounded data
+// while the true error grows, and the restart's FP64 re-centring is the refinement step
+// that removes that error. At a restart the primal weight is smoothed as in PDLP:
 // log omega <- theta log(|Dy|/|Dx|) + (1 - theta) log omega (theta = 0.5 by default).
 struct ControlArgs {
     const double* px; int Rx;      // primal-step partials (cols), fixed tile height Rx
@@ -853,6 +857,7 @@ struct ControlArgs {
     const int* col_off;
     int K;
     double restart_sufficient, restart_necessary, restart_artificial, pw_theta;
+    double kkt_drift;              // 0: off (FP64 and BF16 iterates)
     double* hist;                  // per instance: [checks] rel_kkt history (may be null)
     int hist_cap;
 };
@@ -895,10 +900,13 @@ __global__ __launch_bounds__(kBlock) void control_kernel(InstState* __restrict__
         if (s.restarted_prev) {
             s.r0 = rf;
             s.r_prev = rf;
+            s.kkt_min = k8[0];
         }
+        if (s.kkt_min <= 0.0 || k8[0] < s.kkt_min) s.kkt_min = k8[0];
         const bool restart = (r <= a.restart_sufficient * s.r0) ||
                              (r <= a.restart_necessary * s.r0 && r > s.r_prev) ||
-                             (static_cast<double>(s.k) >= a.restart_artificial * static_cast<double>(s.iters));
+                             (static_cast<double>(s.k) >= a.restart_artificial * static_cast<double>(s.iters)) ||
+                             (a.kkt_drift > 0.0 && k8[0] > a.kkt_drift * s.kkt_min);
         s.r_prev = r;
         s.restarted_prev = 0;
         if (restart) {
