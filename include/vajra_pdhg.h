@@ -123,6 +123,10 @@ typedef struct vj_params {
     double restart_artificial;     /* default 0.5 (PDLP: 0.36) */
     double primal_weight_smoothing;/* theta in log w <- theta log(|dy|/|dx|) + (1-theta) log w; default 0.5 */
     int power_iterations;          /* RESTARTED: power-iteration steps for ||A||_2; default 64 */
+    double restart_drift;          /* RESTARTED: also restart (an FP64 refinement step) when a check's
+                                      FP64 rel-KKT exceeds restart_drift times the least rel-KKT of
+                                      the epoch; 0 = default (2 with FP32 iterates, off with FP64
+                                      and BF16), < 0 = off, > 1 = this factor for every precision */
 } vj_params;
 /* Fill defaults for a mode (reference defaults for VJ_MODE_REFERENCE). */
 void vj_params_default(vj_params* p, int mode);

@@ -66,6 +66,17 @@ def test_fp32_refinement_to_1e8(engine):
     assert abs(o32 - o64) <= 1e-7 * (1 + abs(o64)), (o32, o64)
 
 
+def test_restart_drift_parameter_range(engine):
+    """restart_drift in (0, 1] would restart at every check: rejected as an invalid
+    argument; 0 (default), negative (off) and factors above 1 are accepted."""
+    engine.upload(lp_cases.cfg0_lp())
+    for bad in (0.5, 1.0, float("nan")):
+        with pytest.raises(vp.VjError):
+            engine.set_params(vp.default_params(vp.MODE_RESTARTED, restart_drift=bad))
+    for ok in (0.0, -1.0, 1.5, 4.0):
+        engine.set_params(vp.default_params(vp.MODE_RESTARTED, restart_drift=ok))
+
+
 def test_cfg0_objective_against_highs(engine):
     """The FP64 engine's objective on the 2,000 x 5,000 LP against the optimum SciPy-HiGHS
     found for the same LP (the reference's own dual simplex fails on it: status 9, recorded

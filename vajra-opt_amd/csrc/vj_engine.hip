@@ -1012,7 +1012,8 @@ int enqueue_block(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     a.restart_sufficient = ctx->params.restart_sufficient;
     a.restart_necessary = ctx->params.restart_necessary;
     a.restart_artificial = ctx->params.restart_artificial;
-    a.kkt_drift = E.prec == VJ_PREC_FP32 ? kRefineDrift : 0.0;
+    const double drift = ctx->params.restart_drift;
+    a.kkt_drift = drift > 1.0 ? drift : (drift == 0.0 && E.prec == VJ_PREC_FP32 ? kRefineDrift : 0.0);
     a.pw_theta = ctx->params.primal_weight_smoothing;
     a.hist = E.hist_cap > 0 ? E.hist.as<double>() : nullptr;
     a.hist_cap = E.hist_cap;
