@@ -536,9 +536,10 @@ void build_stream_tiles(int rows, const std::vector<int64_t>& rp, const std::vec
 }
 
 // Row tiles of a split layout (S blocks, absolute pointers bp[k*(rows+1) + r]) for the
-// tile-streamed split kernel: at most kSplitRows rows and at most `cap` entries in every
+// tile-streamed split kernel: at most max_rows rows and at most `cap` entries in every
 // block; a row longer than `cap` in some block is a tile of its own.
-void build_split_tiles(int rows, int S, int cap, const std::vector<int32_t>& bp, std::vector<int32_t>& tile_row) {
+void build_split_tiles(int rows, int S, int cap, int max_rows, const std::vector<int32_t>& bp,
+                       std::vector<int32_t>& tile_row) {
     tile_row.assign(1, 0);
     auto fits = [&](int r, int e) {
         for (int k = 0; k < S; ++k) {
@@ -551,7 +552,7 @@ void build_split_tiles(int rows, int S, int cap, const std::vector<int32_t>& bp,
     while (r < rows) {
         int e = r + 1;
         if (fits(r, e))
-            while (e < rows && e - r < kSplitRows && fits(r, e + 1)) ++e;
+            while (e < rows && e - r < max_rows && fits(r, e + 1)) ++e;
         tile_row.push_back(e);
         r = e;
     }
@@ -838,11 +839,13 @@ int setup_one_split(vj_ctx* ctx, FastEngine& E, int S, int rows, int cols, const
         VJ_HIP(hipMemcpy(perm.p, hperm.data(), static_cast<size_t>(nnz) * 4, hipMemcpyHostToDevice));
         built = S;
     }
-    // tiles depend on the LDS capacity in entries of the working precision
+    // tiles depend on the LDS capacity in entries and on the tile height of the working
+    // precision (both differ between FP64 and the 4-byte iterates, so cap keys the rebuild)
     const int cap = E.prec == VJ_PREC_FP64 ? split_cap<double>() : split_cap<float>();
+    const int max_rows = E.prec == VJ_PREC_FP64 ? split_rows<double>() : split_rows<float>();
     if (tiles_cap != cap) {
         std::vector<int32_t> htile;
-        build_split_tiles(rows, S, cap, hbp, htile);
+        build_split_tiles(rows, S, cap, max_rows, hbp, htile);
         ntiles = static_cast<int>(htile.size()) - 1;
         VJ_TRY(tiles.alloc(htile.size() * 4));
         VJ_HIP(hipMemcpy(tiles.p, htile.data(), htile.size() * 4, hipMemcpyHostToDevice));

@@ -500,13 +500,18 @@ __global__ __launch_bounds__(kBlock) void split_partials(int rows, int S, const 
 
 // Tile-streamed column-split partials. Workgroup w handles block k = w % S (the same XCD
 // mapping as split_partials) and row tile t = w / S. The host builds tiles common to all
-// blocks: at most kSplitRows rows, and at most split_cap<T>() entries in every block; a
+// blocks: at most split_rows<T>() rows, and at most split_cap<T>() entries in every block; a
 // single row longer than that in some block is a tile of its own, reduced by the whole
 // workgroup.
 // The block's span of values and indices is streamed with coalesced loads independent of
-// the row structure, the products go to LDS, and each thread adds up to kSplitRows/256 rows
-// of the tile in index order (deterministic).
-constexpr int kSplitRows = 1024;
+// the row structure, the products go to LDS, and each thread adds up to split_rows<T>()/256
+// rows of the tile in index order (deterministic).
+// Rows per tile: 256 for 4-byte iterates, 512 for FP64. Shorter tiles balance the CUs better;
+// on the 1M x 2M LP (about 3 entries per row and block) the FP32 A x_bar step took 82.8 us at
+// 256 rows, 84.2 at 512, 87.0 at 1024 and 92 at 2048, the FP64 steps 119.1 + 112.8 us at 512
+// rows, 120.4 + 114.3 at 256 and 122.1 + 115.6 at 1024.
+template <class T>
+constexpr int split_rows() { return sizeof(T) == 8 ? 512 : 256; }
 // LDS product buffer of one split tile: 16 KB (4096 FP32 / 2048 FP64 entries), so that
 // eight workgroups of four waves fit a CU in either precision (8 and 32 KB measured the same).
 constexpr int kSplitLdsBytes = 16384;
@@ -531,7 +536,7 @@ __global__ __launch_bounds__(kBlock) void split_stream_partials(int rows, int S,
         if (threadIdx.x == 0) out[r0] = acc;
         return;
     }
-    constexpr int RPT = kSplitRows / kBlock;
+    constexpr int RPT = split_rows<T>() / kBlock;
     int lo[RPT], hi[RPT];
 #pragma unroll
     for (int u = 0; u < RPT; ++u) {
