@@ -29,12 +29,17 @@ def check_point(lp, r, x, y, tol):
     return k
 
 
+@pytest.mark.parametrize("lanes", [0, -3], ids=["autotuned", "sliced_ell"])
 @pytest.mark.parametrize("name", sorted(lp_cases.kat_lps()))
-def test_kat_fp64(engine, name):
+def test_kat_fp64(engine, name, lanes):
+    """The reference's known-answer LPs (tiny, with empty and singleton rows and fewer rows
+    than one ELL slice), with the autotuned kernels and with the sliced-ELL kernels forced."""
     lp, obj, xs = lp_cases.kat_lps()[name]
-    print(f"[case] restarted fp64 kat {name}", flush=True)
-    r, x, rd, y = solve(engine, lp)
+    print(f"[case] restarted fp64 kat {name} lanes={lanes}", flush=True)
+    r, x, rd, y = solve(engine, lp, lanes_A=lanes, lanes_At=lanes)
     assert r.status == 1, r.as_dict()
+    if lanes == -3:
+        assert (r.lanes_A, r.lanes_At) == (-3, -3), r.as_dict()
     check_point(lp, r, x, y, 1e-6)
     got = float(np.dot(lp.c, x)) + lp.offset
     assert abs(got - obj) <= 1e-5 * (1 + abs(obj))
@@ -94,12 +99,19 @@ def test_cfg0_objective_against_highs(engine):
     assert abs(got - opt) <= 1e-6 * (1 + abs(opt)), (got, opt)
 
 
+@pytest.mark.parametrize("prec,lanes", [("fp64", 0), ("fp64", -3), ("fp32", -4)])
 @pytest.mark.parametrize("name", ["afiro", "crude_blend"])
-def test_mps_fixtures(engine, name):
+def test_mps_fixtures(engine, name, prec, lanes):
+    """Netlib afiro and crude_blend to rel-KKT 1e-8 against their published optima, with the
+    autotuned kernels and with the sliced-ELL kernels forced (FP64, and FP32 records with FP64
+    refinement)."""
     lp = lp_cases.mps_lp(name)
     opt = lp_cases.MPS_OPTIMA[name]
-    r, x, rd, y = solve(engine, lp, tol=1e-8)
+    print(f"[case] mps {name} {prec} lanes={lanes}", flush=True)
+    r, x, rd, y = solve(engine, lp, tol=1e-8, precision=prec, lanes_A=lanes, lanes_At=lanes)
     assert r.status == 1, r.as_dict()
+    if lanes < 0:
+        assert (r.lanes_A, r.lanes_At) == (lanes, lanes), r.as_dict()
     check_point(lp, r, x, y, 1e-8)
     got = float(np.dot(lp.c, x)) + lp.offset
     assert abs(got - opt) <= 1e-6 * (1 + abs(opt))
