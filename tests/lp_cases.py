@@ -168,3 +168,31 @@ MPS_OPTIMA = {"afiro": -464.75314286, "crude_blend": 214.145945946}
 
 def mps_lp(name):
     return read_mps(os.path.join(GOLDEN, f"{name}.mps"))
+
+
+def structured_lp(seed, m, n):
+    """A random LP with the structure the reference's solver treats specially: empty rows
+    and columns, fixed, free and one-sided columns, one-sided and equality rows, tiny
+    coefficients, a nonzero objective offset and a sense chosen by the seed."""
+    rng = np.random.default_rng(1000 + seed)
+    A = rng.standard_normal((m, n)) * (rng.random((m, n)) < 0.25)
+    if m > 2:
+        A[1, :] = 0.0                       # empty row
+    if n > 3:
+        A[:, 2] = 0.0                       # empty column
+    # tiny coefficients just above the reference's triplet drop threshold (kZeroDrop = 1e-11,
+    # sparse.cpp:57-60), so the reference's model keeps them as the oracle does
+    A[rng.random((m, n)) < 0.03] = 5e-11
+    c = rng.standard_normal(n)
+    lc, uc = -rng.random(n), 1 + rng.random(n)
+    lc[::5] = -1e20                         # free below
+    uc[1::6] = 1e20                         # free above
+    lc[3::7] = uc[3::7] = 0.5               # fixed
+    lr, ur = -np.ones(m) - rng.random(m), np.ones(m) + rng.random(m)
+    lr[::4] = -1e20                         # <= rows
+    ur[2::5] = 1e20                         # >= rows
+    lr[3::6] = ur[3::6]                     # equality rows
+    lp = vp.LP.from_dense(A, c, lc, uc, lr, ur)
+    lp.maximize = bool(seed % 2)
+    lp.offset = float(seed) - 1.5
+    return lp

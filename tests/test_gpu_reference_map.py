@@ -94,3 +94,20 @@ def test_reference_afiro_iteration_limit(engine):
     o = oracle_solve(lp, max_iterations=10)
     assert r.status in (5, 6) and r.status == o["status"] and r.iterations == o["iterations"] == 11
     assert np.array_equal(x, o["x"])
+
+
+@pytest.mark.parametrize("seed,m,n", [(0, 12, 30), (1, 30, 12), (2, 1, 9), (3, 9, 1), (4, 40, 40), (5, 3, 3)])
+def test_structured_lps_bit_exact(engine, seed, m, n):
+    """LPs with empty rows and columns, fixed, free and one-sided columns, equality and
+    one-sided rows, tiny coefficients, offsets and both senses (the same LPs the oracle is
+    pinned on against the compiled reference): the device reference map equals the oracle
+    bit for bit, at several check frequencies, including a limit reached between checks."""
+    lp = lp_cases.structured_lp(seed, m, n)
+    for iters, freq in ((1, 1), (23, 7), (400, 100), (1500, 40)):
+        print(f"[case] structured seed={seed} {m}x{n} iters={iters} freq={freq}", flush=True)
+        r, x, rd, _ = gpu_ref_solve(engine, lp, max_iterations=iters, check_frequency=freq)
+        o = oracle_solve(lp, max_iterations=iters, check_frequency=freq)
+        assert np.array_equal(x, o["x"]) and np.array_equal(rd, o["row_dual"])
+        assert r.status == o["status"] and r.iterations == o["iterations"]
+        assert r.restart_count == o["restart_count"]
+        assert r.tau == o["final_tau"] and r.sigma == o["final_sigma"]

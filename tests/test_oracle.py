@@ -250,3 +250,36 @@ def test_mps_reader_against_published_optima():
         assert abs(meta[f"{name}_highs_objective"] - lp_cases.MPS_OPTIMA[name]) < 1e-6 * (1 + abs(lp_cases.MPS_OPTIMA[name]))
     afiro = lp_cases.mps_lp("afiro")
     assert (afiro.m, afiro.n, afiro.nnz) == (27, 32, 83)
+
+
+@pytest.mark.skipif(not O.have_ref(), reason="oracle/_ref not built (no reference checkout)")
+@pytest.mark.parametrize("seed,m,n", [(0, 12, 30), (1, 30, 12), (2, 1, 9), (3, 9, 1), (4, 40, 40), (5, 3, 3)])
+def test_oracle_equals_compiled_reference_structured(seed, m, n):
+    """The restated CPU PDHG (oracle) against the reference's solve_pdhg_cpu compiled from its
+    sources, bit for bit (iterates, duals, row values, status, iteration count, step sizes,
+    restarts), on structured LPs and at several check frequencies, including the limit
+    reached between checks."""
+    lp = lp_cases.structured_lp(seed, m, n)
+    for iters, freq in ((1, 1), (23, 7), (400, 100), (1500, 40)):
+        o = O.pdhg_solve(lp, O.default_opts(max_iterations=iters, check_frequency=freq))
+        r = O.ref_solve(lp, iteration_limit=iters, check_frequency=freq)
+        assert o["status"] == r["status"] and o["iterations"] == r["iterations"], (iters, freq, o["status"], r["status"])
+        assert np.array_equal(o["x"], r["x"]) and np.array_equal(o["row_dual"], r["row_dual"])
+        assert np.array_equal(o["col_dual"], r["col_dual"]) and np.array_equal(o["row_value"], r["row_value"])
+        assert o["tau"] == r["tau"] and o["sigma"] == r["sigma"] and o["restart_count"] == r["restart_count"]
+        assert o["objective"] == r["objective"]
+
+
+@pytest.mark.skipif(not O.have_ref(), reason="oracle/_ref not built (no reference checkout)")
+@pytest.mark.parametrize("m,n", [(0, 4), (3, 0)])
+def test_oracle_trivial_sizes_equal_compiled_reference(m, n):
+    """LPs without rows or without columns take the reference's trivial exits
+    (pdhg.cpp:88-127); the oracle returns the same status, point and objective."""
+    rng = np.random.default_rng(7)
+    A = np.zeros((m, n))
+    lp = vp.LP.from_dense(A, rng.standard_normal(n), -rng.random(n), 1 + rng.random(n), -np.ones(m), np.ones(m))
+    o = O.pdhg_solve(lp, O.default_opts(max_iterations=100, check_frequency=10))
+    r = O.ref_solve(lp, iteration_limit=100, check_frequency=10)
+    assert o["status"] == r["status"] and o["iterations"] == r["iterations"]
+    assert np.array_equal(o["x"], r["x"]) and np.array_equal(o["row_dual"], r["row_dual"])
+    assert o["objective"] == r["objective"]
