@@ -196,3 +196,65 @@ def structured_lp(seed, m, n):
     lp.maximize = bool(seed % 2)
     lp.offset = float(seed) - 1.5
     return lp
+
+
+def structured_feasible_lp(seed, m, n):
+    """The structure of structured_lp (empty row and column, fixed, free and one-sided
+    columns, one-sided, ranged and equality rows, tiny coefficients, offset, both senses) in
+    a feasible, bounded LP: the row bounds are built around A x0 for a point x0 inside the
+    column bounds, and the cost is c = A^T y0 + z with y0 of the sign each row type allows
+    and z of the sign each column's finite bounds allow (zero on free columns), so (y0, z)
+    is dual feasible."""
+    rng = np.random.default_rng(2000 + seed)
+    A = rng.standard_normal((m, n)) * (rng.random((m, n)) < 0.25)
+    if m > 2:
+        A[1, :] = 0.0
+    if n > 3:
+        A[:, 2] = 0.0
+    A[rng.random((m, n)) < 0.03] = 5e-11
+    lc, uc = -rng.random(n), 1 + rng.random(n)
+    lc[::5] = -1e20
+    uc[1::6] = 1e20
+    lc[3::7] = uc[3::7] = 0.5
+    x0 = np.where(lc > -1e20, lc, np.where(uc < 1e20, uc - 2.0, rng.standard_normal(n))) + rng.random(n) * 0.5
+    x0 = np.minimum(x0, np.where(uc < 1e20, uc, x0))
+    x0[3::7] = 0.5
+    ax = A @ x0
+    lr, ur = ax - rng.random(m), ax + rng.random(m)
+    lr[::4] = -1e20
+    ur[2::5] = 1e20
+    lr[3::6] = ur[3::6] = ax[3::6]
+    y0 = rng.standard_normal(m)
+    y0 = np.where(lr <= -1e20, -np.abs(y0), np.where(ur >= 1e20, np.abs(y0), y0))
+    y0[(lr <= -1e20) & (ur >= 1e20)] = 0.0   # a free row carries no dual
+    z = rng.random(n)
+    z = np.where((lc > -1e20) & (uc < 1e20), rng.standard_normal(n), np.where(lc > -1e20, z, np.where(uc < 1e20, -z, 0.0)))
+    c = A.T @ y0 + z
+    maximize = bool(seed % 2)
+    lp = vp.LP.from_dense(A, -c if maximize else c, lc, uc, lr, ur)
+    lp.maximize = maximize
+    lp.offset = float(seed) - 1.5
+    return lp
+
+
+def highs_optimum(lp):
+    """Objective of lp by SciPy's HiGHS (an independent solver), or None if it does not
+    report an optimum."""
+    from scipy.optimize import linprog
+    A = lp.dense()
+    sgn = -1.0 if lp.maximize else 1.0
+    fin = lambda v: None if abs(v) >= 1e20 else float(v)
+    bounds = [(fin(l), fin(u)) for l, u in zip(lp.col_lower, lp.col_upper)]
+    a_ub, b_ub, a_eq, b_eq = [], [], [], []
+    for i in range(lp.m):
+        lo, hi = lp.row_lower[i], lp.row_upper[i]
+        if lo == hi:
+            a_eq.append(A[i]); b_eq.append(lo)
+            continue
+        if hi < 1e20:
+            a_ub.append(A[i]); b_ub.append(hi)
+        if lo > -1e20:
+            a_ub.append(-A[i]); b_ub.append(-lo)
+    r = linprog(sgn * lp.c, A_ub=np.array(a_ub) if a_ub else None, b_ub=b_ub or None,
+                A_eq=np.array(a_eq) if a_eq else None, b_eq=b_eq or None, bounds=bounds, method="highs")
+    return sgn * r.fun + lp.offset if r.status == 0 else None

@@ -234,3 +234,24 @@ def test_sliced_ell_variants(engine, prec, lanes):
     assert r0.status == 1 and (r0.lanes_A, r0.lanes_At) == (0, 0)
     o0, o1 = float(np.dot(lp.c, x0)), float(np.dot(lp.c, x1))
     assert abs(o1 - o0) <= 2e-5 * (1 + abs(o0)), (o1, o0)
+
+
+STRUCTURED = [(0, 12, 30), (1, 30, 12), (2, 1, 9), (3, 9, 1), (4, 40, 40), (5, 60, 150), (6, 150, 60), (7, 3, 3)]
+
+
+@pytest.mark.parametrize("prec,lanes", [("fp64", 0), ("fp32", 0), ("fp64", -3)])
+@pytest.mark.parametrize("seed,m,n", STRUCTURED)
+def test_structured_lps_against_highs(engine, seed, m, n, prec, lanes):
+    """Feasible, bounded LPs with empty rows and columns, fixed, free and one-sided columns,
+    equality, ranged and one-sided rows, tiny coefficients, offsets and both senses: the
+    restarted engine reaches rel-KKT 1e-6 (oracle-checked on the downloaded point) and the
+    objective of SciPy's HiGHS to 1e-5 relative."""
+    lp = lp_cases.structured_feasible_lp(seed, m, n)
+    opt = lp_cases.highs_optimum(lp)
+    assert opt is not None
+    print(f"[case] structured seed={seed} {m}x{n} {prec} lanes={lanes}", flush=True)
+    r, x, rd, y = solve(engine, lp, precision=prec, lanes_A=lanes, lanes_At=lanes)
+    assert r.status == 1, r.as_dict()
+    check_point(lp, r, x, y, 1e-6)
+    got = float(np.dot(lp.c, x)) + lp.offset
+    assert abs(got - opt) <= 1e-5 * (1 + abs(opt)), (got, opt)
