@@ -223,11 +223,15 @@ typedef struct vj_batch {
     const int64_t* csc_col_ptr;    /* sum(n+1) */
     const int32_t* csc_row_idx;
     const double* csc_vals;
-    const double* c;               /* sum(n) */
+    const double* c;               /* sum(n), objective as given (see maximize) */
     const double* col_lower;
     const double* col_upper;
     const double* row_lower;       /* sum(m) */
     const double* row_upper;
+    const int32_t* maximize;       /* count, 1: instance maximises c^T x (negated internally, as in
+                                      vj_lp); NULL: every instance minimises */
+    const double* objective_offset;/* count, added to the instance's objective (it enters the
+                                      relative gap of the stopping test); NULL: all zero */
 } vj_batch;
 int vj_batch_upload(vj_ctx* ctx, const vj_batch* b);
 /* Solve every instance; per-instance results go to out[count]. */

@@ -255,3 +255,24 @@ def test_structured_lps_against_highs(engine, seed, m, n, prec, lanes):
     check_point(lp, r, x, y, 1e-6)
     got = float(np.dot(lp.c, x)) + lp.offset
     assert abs(got - opt) <= 1e-5 * (1 + abs(opt)), (got, opt)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_structured_batch_against_highs(engine, prec):
+    """The eight structured LPs (1x9 and 9x1 among them) solved as one block-diagonal batch,
+    twice over in mixed order, so that tiles span several instances of different shapes:
+    every instance reaches rel-KKT 1e-6 (oracle-checked) and its HiGHS objective to 1e-5."""
+    lps = [lp_cases.structured_feasible_lp(*s) for s in STRUCTURED]
+    lps = lps + lps[::-1]
+    opts = [lp_cases.highs_optimum(lp) for lp in lps]
+    print(f"[case] structured batch {prec} ({len(lps)} instances)", flush=True)
+    engine.upload_batch(lps)
+    res = engine.solve_batch(vp.default_params(vp.MODE_RESTARTED, precision=prec, tolerance=1e-6,
+                                               max_iterations=200000))
+    xs, ys = engine.download_batch()
+    for lp, opt, r, x, y in zip(lps, opts, res, xs, ys):
+        assert r.status == 1, r.as_dict()
+        k = O.rel_kkt(lp, x, y)
+        assert k[0] <= 1e-6 and abs(k[0] - r.rel_kkt) <= 1e-9 + 1e-6 * k[0], (k, r.rel_kkt)
+        got = float(np.dot(lp.c, x)) + lp.offset
+        assert abs(got - opt) <= 1e-5 * (1 + abs(opt)), (got, opt)

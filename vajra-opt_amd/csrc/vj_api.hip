@@ -364,6 +364,16 @@ int vj_batch_upload(vj_ctx* ctx, const vj_batch* b) {
     P.nnz_i.assign(b->nnz, b->nnz + b->count);
     P.maximize_i.assign(b->count, 0);
     P.offset_i.assign(b->count, 0.0);
+    for (int q = 0; q < b->count; ++q) {
+        const bool mx = b->maximize && b->maximize[q] != 0;
+        const double off = b->objective_offset ? b->objective_offset[q] : 0.0;
+        if (!std::isfinite(off)) {
+            set_error("non-finite objective offset in batch instance " + std::to_string(q));
+            return VJ_ERR_INVALID_ARGUMENT;
+        }
+        P.maximize_i[q] = mx ? 1 : 0;
+        P.offset_i[q] = (mx ? -1.0 : 1.0) * off;   // canonical (minimisation) sign, as for one LP
+    }
     P.h_rp.resize(P.m + 1);
     P.h_tp.resize(P.n + 1);
     P.h_ci.resize(P.nnz);

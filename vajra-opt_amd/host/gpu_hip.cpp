@@ -246,7 +246,8 @@ std::vector<Solution> solve_pdhg_gpu_batch(const std::vector<Model>& models, con
     std::vector<int32_t> ms(cnt), ns(cnt);
     std::vector<int64_t> zs(cnt), rp, tp;
     std::vector<int32_t> ci, ti;
-    std::vector<double> av, tv, c, lc, uc, lr, ur;
+    std::vector<double> av, tv, c, lc, uc, lr, ur, off(cnt);
+    std::vector<int32_t> mx(cnt);
     for (int q = 0; q < cnt; ++q) {
         const Model& M = models[q];
         if (M.num_rows == 0 || M.num_cols == 0 || M.A.nnz() == 0)
@@ -261,14 +262,16 @@ std::vector<Solution> solve_pdhg_gpu_batch(const std::vector<Model>& models, con
         tp.insert(tp.end(), M.A.col_ptr.begin(), M.A.col_ptr.end());
         ti.insert(ti.end(), M.A.row_idx.begin(), M.A.row_idx.end());
         tv.insert(tv.end(), M.A.values.begin(), M.A.values.end());
-        for (double v : M.c) c.push_back(M.sense == ObjSense::kMaximize ? -v : v);  // batch takes canonical c
+        c.insert(c.end(), M.c.begin(), M.c.end());
+        mx[q] = M.sense == ObjSense::kMaximize ? 1 : 0;
+        off[q] = M.objective_offset;
         lc.insert(lc.end(), M.col_lower.begin(), M.col_lower.end());
         uc.insert(uc.end(), M.col_upper.begin(), M.col_upper.end());
         lr.insert(lr.end(), M.row_lower.begin(), M.row_lower.end());
         ur.insert(ur.end(), M.row_upper.begin(), M.row_upper.end());
     }
     vj_batch b{cnt, ms.data(), ns.data(), zs.data(), rp.data(), ci.data(), av.data(), tp.data(), ti.data(), tv.data(),
-               c.data(), lc.data(), uc.data(), lr.data(), ur.data()};
+               c.data(), lc.data(), uc.data(), lr.data(), ur.data(), mx.data(), off.data()};
     Ctx ctx(0);
     check(vj_batch_upload(ctx.c, &b), "vj_batch_upload");
     vj_params p = params_from(options);

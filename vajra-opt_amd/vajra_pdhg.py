@@ -84,7 +84,7 @@ class VjBatch(ctypes.Structure):
     _fields_ = [("count", _c_int)] + [
         (f, _vp) for f in ("m", "n", "nnz", "csr_row_ptr", "csr_col_idx", "csr_vals", "csc_col_ptr",
                            "csc_row_idx", "csc_vals", "c", "col_lower", "col_upper", "row_lower",
-                           "row_upper")]
+                           "row_upper", "maximize", "objective_offset")]
 
 
 class KernelStat(ctypes.Structure):
@@ -341,13 +341,15 @@ class Engine:
             rp=cat([c[0] for c in csr], np.int64), ci=cat([c[1] for c in csr], np.int32),
             av=cat([c[2] for c in csr], np.float64), tp=cat([lp.col_ptr for lp in lps], np.int64),
             ti=cat([lp.row_idx for lp in lps], np.int32), tv=cat([lp.vals for lp in lps], np.float64),
-            c=cat([lp.canonical_c() for lp in lps], np.float64),
+            c=cat([lp.c for lp in lps], np.float64),
+            mx=np.array([int(lp.maximize) for lp in lps], np.int32),
+            off=np.array([float(lp.offset) for lp in lps], np.float64),
             lc=cat([lp.col_lower for lp in lps], np.float64), uc=cat([lp.col_upper for lp in lps], np.float64),
             lr=cat([lp.row_lower for lp in lps], np.float64), ur=cat([lp.row_upper for lp in lps], np.float64))
         self._keep = (ms, ns, zs, parts)
         b = VjBatch(cnt, _p(ms), _p(ns), _p(zs), _p(parts["rp"]), _p(parts["ci"]), _p(parts["av"]),
                     _p(parts["tp"]), _p(parts["ti"]), _p(parts["tv"]), _p(parts["c"]), _p(parts["lc"]),
-                    _p(parts["uc"]), _p(parts["lr"]), _p(parts["ur"]))
+                    _p(parts["uc"]), _p(parts["lr"]), _p(parts["ur"]), _p(parts["mx"]), _p(parts["off"]))
         _check(_lib.vj_batch_upload(self._ctx, ctypes.byref(b)), "vj_batch_upload")
         self.batch = lps
         self.lp = None
