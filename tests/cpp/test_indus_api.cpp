@@ -480,10 +480,17 @@ void test_batch() {
     mx.sense = indus::ObjSense::kMaximize;
     ms.push_back(mx);
     ms.push_back(indus::io::read_mps(g_dir + "/afiro.mps"));
+    // objective offsets enter the relative gap of the stopping test: one per sense
+    indus::Model off_min = two_var();
+    off_min.objective_offset = 1000.0;
+    ms.push_back(off_min);
+    indus::Model off_max = mx;
+    off_max.objective_offset = -50.0;
+    ms.push_back(off_max);
     indus::Options o = pdhg_options(1e-8, 100000);
     const auto sols = indus::gpu::solve_pdhg_gpu_batch(ms, o);
-    const double opt[4] = {-7.0, -5.0, 15.0, -464.75314285714285};
-    CHECK(sols.size() == 4);
+    const double opt[6] = {-7.0, -5.0, 15.0, -464.75314285714285, 993.0, -35.0};
+    CHECK(sols.size() == 6);
     for (size_t q = 0; q < sols.size(); ++q) {
         CHECK(sols[q].status == indus::SolveStatus::kOptimal);
         CHECK_NEAR(sols[q].objective_value, opt[q], 1e-6 * (1 + std::fabs(opt[q])));
