@@ -13,9 +13,9 @@ import vajra_pdhg as vp
 pytestmark = pytest.mark.gpu
 
 
-def solve(engine, lp, tol=1e-6, **kw):
+def solve(engine, lp, tol=1e-6, max_iterations=200000, **kw):
     engine.upload(lp)
-    p = vp.default_params(vp.MODE_RESTARTED, tolerance=tol, max_iterations=200000, **kw)
+    p = vp.default_params(vp.MODE_RESTARTED, tolerance=tol, max_iterations=max_iterations, **kw)
     r = engine.solve(p)
     x, rd, y = engine.download()
     return r, x, rd, y
@@ -276,3 +276,46 @@ def test_structured_batch_against_highs(engine, prec):
         assert k[0] <= 1e-6 and abs(k[0] - r.rel_kkt) <= 1e-9 + 1e-6 * k[0], (k, r.rel_kkt)
         got = float(np.dot(lp.c, x)) + lp.offset
         assert abs(got - opt) <= 1e-5 * (1 + abs(opt)), (got, opt)
+
+
+def test_infeasible_lps_never_reported_optimal(engine):
+    """LPs that SciPy-HiGHS finds infeasible: the restarted engine (which, like the
+    reference's PDHG, has no infeasibility certificate) stops at its iteration limit and
+    never reports them optimal, singly or inside a batch, where the feasible instances
+    still reach rel-KKT 1e-6 and their HiGHS objectives."""
+    bad = [lp_cases.structured_lp(s, m, n) for s, m, n in [(1, 30, 12), (4, 40, 40)]]
+    for lp in bad:
+        assert lp_cases.highs_optimum(lp) is None
+        print(f"[case] infeasible {lp.m}x{lp.n}", flush=True)
+        r, x, rd, y = solve(engine, lp, max_iterations=20000)
+        assert r.status != 1, r.as_dict()
+    good = [lp_cases.structured_feasible_lp(*s) for s in STRUCTURED[:4]]
+    lps = [good[0], bad[0], good[1], good[2], bad[1], good[3]]
+    print("[case] batch with infeasible instances", flush=True)
+    engine.upload_batch(lps)
+    res = engine.solve_batch(vp.default_params(vp.MODE_RESTARTED, tolerance=1e-6, max_iterations=20000))
+    xs, ys = engine.download_batch()
+    for lp, r, x, y in zip(lps, res, xs, ys):
+        opt = lp_cases.highs_optimum(lp)
+        if opt is None:
+            assert r.status != 1, r.as_dict()
+            continue
+        assert r.status == 1, r.as_dict()
+        assert O.rel_kkt(lp, x, y)[0] <= 1e-6
+        got = float(np.dot(lp.c, x)) + lp.offset
+        assert abs(got - opt) <= 1e-5 * (1 + abs(opt)), (got, opt)
+
+
+@pytest.mark.parametrize("where", ["c", "vals"])
+def test_nan_input_is_a_numerical_error(engine, where):
+    """A NaN in the costs or the matrix ends the solve with NUMERICAL_ERROR (the reference's
+    kNumericalError for non-finite iterates, pdhg.cpp:215-219), not with a point."""
+    lp = lp_cases.structured_feasible_lp(0, 12, 30)
+    if where == "c":
+        lp.c[4] = np.nan
+    else:
+        lp.vals[3] = np.nan
+    print(f"[case] NaN in {where}", flush=True)
+    engine.upload(lp)
+    r = engine.solve(vp.default_params(vp.MODE_RESTARTED, tolerance=1e-6, max_iterations=5000))
+    assert r.status == 9, r.as_dict()
