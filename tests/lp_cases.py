@@ -258,3 +258,21 @@ def highs_optimum(lp):
     r = linprog(sgn * lp.c, A_ub=np.array(a_ub) if a_ub else None, b_ub=b_ub or None,
                 A_eq=np.array(a_eq) if a_eq else None, b_eq=b_eq or None, bounds=bounds, method="highs")
     return sgn * r.fun + lp.offset if r.status == 0 else None
+
+
+def rescaled_lp(lp, seed, decades=3.0):
+    """The same LP with row i scaled by r_i and column j by s_j, both 10^U[-decades, decades]:
+    A' = R A S, x = S x', c' = S c, column bounds / s_j, row bounds * r_i. The optimal
+    objective is unchanged; the coefficient range grows by up to 10^(4 decades)."""
+    rng = np.random.default_rng(3000 + seed)
+    r = 10.0 ** rng.uniform(-decades, decades, lp.m)
+    s = 10.0 ** rng.uniform(-decades, decades, lp.n)
+    A = (lp.dense() * r[:, None]) * s[None, :]
+    big = 1e20
+    div = lambda v, f: np.where(np.abs(v) >= big, v, v / f)
+    mul = lambda v, f: np.where(np.abs(v) >= big, v, v * f)
+    out = vp.LP.from_dense(A, lp.c * s, div(lp.col_lower, s), div(lp.col_upper, s), mul(lp.row_lower, r),
+                           mul(lp.row_upper, r))
+    out.maximize = lp.maximize
+    out.offset = lp.offset
+    return out

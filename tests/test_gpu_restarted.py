@@ -319,3 +319,25 @@ def test_nan_input_is_a_numerical_error(engine, where):
     engine.upload(lp)
     r = engine.solve(vp.default_params(vp.MODE_RESTARTED, tolerance=1e-6, max_iterations=5000))
     assert r.status == 9, r.as_dict()
+
+
+@pytest.mark.parametrize("decades", [1.0, 3.0])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("seed,m,n", [(0, 12, 30), (1, 30, 12), (4, 40, 40), (5, 60, 150), (6, 150, 60)])
+def test_badly_scaled_lps(engine, seed, m, n, prec, decades):
+    """The feasible structured LPs with rows and columns rescaled by 10^U[-d,d], d = 1 and 3 (an exact
+    equivalence, so the optimum is unchanged): Ruiz and Pock-Chambolle scaling on the device
+    absorb it, and the restarted engine still reaches rel-KKT 1e-6 (oracle-checked on the
+    rescaled LP) and the HiGHS objective of the original LP to 1e-5. These LPs hold a row and
+    a column of tiny coefficients only; without a bound on the equilibration factors their
+    bounds / cost were inflated by up to 1e10 and the bound/objective rescaling stalled the
+    iteration (rel-KKT frozen near 1)."""
+    lp0 = lp_cases.structured_feasible_lp(seed, m, n)
+    opt = lp_cases.highs_optimum(lp0)
+    lp = lp_cases.rescaled_lp(lp0, seed, decades)
+    print(f"[case] rescaled seed={seed} {m}x{n} {prec} 10^+-{decades}", flush=True)
+    r, x, rd, y = solve(engine, lp, precision=prec)
+    assert r.status == 1, r.as_dict()
+    check_point(lp, r, x, y, 1e-6)
+    got = float(np.dot(lp.c, x)) + lp.offset
+    assert abs(got - opt) <= 1e-5 * (1 + abs(opt)), (got, opt)

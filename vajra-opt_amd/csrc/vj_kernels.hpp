@@ -64,15 +64,26 @@ __global__ __launch_bounds__(kBlock) void scale_rows(int rows, const int* __rest
     for (int p = rp[r] + lane; p < rp[r + 1]; p += G) av[p] *= so * s_other[idx[p]];
 }
 
-// factor = norm > 1e-11 ? 1/sqrt(norm) : 1; scale *= factor
+// factor = norm > 1e-11 ? 1/sqrt(norm) : 1, bounded so that the cumulative scale stays in
+// [1/kMaxScale, kMaxScale]; scale *= factor.
+// The bound matters for a row (column) whose entries are all tiny next to its bounds (cost):
+// unbounded equilibration multiplies its bounds (cost) by up to 1/|a|, e.g. 1e10 for 1e-10
+// coefficients, and the bound/objective rescaling that follows then divides every other
+// row's bounds (every other cost) by that inflated norm, which stalled the iteration on such
+// LPs. Matrices whose rows and columns have norms within 1e+-6 are not affected (the 1M x 2M
+// LP's factors lie in [0.007, 5500]).
+constexpr double kMaxScale = 1e6;
 __global__ void make_scale_step(int len, const double* __restrict__ norm,
                                 double* __restrict__ step, double* __restrict__ cum) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= len) return;
     const double v = norm[i];
-    const double f = v > 1e-11 ? 1.0 / sqrt(v) : 1.0;
+    const double c0 = cum[i];
+    double f = v > 1e-11 ? 1.0 / sqrt(v) : 1.0;
+    if (c0 * f > kMaxScale) f = kMaxScale / c0;
+    else if (c0 * f < 1.0 / kMaxScale) f = (1.0 / kMaxScale) / c0;
     step[i] = f;
-    cum[i] *= f;
+    cum[i] = c0 * f;
 }
 
 // Elementwise helpers.
