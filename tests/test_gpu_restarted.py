@@ -341,3 +341,24 @@ def test_badly_scaled_lps(engine, seed, m, n, prec, decades):
     check_point(lp, r, x, y, 1e-6)
     got = float(np.dot(lp.c, x)) + lp.offset
     assert abs(got - opt) <= 1e-5 * (1 + abs(opt)), (got, opt)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("seed,m,n,decades", [(11, 2, 64, 0.0), (11, 2, 64, 2.0), (33, 15, 6, 0.0)])
+def test_near_empty_columns_with_costs(engine, seed, m, n, decades, prec):
+    """LPs with a column whose only coefficients are tiny (1e-11 to 1e-8) but whose cost is
+    not: equilibration scales such a column to its bound (1e6), and its inflated cost used to
+    dominate the objective norm (crushing every other cost in the rescaling) and the initial
+    primal weight, so the rows never became active (duals stuck at 0). The rescaling and the
+    primal weight now leave such rows and columns out; the engine reaches rel-KKT 1e-6 and
+    the HiGHS objective."""
+    lp0 = lp_cases.structured_feasible_lp(seed, m, n)
+    lp = lp_cases.rescaled_lp(lp0, seed, decades) if decades else lp0
+    opt = lp_cases.highs_optimum(lp)
+    assert opt is not None
+    print(f"[case] near-empty columns seed={seed} {m}x{n} 10^+-{decades} {prec}", flush=True)
+    r, x, rd, y = solve(engine, lp, precision=prec)
+    assert r.status == 1, r.as_dict()
+    check_point(lp, r, x, y, 1e-6)
+    got = float(np.dot(lp.c, x)) + lp.offset
+    assert abs(got - opt) <= 1e-5 * (1 + abs(opt)), (got, opt)

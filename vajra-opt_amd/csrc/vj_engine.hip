@@ -448,16 +448,16 @@ int launch_kkt(vj_ctx* ctx, FastEngine* E, int GA, int GAt, bool multi, const do
 
 // Per-instance sum of squares of a vector (mode 0) or of the finite-bound vector (mode 1).
 int seg_norm2(vj_ctx* ctx, FastEngine& E, int len, int mode, const double* v, const double* lo,
-              const double* hi, bool rows, double* out) {
+              const double* hi, bool rows, double* out, const double* skip = nullptr) {
     Problem& P = ctx->prob;
     const int tiles = grid_for(len, kBlock);
     const int* inst = rows ? P.row_inst.as<int>() : P.col_inst.as<int>();
     if (E.multi)
-        hipLaunchKernelGGL((seg_sumsq<true>), dim3(tiles), dim3(kBlock), 0, ctx->stream, len, mode, v, lo, hi, inst,
-                           E.part_s.as<double>());
+        hipLaunchKernelGGL((seg_sumsq<true>), dim3(tiles), dim3(kBlock), 0, ctx->stream, len, mode, v, lo, hi, skip,
+                           inst, E.part_s.as<double>());
     else
-        hipLaunchKernelGGL((seg_sumsq<false>), dim3(tiles), dim3(kBlock), 0, ctx->stream, len, mode, v, lo, hi, inst,
-                           E.part_s.as<double>());
+        hipLaunchKernelGGL((seg_sumsq<false>), dim3(tiles), dim3(kBlock), 0, ctx->stream, len, mode, v, lo, hi, skip,
+                           inst, E.part_s.as<double>());
     VJ_LAUNCHED();
     hipLaunchKernelGGL(seg_finish, dim3(P.count), dim3(kBlock), 0, ctx->stream, E.part_s.as<double>(), 1, 0,
                        rows ? P.d_row_off.as<int>() : P.d_col_off.as<int>(), kBlock, out);
@@ -708,9 +708,12 @@ int setup(vj_ctx* ctx, FastEngine& E) {
     hipLaunchKernelGGL(scale_rows_data, dim3(grid_for(m, kBlock)), dim3(kBlock), 0, s, m, E.Rsc.as<double>(),
                        P.lr.as<double>(), P.ur.as<double>(), E.lr_s.as<double>(), E.ur_s.as<double>());
     VJ_LAUNCHED();
-    // bound / objective rescaling: cs = 1/(1+|c_s|), bs = 1/(1+|b_s|) per instance
-    VJ_TRY(seg_norm2(ctx, E, n, 0, E.c_s.as<double>(), nullptr, nullptr, false, E.inst_tmp_a.as<double>()));
-    VJ_TRY(seg_norm2(ctx, E, m, 1, nullptr, E.lr_s.as<double>(), E.ur_s.as<double>(), true, E.inst_tmp_b.as<double>()));
+    // bound / objective rescaling: cs = 1/(1+|c_s|), bs = 1/(1+|b_s|) per instance, the norms
+    // taken without the columns / rows whose equilibration factor reached its bound
+    VJ_TRY(seg_norm2(ctx, E, n, 0, E.c_s.as<double>(), nullptr, nullptr, false, E.inst_tmp_a.as<double>(),
+                     E.Csc.as<double>()));
+    VJ_TRY(seg_norm2(ctx, E, m, 1, nullptr, E.lr_s.as<double>(), E.ur_s.as<double>(), true, E.inst_tmp_b.as<double>(),
+                     E.Rsc.as<double>()));
     std::vector<double> cn2(cnt), bn2(cnt), cs(cnt, 1.0), bs(cnt, 1.0), objfac(cnt), omega0(cnt, 1.0);
     VJ_HIP(hipMemcpyAsync(cn2.data(), E.inst_tmp_a.p, cnt * 8, hipMemcpyDeviceToHost, s));
     VJ_HIP(hipMemcpyAsync(bn2.data(), E.inst_tmp_b.p, cnt * 8, hipMemcpyDeviceToHost, s));

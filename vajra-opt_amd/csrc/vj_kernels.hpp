@@ -92,18 +92,27 @@ __global__ void fill_f64(int64_t len, double* __restrict__ p, double v) {
     if (i < len) p[i] = v;
 }
 
+// Whether a cumulative equilibration factor sits at the bound of make_scale_step.
+__device__ __forceinline__ bool scale_at_bound(double s) {
+    return s >= kMaxScale * (1.0 - 1e-9) || s <= (1.0 / kMaxScale) * (1.0 + 1e-9);
+}
+
 // Per-element contribution for per-instance sums of squares:
 // mode 0: v[i]^2; mode 1: (max finite |lo|,|hi|)^2 (norm of the bound vector).
+// skip (may be null): elements whose equilibration factor skip[i] sits at its bound add
+// nothing (rows / columns of tiny coefficients only, whose bounds / costs equilibration
+// inflated; the bound/objective rescaling and the initial primal weight follow the rest).
 template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void seg_sumsq(int len, int mode, const double* __restrict__ v,
                                                     const double* __restrict__ lo,
                                                     const double* __restrict__ hi,
+                                                    const double* __restrict__ skip,
                                                     const int* __restrict__ inst_of,
                                                     double* __restrict__ part) {
     const int tile = blockIdx.x;
     const int i = tile * kBlock + threadIdx.x;
     double c[1] = {0.0};
-    if (i < len) {
+    if (i < len && !(skip && scale_at_bound(skip[i]))) {
         if (mode == 0) {
             c[0] = v[i] * v[i];
         } else {
