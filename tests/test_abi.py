@@ -114,3 +114,35 @@ def test_lp_csr_matches_dense():
         cols = ci[rp[i]:rp[i + 1]]
         assert np.all(np.diff(cols) > 0)
         np.testing.assert_array_equal(v[rp[i]:rp[i + 1]], A[i, cols])
+
+
+def test_ctypes_mirrors_match_the_c_header(tmp_path):
+    """The ctypes structures of vajra_pdhg.py have the size and field offsets that a C
+    compiler gives the structures of include/vajra_pdhg.h (a field added on one side only
+    would shift every later field)."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    pairs = [("vj_device_info", vp.DeviceInfo), ("vj_params", vp.Params), ("vj_result", vp.Result),
+             ("vj_lp", vp.VjLp), ("vj_kernel_stat", vp.KernelStat), ("vj_batch", vp.VjBatch)]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "vajra_pdhg.h"', 'int main(void) {']
+    for cname, py in pairs:
+        lines.append(f'    printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'    printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ['    return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    subprocess.run([cc, "-std=c11", "-I", os.path.join(root, "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        a, b, v = line.split()
+        got[(a, b)] = int(v)
+    for cname, py in pairs:
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
