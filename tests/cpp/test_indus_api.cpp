@@ -735,7 +735,31 @@ int api_solve(int argc, char** argv) {
     return 0;
 }
 
+// solve_mps <file.mps> <fp64|fp32> <tol> <presolve 0|1>: indus::solve on the device (the
+// reader, presolve, the PDHG engine, postsolve and the quality audit of the C++ API) on one
+// model file; prints the status, the objective, the reductions and the relative KKT error.
+int solve_mps(int argc, char** argv) {
+    if (argc < 6) return 2;
+    const indus::Model M = indus::io::read_mps(argv[2]);
+    indus::Options o;
+    o.set("algorithm", "pdhg_cuda");
+    o.use_gpu = true;
+    o.enable_presolve = std::atoi(argv[5]) != 0;
+    o.set("tolerance", std::atof(argv[4]));
+    o.set("pdhg_precision", argv[3]);
+    o.iteration_limit = 1000000;
+    o.time_limit = 100.0;
+    const indus::Solution s = indus::solve(M, o);
+    const auto k = indus::verifier::relative_kkt(M, s);
+    std::printf("solve_mps status=%d objective=%.17g reductions=%d iterations=%lld rel_kkt=%.6e\n",
+                static_cast<int>(s.status), s.objective_value, s.presolve_total_reductions,
+                static_cast<long long>(s.iterations), k.rel_kkt);
+    std::printf("status_message %s\n", s.status_message.c_str());
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc >= 2 && std::string(argv[1]) == "solve_mps") return solve_mps(argc, argv);
     if (argc >= 2 && std::string(argv[1]) == "dump") return dump(argc, argv);
     if (argc >= 2 && std::string(argv[1]) == "api_solve") return api_solve(argc, argv);
     if (argc >= 2 && std::string(argv[1]) == "presolve_dump") return presolve_dump_mode(argc, argv);

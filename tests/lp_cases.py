@@ -276,3 +276,57 @@ def rescaled_lp(lp, seed, decades=3.0):
     out.maximize = lp.maximize
     out.offset = lp.offset
     return out
+
+
+def write_mps(lp, path, name="LPCASE"):
+    """Free-format MPS of lp (no objective offset: MPS carries none). Rows: equality -> E,
+    both sides finite -> G with a RANGE, upper only -> L, lower only -> G, free -> L with a
+    1e20 right-hand side. Columns: FR / MI+UP / LO / FX / LO+UP; every column lists its
+    objective entry so that empty columns stay in the file."""
+    fin = lambda v: abs(v) < 1e20
+    A = lp.dense()
+    with open(path, "w") as f:
+        f.write(f"NAME {name}\n")
+        if lp.maximize:
+            f.write("OBJSENSE\n    MAX\n")
+        f.write("ROWS\n N OBJ\n")
+        kinds, rhs, rng = [], [], {}
+        for i in range(lp.m):
+            lo, hi = lp.row_lower[i], lp.row_upper[i]
+            if fin(lo) and fin(hi) and lo == hi:
+                kinds.append("E"); rhs.append(lo)
+            elif fin(lo) and fin(hi):
+                kinds.append("G"); rhs.append(lo); rng[i] = hi - lo
+            elif fin(hi):
+                kinds.append("L"); rhs.append(hi)
+            elif fin(lo):
+                kinds.append("G"); rhs.append(lo)
+            else:
+                kinds.append("L"); rhs.append(1e20)
+            f.write(f" {kinds[-1]} R{i}\n")
+        f.write("COLUMNS\n")
+        for j in range(lp.n):
+            f.write(f"    C{j} OBJ {float(lp.c[j])!r}\n")
+            for i in np.nonzero(A[:, j])[0]:
+                f.write(f"    C{j} R{i} {float(A[i, j])!r}\n")
+        f.write("RHS\n")
+        for i in range(lp.m):
+            f.write(f"    RHS R{i} {float(rhs[i])!r}\n")
+        if rng:
+            f.write("RANGES\n")
+            for i, r in rng.items():
+                f.write(f"    RNG R{i} {float(r)!r}\n")
+        f.write("BOUNDS\n")
+        for j in range(lp.n):
+            lo, hi = lp.col_lower[j], lp.col_upper[j]
+            if not fin(lo) and not fin(hi):
+                f.write(f" FR BND C{j}\n")
+            elif not fin(lo):
+                f.write(f" MI BND C{j}\n UP BND C{j} {float(hi)!r}\n")
+            elif not fin(hi):
+                f.write(f" LO BND C{j} {float(lo)!r}\n")
+            elif lo == hi:
+                f.write(f" FX BND C{j} {float(lo)!r}\n")
+            else:
+                f.write(f" LO BND C{j} {float(lo)!r}\n UP BND C{j} {float(hi)!r}\n")
+        f.write("ENDATA\n")
