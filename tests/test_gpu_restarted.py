@@ -362,3 +362,29 @@ def test_near_empty_columns_with_costs(engine, seed, m, n, decades, prec):
     check_point(lp, r, x, y, 1e-6)
     got = float(np.dot(lp.c, x)) + lp.offset
     assert abs(got - opt) <= 1e-5 * (1 + abs(opt)), (got, opt)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_badly_scaled_batch(engine, prec):
+    """The rescaled LPs (10^+-1 and 10^+-3, with their near-empty rows and columns) and the
+    near-empty-column LPs as one batch: per-instance equilibration bounds and per-instance
+    rescaling norms; every instance reaches rel-KKT 1e-6 and its HiGHS objective."""
+    lps = []
+    for dec in (1.0, 3.0):
+        for s, m, n in [(0, 12, 30), (1, 30, 12), (4, 40, 40), (5, 60, 150), (6, 150, 60)]:
+            lps.append(lp_cases.rescaled_lp(lp_cases.structured_feasible_lp(s, m, n), s, dec))
+    lps.append(lp_cases.structured_feasible_lp(11, 2, 64))
+    lps.append(lp_cases.rescaled_lp(lp_cases.structured_feasible_lp(11, 2, 64), 11, 2.0))
+    lps.append(lp_cases.structured_feasible_lp(33, 15, 6))
+    opts = [lp_cases.highs_optimum(lp) for lp in lps]
+    print(f"[case] badly scaled batch {prec} ({len(lps)} instances)", flush=True)
+    engine.upload_batch(lps)
+    res = engine.solve_batch(vp.default_params(vp.MODE_RESTARTED, precision=prec, tolerance=1e-6,
+                                               max_iterations=200000))
+    xs, ys = engine.download_batch()
+    for lp, opt, r, x, y in zip(lps, opts, res, xs, ys):
+        assert r.status == 1, r.as_dict()
+        k = O.rel_kkt(lp, x, y)
+        assert k[0] <= 1e-6 and abs(k[0] - r.rel_kkt) <= 1e-9 + 1e-6 * k[0], (k, r.rel_kkt)
+        got = float(np.dot(lp.c, x)) + lp.offset
+        assert abs(got - opt) <= 1e-5 * (1 + abs(opt)), (got, opt)
