@@ -1,0 +1,47 @@
+"""The numpy model of the restarted engine (tests/engine_model.py) on the LP families that
+found the equilibration stalls: without the bound on the cumulative equilibration factors,
+and without leaving the bounded rows and columns out of the rescaling norms, the iteration
+stalls (rel-KKT frozen far above the tolerance); with both, every LP of the families
+converges and meets its SciPy-HiGHS objective. The device tests
+(tests/test_gpu_restarted.py) check the same LPs on the engine itself."""
+import numpy as np
+import pytest
+
+import engine_model as M
+import lp_cases
+
+
+def _lp(seed, m, n, dec):
+    lp = lp_cases.structured_feasible_lp(seed, m, n)
+    return lp_cases.rescaled_lp(lp, seed, dec) if dec else lp
+
+
+def test_unbounded_equilibration_stalls():
+    """A row and a column of tiny coefficients only: equilibrated without a bound, their
+    bounds / costs grow by up to 4e10 and the rescaling crushes everything else."""
+    lp = _lp(0, 12, 30, 1.0)
+    r = M.solve(lp, max_iters=20000, max_scale=None)
+    assert r["status"] == 6 and r["kkt"][0] > 0.5, r["kkt"]
+    r = M.solve(lp, max_iters=20000)
+    assert r["status"] == 1, r["kkt"]
+
+
+def test_near_empty_column_needs_the_norm_exclusion():
+    """A column with one tiny coefficient and a real cost, scaled to the bound: its inflated
+    cost still dominates the objective norm unless bounded columns are left out of it."""
+    lp = _lp(11, 2, 64, 2.0)
+    r = M.solve(lp, max_iters=40000, exclude_bounded=False)
+    assert r["status"] == 6, r["kkt"]
+    r = M.solve(lp, max_iters=40000)
+    assert r["status"] == 1, r["kkt"]
+
+
+@pytest.mark.parametrize("dec", [0.0, 1.0, 3.0])
+def test_structured_and_rescaled_lps_converge(dec):
+    for s, m, n in [(0, 12, 30), (1, 30, 12), (2, 1, 9), (3, 9, 1), (4, 40, 40), (5, 60, 150), (6, 150, 60)]:
+        lp = _lp(s, m, n, dec)
+        opt = lp_cases.highs_optimum(lp)
+        r = M.solve(lp, max_iters=60000)
+        assert r["status"] == 1, (s, dec, r["kkt"][:4])
+        got = float(np.dot(lp.c, r["x"])) + lp.offset
+        assert abs(got - opt) <= 1e-5 * (1 + abs(opt)), (s, dec, got, opt)
