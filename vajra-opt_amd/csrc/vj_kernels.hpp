@@ -527,8 +527,8 @@ __global__ __launch_bounds__(kBlock) void split_partials(int rows, int S, const 
 // the row structure, the products go to LDS, and each thread adds up to split_rows<T>()/256
 // rows of the tile in index order (deterministic).
 // Rows per tile: 256 for 4-byte iterates, 512 for FP64. Shorter tiles balance the CUs better;
-// on the 1M x 2M LP (about 3 entries per row and block) the FP32 A x_bar step took 82.8 us at
-// 256 rows, 84.2 at 512, 87.0 at 1024 and 92 at 2048, the FP64 steps 119.1 + 112.8 us at 512
+// on the 1M x 2M LP (about 3 entries per row and block) the FP32 A x_bar step took 89.2 us at
+// 128 rows, 82.8 at 256, 84.2 at 512, 87.0 at 1024 and 92 at 2048, the FP64 steps 119.1 + 112.8 us at 512
 // rows, 120.4 + 114.3 at 256 and 122.1 + 115.6 at 1024.
 template <class T>
 constexpr int split_rows() { return sizeof(T) == 8 ? 512 : 256; }
