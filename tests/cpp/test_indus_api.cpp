@@ -498,6 +498,24 @@ void test_batch() {
     }
 }
 
+// Models without rows, or with rows but no nonzeros, go to the CPU PDHG, as the reference's
+// GPU entry delegates them (pdhg_cuda.cu:127-134): the same solution as solve_pdhg_cpu.
+void test_gpu_trivial_sizes() {
+    const std::vector<indus::Model> ms = {
+        make(0, 3, {1, -1, 0}, {0, 0, -1}, {2, 2, 1}, {}, {}, {}),
+        make(2, 3, {1, -1, 0}, {0, 0, -1}, {2, 2, 1}, {-1, -1e20}, {1, 4}, {})};
+    const indus::Options o = pdhg_options(1e-6, 1000);
+    for (const auto& M : ms) {
+        const indus::Solution g = indus::gpu::solve_pdhg_gpu(M, o);
+        const indus::Solution c = indus::pdhg::solve_pdhg_cpu(M, o);
+        CHECK(g.status == c.status);
+        CHECK(g.objective_value == c.objective_value);
+        CHECK(g.col_value == c.col_value);
+        CHECK(g.row_dual == c.row_dual);
+        CHECK_NEAR(g.objective_value, -2.0, 1e-9);
+    }
+}
+
 void test_mixed_precision_option() {
     const indus::Model M = indus::io::read_mps(g_dir + "/crude_blend.mps");
     indus::Options o = pdhg_options(1e-8, 200000);
@@ -791,6 +809,7 @@ int main(int argc, char** argv) {
     if (g_gpu) {
         tests.push_back({"cpu_gpu_reference_map_identical", test_cpu_gpu_reference_map_identical});
         tests.push_back({"batch", test_batch});
+        tests.push_back({"gpu_trivial_sizes", test_gpu_trivial_sizes});
         tests.push_back({"mixed_precision_option", test_mixed_precision_option});
     }
     for (auto& [name, fn] : tests) {
