@@ -498,6 +498,20 @@ void test_batch() {
     }
 }
 
+// time_limit <= 0 answers TIME_LIMIT before any solve, with the reference's message
+// (model.cpp:403-409), on the CPU and the device path alike.
+void test_time_limit_zero() {
+    const indus::Model M = two_var();
+    for (double tl : {0.0, -1.0}) {
+        indus::Options o = pdhg_options(1e-6, 1000);
+        o.time_limit = tl;
+        const indus::Solution s = indus::solve(M, o);
+        CHECK(s.status == indus::SolveStatus::kTimeLimit);
+        CHECK(s.status_message == "Time limit exceeded before solve commenced");
+        CHECK(s.iterations == 0 && s.col_value.empty());
+    }
+}
+
 // Models without rows, or with rows but no nonzeros, go to the CPU PDHG, as the reference's
 // GPU entry delegates them (pdhg_cuda.cu:127-134): the same solution as solve_pdhg_cpu.
 void test_gpu_trivial_sizes() {
@@ -805,6 +819,7 @@ int main(int argc, char** argv) {
         {"lp_reader", test_lp_reader},
         {"writers", test_writers},
         {"verifier_rejects", test_verifier_rejects},
+        {"time_limit_zero", test_time_limit_zero},
     };
     if (g_gpu) {
         tests.push_back({"cpu_gpu_reference_map_identical", test_cpu_gpu_reference_map_identical});
