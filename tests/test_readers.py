@@ -192,3 +192,22 @@ def test_python_test_reader_agrees_with_cpp_reader(path):
     for k, v in (("vals", lp.vals), ("c", lp.c), ("col_lower", lp.col_lower), ("col_upper", lp.col_upper),
                  ("row_lower", lp.row_lower), ("row_upper", lp.row_upper)):
         assert np.array_equal(v, d[k]), k
+
+
+@pytest.mark.skipif(not O.have_ref(), reason="oracle/_ref not built (no reference checkout)")
+@pytest.mark.parametrize("seed,m,n", [(0, 12, 30), (2, 1, 9), (3, 9, 1), (5, 60, 150)])
+def test_mps_writer_round_trip_through_the_reference_reader(tmp_path, seed, m, n):
+    """The test MPS writer (lp_cases.write_mps, used by the end-to-end solve tests) against the
+    reference's reader: the same matrix, costs, column bounds, sense and row bounds (a ranged
+    row's upper bound, lower + range, to one ulp)."""
+    import scipy.sparse as sp
+    lp = lp_cases.structured_feasible_lp(seed, m, n)
+    path = str(tmp_path / "lp.mps")
+    lp_cases.write_mps(lp, path)
+    r = O.ref_read_mps(path)
+    A = sp.csc_matrix((r["vals"], r["row_idx"], r["col_ptr"]), shape=(r["m"], r["n"])).toarray()
+    cap = lambda v: np.where(np.abs(v) >= 1e20, np.sign(v) * 1e20, v)
+    assert np.array_equal(A, lp.dense()) and np.array_equal(r["c"], lp.c) and r["maximize"] == lp.maximize
+    assert np.array_equal(cap(r["col_lower"]), cap(lp.col_lower)) and np.array_equal(cap(r["col_upper"]), cap(lp.col_upper))
+    assert np.array_equal(cap(r["row_lower"]), cap(lp.row_lower))
+    np.testing.assert_allclose(cap(r["row_upper"]), cap(lp.row_upper), rtol=1e-15, atol=1e-15)
