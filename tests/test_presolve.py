@@ -198,3 +198,18 @@ def test_random_models_reach_the_reductions(tmp_path):
     print("reduction totals over 60 random models:", totals.tolist(), "non-optimal exits:", exits)
     assert (totals[[0, 1, 2, 3, 5, 6, 7]] > 0).all(), totals
     assert exits <= 30, exits
+
+
+@needs_ref
+@pytest.mark.parametrize("kind", ["structured", "feasible"])
+@pytest.mark.parametrize("seed,m,n", [(0, 12, 30), (1, 30, 12), (2, 1, 9), (3, 9, 1), (4, 40, 40), (5, 60, 150),
+                                      (6, 150, 60), (7, 3, 3)])
+def test_presolve_structured_lps_match_reference(kind, seed, m, n, tmp_path):
+    """The structured LPs of the solver tests (empty rows and columns, fixed, free and
+    one-sided columns, equality, ranged and one-sided rows, tiny coefficients, both senses;
+    feasible or not) written as MPS: presolve and postsolve dumps equal the reference's."""
+    import lp_cases
+    lp = (lp_cases.structured_lp if kind == "structured" else lp_cases.structured_feasible_lp)(seed, m, n)
+    path = str(tmp_path / f"{kind}{seed}.mps")
+    lp_cases.write_mps(lp, path)
+    _compare(path, tmp_path)
