@@ -268,6 +268,23 @@ def test_oracle_equals_compiled_reference_structured(seed, m, n):
         assert np.array_equal(o["col_dual"], r["col_dual"]) and np.array_equal(o["row_value"], r["row_value"])
         assert o["tau"] == r["tau"] and o["sigma"] == r["sigma"] and o["restart_count"] == r["restart_count"]
         assert o["objective"] == r["objective"]
+    # quality audit and verifier at a random point (model.cpp:203-374, verifier.cpp:25-330)
+    rng = np.random.default_rng(50 + seed)
+    x, yd = rng.standard_normal(lp.n), rng.standard_normal(lp.m)
+    q = O.recompute_quality(lp, x, yd)
+    qr = np.zeros(5)
+    O.ref_lib().ref_quality(lp.m, lp.n, O._p(lp.col_ptr), O._p(lp.row_idx), O._p(lp.vals), O._p(lp.c),
+                            O._p(lp.col_lower), O._p(lp.col_upper), O._p(lp.row_lower), O._p(lp.row_upper),
+                            int(lp.maximize), O._c_dbl(lp.offset), O._p(x), O._p(yd), None, O._p(qr))
+    assert [q["max_primal_violation"], q["max_dual_violation"], q["max_complementarity_violation"]] == list(qr[:3])
+    vr = np.zeros(10)
+    O.ref_lib().ref_verify(lp.m, lp.n, O._p(lp.col_ptr), O._p(lp.row_idx), O._p(lp.vals), O._p(lp.c),
+                           O._p(lp.col_lower), O._p(lp.col_upper), O._p(lp.row_lower), O._p(lp.row_upper),
+                           int(lp.maximize), O._c_dbl(lp.offset), O._p(x), O._p(yd), O._c_dbl(0.75), 1, O._c_dbl(1e-6),
+                           O._p(vr))
+    v = O.verify_solution(lp, x, yd, 0.75, tol=1e-6)
+    assert [v["passed"], v["max_primal_violation"], v["max_dual_violation"], v["objective_discrepancy"]] == [
+        bool(vr[0]), vr[1], vr[2], vr[4]]
 
 
 @pytest.mark.skipif(not O.have_ref(), reason="oracle/_ref not built (no reference checkout)")
