@@ -205,6 +205,15 @@ int vj_kernel_stats(vj_ctx* ctx, vj_kernel_stat* out, int cap);
 /* Time `reps` back-to-back launches of each fused iteration kernel on the current
  * iterates (after a solve; iterates are restored). Fills stats as vj_kernel_stats. */
 int vj_benchmark_kernels(vj_ctx* ctx, int reps, vj_kernel_stat* out, int cap);
+/* Time check blocks of the last VJ_MODE_RESTARTED solve on its device-resident problem: one
+ * block is check_interval iterations, the FP64 check, the control kernel and the conditional
+ * restart, replayed from the solve's hipGraph. Every instance iterates through all blocks
+ * (none may stop in between); the final state of the solve is restored afterwards and the
+ * downloaded point is that of the solve. `warmup` untimed blocks, then `steps` blocks timed
+ * with HIP events on the context stream: ms_per_block = their average, iterations_per_block
+ * = the check interval. (Benchmark entry; the reference times whole solves only,
+ * GpuTimingDiagnostics, include/indus/gpu.hpp:37-46.) */
+int vj_benchmark_blocks(vj_ctx* ctx, int warmup, int steps, double* ms_per_block, int* iterations_per_block);
 
 /* ---- batched LPs (new functionality; no reference counterpart) --------------- */
 /* Instances are packed block-diagonally into one CSR and one CSR(A^T); each instance

@@ -1,0 +1,116 @@
+"""The benchmark workloads at their full sizes, end to end on the device, each returned point
+checked by the oracle: the rel-KKT recomputed on the host from the downloaded (x, y) must be
+<= 1e-6 and agree with the device's own FP64 audit.
+
+* 1,000,000 x 2,000,000 random LP (12M nonzeros, 10% of the columns scaled by 10^U[-3,3]),
+  FP64 and FP32 iterates with FP64 refinement, autotuned kernels (column split, sliced ELL,
+  tuned check interval, drift restarts); the two objectives agree to 1e-6 relative.
+* 512 uneven random LPs (m ~ U{500..20000}, n = 2.5 m) as one batch, FP32 and FP64.
+* 128 block-staircase LPs (T ~ U{12..96} periods of 300 x 800) as one batch, FP32 and FP64.
+
+"Converged" is the build-defined rel-KKT of the restarted engine; the reference's own
+stopping rule (src/solvers/pdhg/pdhg.cpp:249-365) has no relative KKT error."""
+import numpy as np
+import pytest
+
+import oracle as O
+import vajra_pdhg as vp
+
+pytestmark = pytest.mark.gpu
+
+SEED = 0x5EED2026
+LARGE = dict(m=1_000_000, n=2_000_000)
+
+
+def params(prec, **kw):
+    return vp.default_params(vp.MODE_RESTARTED, precision=prec, tolerance=1e-6, max_iterations=400000,
+                             time_limit=100.0, **kw)
+
+
+@pytest.fixture(scope="module")
+def large():
+    lp = vp.gen_random_lp(SEED + 1, LARGE["m"], LARGE["n"], 6, 0.1, 3.0)
+    eng = vp.Engine(0)
+    eng.upload(lp)
+    yield lp, eng, {}
+    eng.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_large_random_lp(large, prec):
+    lp, eng, seen = large
+    print(f"[case] 1M x 2M random LP {prec}", flush=True)
+    r = eng.solve(params(prec))
+    x, _, y = eng.download()
+    print(f"[case] {prec}: {vp.STATUS[r.status]} iters={r.iterations} restarts={r.restart_count} "
+          f"rel_kkt={r.rel_kkt:.3e} splits=({r.col_splits_A},{r.col_splits_At}) lanes=({r.lanes_A},{r.lanes_At}) "
+          f"K={r.check_interval} iter_s={r.iteration_time:.2f}", flush=True)
+    assert r.status == 1, r.as_dict()
+    k = O.rel_kkt(lp, x, y)
+    assert k[0] <= 1e-6, k
+    assert abs(k[0] - r.rel_kkt) <= 1e-9 + 1e-6 * k[0], (k, r.rel_kkt)
+    seen[prec] = float(np.dot(lp.c, x)) + lp.offset
+    if len(seen) == 2:
+        o32, o64 = seen["fp32"], seen["fp64"]
+        print(f"[case] objectives fp64 {o64:.12e} fp32 {o32:.12e} rel diff {abs(o32 - o64) / (1 + abs(o64)):.2e}")
+        assert abs(o32 - o64) <= 1e-6 * (1 + abs(o64)), (o32, o64)
+
+
+@pytest.mark.skip(reason="BF16 matrix values stall near rel-KKT 6e-5 on the 1M x 2M LP: 8-bit mantissas "
+                         "against column scales of 10^+-3 leave an error the FP64 refinement does not remove "
+                         "(BF16 reaches 1e-6 on the 2,000 x 5,000 LP, tests/test_gpu_restarted.py)")
+def test_large_random_lp_bf16(large):
+    lp, eng, _ = large
+    r = eng.solve(params("bf16"))
+    x, _, y = eng.download()
+    assert r.status == 1 and O.rel_kkt(lp, x, y)[0] <= 1e-6
+
+
+def check_batch(lps, eng, prec):
+    r = eng.solve_batch(params(prec))
+    xs, ys = eng.download_batch()
+    opt = sum(1 for q in r if q.status == 1)
+    print(f"[case] {prec}: {opt}/{len(lps)} optimal, max iters {max(q.iterations for q in r)}, "
+          f"worst device rel_kkt {max(q.rel_kkt for q in r):.3e}", flush=True)
+    worst = 0.0
+    for i, (lp, q, x, y) in enumerate(zip(lps, r, xs, ys)):
+        assert q.status == 1, (i, q.as_dict())
+        k = O.rel_kkt(lp, x, y)
+        assert k[0] <= 1e-6, (i, k)
+        assert abs(k[0] - q.rel_kkt) <= 1e-9 + 1e-6 * k[0], (i, k, q.rel_kkt)
+        worst = max(worst, k[0])
+    print(f"[case] {prec}: worst oracle rel_kkt {worst:.3e}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def random_batch():
+    ms = vp.gen_uniform_ints(SEED + 3, 512, 500, 20000)
+    lps = [vp.gen_random_lp((SEED + 3) * 1000 + i, int(m), int(2.5 * m), 6) for i, m in enumerate(ms)]
+    eng = vp.Engine(0)
+    eng.upload_batch(lps)
+    yield lps, eng
+    eng.close()
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_random_batch_512(random_batch, prec):
+    lps, eng = random_batch
+    print(f"[case] 512 uneven random LPs {prec} (sum nnz {sum(lp.nnz for lp in lps)})", flush=True)
+    check_batch(lps, eng, prec)
+
+
+@pytest.fixture(scope="module")
+def staircase_batch():
+    ts = vp.gen_uniform_ints(SEED + 4, 128, 12, 96)
+    lps = [vp.gen_staircase_lp((SEED + 4) * 1000 + i, int(t)) for i, t in enumerate(ts)]
+    eng = vp.Engine(0)
+    eng.upload_batch(lps)
+    yield lps, eng
+    eng.close()
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_staircase_batch_128(staircase_batch, prec):
+    lps, eng = staircase_batch
+    print(f"[case] 128 staircase LPs {prec} (sum nnz {sum(lp.nnz for lp in lps)})", flush=True)
+    check_batch(lps, eng, prec)

@@ -548,4 +548,15 @@ int vj_benchmark_kernels(vj_ctx* ctx, int reps, vj_kernel_stat* out, int cap) {
     return written;
 }
 
+int vj_benchmark_blocks(vj_ctx* ctx, int warmup, int steps, double* ms_per_block, int* iterations_per_block) {
+    clear_error();
+    if (!ctx || !ms_per_block || !iterations_per_block || warmup < 0 || steps <= 0) return VJ_ERR_INVALID_ARGUMENT;
+    if (ctx->params.mode != VJ_MODE_RESTARTED) {
+        set_error("check blocks exist in VJ_MODE_RESTARTED");
+        return VJ_ERR_UNSUPPORTED;
+    }
+    VJ_HIP(hipSetDevice(ctx->device));
+    return fast_benchmark_blocks(ctx, warmup, steps, ms_per_block, iterations_per_block);
+}
+
 }  // extern "C"

@@ -202,7 +202,8 @@ struct FastEngine {
     DevBuf tileA, tileAt, tfA, tlA, tfAt, tlAt;
     int hist_cap = 0;
     hipGraphExec_t gexec = nullptr;
-    InstState* h_state[2] = {nullptr, nullptr};
+    InstState* h_state[2] = {nullptr, nullptr};   // pinned, h_state_cap entries each
+    int h_state_cap = 0;
     std::vector<double> spectral;
     std::vector<double> h_hist;
     ~FastEngine() {
@@ -1197,13 +1198,17 @@ int run_typed(vj_ctx* ctx, FastEngine& E, vj_result* out, clk::time_point t_star
     const auto t_iter0 = clk::now();
 
     const int cnt = E.count;
-    for (int b = 0; b < 2; ++b)
-        if (!E.h_state[b]) VJ_HIP(hipHostMalloc(reinterpret_cast<void**>(&E.h_state[b]), cnt * sizeof(InstState), 0));
-    bool use_graph = prm.use_graph != 0;
-    if (E.gexec) {
-        (void)hipGraphExecDestroy(E.gexec);
-        E.gexec = nullptr;
+    if (E.h_state_cap < cnt) {
+        for (auto*& p : E.h_state)
+            if (p) {
+                VJ_HIP(hipHostFree(p));
+                p = nullptr;
+            }
+        E.h_state_cap = 0;
+        for (auto*& p : E.h_state) VJ_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), cnt * sizeof(InstState), 0));
+        E.h_state_cap = cnt;
     }
+    bool use_graph = prm.use_graph != 0;
     if (use_graph) {
         hipGraph_t g = nullptr;
         VJ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -1365,6 +1370,14 @@ int fast_solve(vj_ctx* ctx, vj_result* out) {
     Problem& P = ctx->prob;
     if (!ctx->fast) ctx->fast.reset(new FastEngine());
     FastEngine& E = *ctx->fast;
+    // The previous solve's executable graph holds the device addresses of this engine's
+    // buffers, which the setup below frees and re-allocates (DevBuf::alloc releases first):
+    // drain the stream and destroy that graph before any buffer changes hands.
+    VJ_HIP(hipStreamSynchronize(ctx->stream));
+    if (E.gexec) {
+        VJ_HIP(hipGraphExecDestroy(E.gexec));
+        E.gexec = nullptr;
+    }
     E.prec = ctx->params.precision;
     E.count = P.count;
     E.m = P.m;
@@ -1390,6 +1403,72 @@ int fast_solve(vj_ctx* ctx, vj_result* out) {
     }
     if (rc != VJ_OK) return rc;
     for (int q = 0; q < P.count; ++q) out[q].setup_time = setup_time;
+    return VJ_OK;
+}
+
+namespace {
+
+// Enqueue `count` check blocks of the last solve: its executable graph when one was
+// captured, else the same launches on the stream.
+template <class V, class T>
+int enqueue_blocks(vj_ctx* ctx, FastEngine& E, int count) {
+    const V* av = reinterpret_cast<const V*>(sizeof(V) == 8 ? E.av_s.p : E.av_lo.p);
+    const V* tv = reinterpret_cast<const V*>(sizeof(V) == 8 ? E.tv_s.p : E.tv_lo.p);
+    for (int b = 0; b < count; ++b) {
+        if (E.gexec) VJ_HIP(hipGraphLaunch(E.gexec, ctx->stream));
+        else VJ_TRY((enqueue_block<V, T>(ctx, E, av, tv)));
+    }
+    return VJ_OK;
+}
+
+int enqueue_blocks_any(vj_ctx* ctx, FastEngine& E, int count) {
+    switch (E.prec) {
+        case VJ_PREC_FP64: return enqueue_blocks<double, double>(ctx, E, count);
+        case VJ_PREC_FP32: return enqueue_blocks<float, float>(ctx, E, count);
+        default: return enqueue_blocks<bf16_t, float>(ctx, E, count);
+    }
+}
+
+}  // namespace
+
+int fast_benchmark_blocks(vj_ctx* ctx, int warmup, int steps, double* ms_per_block, int* iters_per_block) {
+    if (!ctx->fast || !ctx->have_solution) {
+        set_error("no restarted-mode solve has run on this context");
+        return VJ_ERR_NOT_READY;
+    }
+    FastEngine& E = *ctx->fast;
+    hipStream_t s = ctx->stream;
+    const size_t sb = static_cast<size_t>(E.count) * sizeof(InstState);
+    std::vector<InstState> saved(E.count), live;
+    VJ_HIP(hipMemcpyAsync(saved.data(), E.st.p, sb, hipMemcpyDeviceToHost, s));
+    VJ_HIP(hipStreamSynchronize(s));
+    // every instance iterates through all blocks: none may converge or reach its limit
+    live = saved;
+    for (auto& z : live) {
+        z.active = 1;
+        z.converged = 0;
+        z.tol = 0.0;
+        z.max_iters = INT64_MAX / 4;
+    }
+    VJ_HIP(hipMemcpyAsync(E.st.p, live.data(), sb, hipMemcpyHostToDevice, s));
+    ScopedEvent se0, se1;
+    VJ_HIP(hipEventCreate(&se0.e));
+    VJ_HIP(hipEventCreate(&se1.e));
+    int rc = enqueue_blocks_any(ctx, E, warmup);
+    if (rc == VJ_OK) {
+        VJ_HIP(hipEventRecord(se0.e, s));
+        rc = enqueue_blocks_any(ctx, E, steps);
+    }
+    VJ_HIP(hipEventRecord(se1.e, s));
+    VJ_HIP(hipEventSynchronize(se1.e));
+    // the returned point was saved before; the final state of the solve goes back
+    VJ_HIP(hipMemcpyAsync(E.st.p, saved.data(), sb, hipMemcpyHostToDevice, s));
+    VJ_HIP(hipStreamSynchronize(s));
+    if (rc != VJ_OK) return rc;
+    float ms = 0.0f;
+    VJ_HIP(hipEventElapsedTime(&ms, se0.e, se1.e));
+    *ms_per_block = static_cast<double>(ms) / steps;
+    *iters_per_block = E.K;
     return VJ_OK;
 }
 

@@ -96,7 +96,7 @@ class KernelStat(ctypes.Structure):
 EXPORTED = [
     "vj_last_error", "vj_device_probe", "vj_params_default", "vj_ctx_create", "vj_ctx_destroy",
     "vj_problem_upload", "vj_params_set", "vj_solve", "vj_download", "vj_reference_iterates",
-    "vj_audit_point", "vj_kernel_stats", "vj_benchmark_kernels", "vj_batch_upload",
+    "vj_audit_point", "vj_kernel_stats", "vj_benchmark_kernels", "vj_benchmark_blocks", "vj_batch_upload",
     "vj_batch_solve", "vj_batch_download", "vj_op_spmv", "vj_op_dual_update",
     "vj_op_primal_update", "vj_op_diagnostics", "vj_gen_random_lp", "vj_gen_staircase_lp",
     "vj_gen_uniform_ints",
@@ -118,6 +118,7 @@ _lib.vj_download.argtypes = [_vp, _vp, _vp, _vp]
 _lib.vj_batch_download.argtypes = [_vp, _vp, _vp]
 _lib.vj_benchmark_kernels.argtypes = [_vp, _c_int, _vp, _c_int]
 _lib.vj_kernel_stats.argtypes = [_vp, _vp, _c_int]
+_lib.vj_benchmark_blocks.argtypes = [_vp, _c_int, _c_int, _vp, _vp]
 
 
 def lib():
@@ -328,6 +329,15 @@ class Engine:
             raise VjError(-k, "vj_benchmark_kernels")
         return [{"name": arr[i].name.decode(), "avg_us": arr[i].avg_us, "launches": arr[i].launches,
                  "bytes_per_launch": arr[i].bytes_per_launch} for i in range(k)]
+
+    def benchmark_blocks(self, warmup, steps):
+        """(ms per check block, iterations per block) of the last restarted solve, replayed on
+        its resident problem (vj_benchmark_blocks)."""
+        ms = ctypes.c_double()
+        k = ctypes.c_int()
+        _check(_lib.vj_benchmark_blocks(self._ctx, warmup, steps, ctypes.byref(ms), ctypes.byref(k)),
+               "vj_benchmark_blocks")
+        return ms.value, k.value
 
     # -- batches --
     def upload_batch(self, lps):
