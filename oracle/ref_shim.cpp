@@ -12,7 +12,10 @@
 //   indus::io::read_mps                   src/io/mps_reader.cpp:48
 //   indus::presolve::PresolveEngine::apply / postsolve  src/presolve/presolve.cpp:10,649
 //   indus::verifier::verify_solution      src/engine/verifier.cpp:25
+//   indus::Options (set / get_*)          src/engine/options.cpp:6-89
+#include "../tests/cpp/options_probe.hpp"
 #include "../tests/cpp/presolve_dump.hpp"
+#include "indus/options.hpp"
 #include "indus/presolve.hpp"
 #include "indus/model.hpp"
 #include "indus/pdhg.hpp"
@@ -20,6 +23,7 @@
 #include "indus/verifier.hpp"
 #include "src/linalg/scaling.hpp"
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -256,6 +260,18 @@ int ref_presolve_dump(const char* path, int lp, const char* out_path) {
     }
     std::fclose(f);
     return 0;
+}
+
+// The options probe script of options_probe.hpp run on the reference's Options; returns the
+// dump's length (the text is cut at cap - 1 bytes).
+int ref_options_probe(char* out, int cap) {
+    const std::string s = options_probe::run<indus::Options>();
+    if (cap > 0) {
+        const size_t k = std::min(s.size(), static_cast<size_t>(cap - 1));
+        std::memcpy(out, s.data(), k);
+        out[k] = 0;
+    }
+    return static_cast<int>(s.size());
 }
 
 }  // extern "C"

@@ -28,6 +28,7 @@
 #include "indus/pdhg.hpp"
 #include "indus/presolve.hpp"
 #include "indus/verifier.hpp"
+#include "options_probe.hpp"
 #include "presolve_dump.hpp"
 #include "vajra_lpgen.h"
 
@@ -791,6 +792,10 @@ int solve_mps(int argc, char** argv) {
 }
 
 int main(int argc, char** argv) {
+    if (argc >= 2 && std::string(argv[1]) == "options_probe") {
+        std::cout << options_probe::run<indus::Options>();
+        return 0;
+    }
     if (argc >= 2 && std::string(argv[1]) == "solve_mps") return solve_mps(argc, argv);
     if (argc >= 2 && std::string(argv[1]) == "dump") return dump(argc, argv);
     if (argc >= 2 && std::string(argv[1]) == "api_solve") return api_solve(argc, argv);

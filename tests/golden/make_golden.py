@@ -93,6 +93,11 @@ def main():
     meta["cfg0_dual_simplex_status"] = s["status"]
     meta["cfg0_dual_simplex_max_primal_violation"] = s["max_primal_violation"]
     meta["cfg0_dual_simplex_max_dual_violation"] = s["max_dual_violation"]
+    # the reference's Options store driven by the script of tests/cpp/options_probe.hpp
+    buf = O.ctypes.create_string_buffer(1 << 16)
+    O.ref_lib().ref_options_probe(buf, len(buf))
+    with open(os.path.join(OUT, "options_probe.txt"), "w") as f:
+        f.write(buf.value.decode())
     with open(os.path.join(OUT, "golden_meta.json"), "w") as f:
         json.dump(meta, f, indent=1, sort_keys=True)
     print(json.dumps(meta, indent=1))

@@ -290,3 +290,21 @@ def test_solve_mps_structured_end_to_end(tmp_path, seed, m, n, prec, presolve):
         assert float(kv["rel_kkt"]) <= 1e-9 * 1.0001, r.stdout
     got = float(kv["objective"])
     assert abs(got - opt) <= 1e-6 * (1 + abs(opt)), (got, opt, r.stdout)
+
+
+def test_options_store_equals_reference():
+    """indus::Options (a key -> member binding table here) behaves as the reference's Options
+    (src/engine/options.cpp:6-89): the script of tests/cpp/options_probe.hpp (defaults, every
+    key and alias, values of another type than the bound member, re-typed keys) gives the same
+    members and get_string / get_double / get_int / get_bool answers, text for text, as the
+    reference compiled from its sources (tests/golden/options_probe.txt, written by that build
+    through tests/golden/make_golden.py; compared live as well where it is built)."""
+    mine = subprocess.run([_driver(), "options_probe"], capture_output=True, text=True, timeout=60, check=True).stdout
+    with open(os.path.join(GOLDEN, "options_probe.txt")) as f:
+        golden = f.read()
+    assert mine == golden
+    if oracle.have_ref():
+        import ctypes
+        buf = ctypes.create_string_buffer(1 << 16)
+        oracle.ref_lib().ref_options_probe(buf, len(buf))
+        assert buf.value.decode() == mine

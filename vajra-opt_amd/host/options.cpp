@@ -1,83 +1,123 @@
-// Options store (src/engine/options.cpp:6-89 of the reference).
+// Options store: the typed members and the string-keyed variant store of the reference's
+// Options (include/indus/options.hpp:10-41, behaviour of src/engine/options.cpp:6-89).
+//
+// Every typed member is reachable through a table of key -> member bindings. A value set
+// under a key always lands in the store; it also updates the member bound to that key when
+// the value's type is the member's type (a value of another type is stored only). The
+// constructor seeds the store with each member's default under its primary key; aliases
+// ("mip_gap", "integer_tol") update the member but are not seeded.
 #include "indus/options.hpp"
+
+#include <cstring>
 
 namespace indus {
 
+namespace {
+
+using MemberRef = std::variant<std::string Options::*, double Options::*, int64_t Options::*, bool Options::*>;
+
+struct Binding {
+    const char* key;
+    MemberRef member;
+    bool primary;   // seeded by the constructor
+};
+
+const Binding kBindings[] = {
+    {"time_limit", &Options::time_limit, true},
+    {"iteration_limit", &Options::iteration_limit, true},
+    {"node_limit", &Options::node_limit, true},
+    {"mip_relative_gap", &Options::mip_relative_gap, true},
+    {"mip_gap", &Options::mip_relative_gap, false},
+    {"mip_absolute_gap", &Options::mip_absolute_gap, true},
+    {"integer_tolerance", &Options::integer_tolerance, true},
+    {"integer_tol", &Options::integer_tolerance, false},
+    {"branching", &Options::branching_strategy, true},
+    {"node_selection", &Options::node_selection, true},
+    {"enable_presolve", &Options::enable_presolve, true},
+    {"enable_scaling", &Options::enable_scaling, true},
+    {"enable_root_cuts", &Options::enable_root_cuts, true},
+    {"use_gpu", &Options::use_gpu, true},
+    {"algorithm", &Options::algorithm, true},
+};
+
+// Write `value` into the member bound to `key`, if the binding's member has type T.
+template <class T>
+void update_bound_member(Options& o, const std::string& key, const T& value) {
+    for (const Binding& b : kBindings) {
+        if (key != b.key) continue;
+        if (const auto* m = std::get_if<T Options::*>(&b.member)) o.**m = value;
+        return;
+    }
+}
+
+// A stored value read as the requested arithmetic type (integers and doubles convert into
+// each other; anything else is absent).
+template <class T, class Store>
+bool numeric_value(const Store& values, const std::string& key, T& out) {
+    auto it = values.find(key);
+    if (it == values.end()) return false;
+    if (const auto* d = std::get_if<double>(&it->second)) {
+        out = static_cast<T>(*d);
+        return true;
+    }
+    if (const auto* i = std::get_if<int64_t>(&it->second)) {
+        out = static_cast<T>(*i);
+        return true;
+    }
+    return false;
+}
+
+}  // namespace
+
 Options::Options() {
-    set("time_limit", time_limit);
-    set("iteration_limit", iteration_limit);
-    set("node_limit", node_limit);
-    set("mip_relative_gap", mip_relative_gap);
-    set("mip_absolute_gap", mip_absolute_gap);
-    set("integer_tolerance", integer_tolerance);
-    set("branching", branching_strategy);
-    set("node_selection", node_selection);
-    set("enable_presolve", enable_presolve);
-    set("enable_scaling", enable_scaling);
-    set("enable_root_cuts", enable_root_cuts);
-    set("use_gpu", use_gpu);
-    set("algorithm", algorithm);
+    for (const Binding& b : kBindings)
+        if (b.primary) std::visit([&](auto member) { set(b.key, this->*member); }, b.member);
 }
 
 void Options::set(const std::string& key, const std::string& value) {
     values_[key] = value;
-    if (key == "algorithm") algorithm = value;
-    else if (key == "branching") branching_strategy = value;
-    else if (key == "node_selection") node_selection = value;
+    update_bound_member(*this, key, value);
 }
 
 void Options::set(const std::string& key, const char* value) { set(key, std::string(value ? value : "")); }
 
 void Options::set(const std::string& key, double value) {
     values_[key] = value;
-    if (key == "time_limit") time_limit = value;
-    else if (key == "mip_relative_gap" || key == "mip_gap") mip_relative_gap = value;
-    else if (key == "mip_absolute_gap") mip_absolute_gap = value;
-    else if (key == "integer_tolerance" || key == "integer_tol") integer_tolerance = value;
+    update_bound_member(*this, key, value);
 }
 
 void Options::set(const std::string& key, int64_t value) {
     values_[key] = value;
-    if (key == "iteration_limit") iteration_limit = value;
-    else if (key == "node_limit") node_limit = value;
+    update_bound_member(*this, key, value);
 }
 
 void Options::set(const std::string& key, bool value) {
     values_[key] = value;
-    if (key == "enable_presolve") enable_presolve = value;
-    else if (key == "enable_scaling") enable_scaling = value;
-    else if (key == "enable_root_cuts") enable_root_cuts = value;
-    else if (key == "use_gpu") use_gpu = value;
+    update_bound_member(*this, key, value);
 }
 
 std::string Options::get_string(const std::string& key, const std::string& default_val) const {
     auto it = values_.find(key);
-    if (it != values_.end())
-        if (const auto* s = std::get_if<std::string>(&it->second)) return *s;
-    return default_val;
+    if (it == values_.end()) return default_val;
+    const auto* s = std::get_if<std::string>(&it->second);
+    return s ? *s : default_val;
 }
 
 double Options::get_double(const std::string& key, double default_val) const {
-    auto it = values_.find(key);
-    if (it == values_.end()) return default_val;
-    if (const auto* d = std::get_if<double>(&it->second)) return *d;
-    if (const auto* i = std::get_if<int64_t>(&it->second)) return static_cast<double>(*i);
-    return default_val;
+    double v = default_val;
+    return numeric_value(values_, key, v) ? v : default_val;
 }
 
 int64_t Options::get_int(const std::string& key, int64_t default_val) const {
-    auto it = values_.find(key);
-    if (it == values_.end()) return default_val;
-    if (const auto* i = std::get_if<int64_t>(&it->second)) return *i;
-    if (const auto* d = std::get_if<double>(&it->second)) return static_cast<int64_t>(*d);
-    return default_val;
+    int64_t v = default_val;
+    return numeric_value(values_, key, v) ? v : default_val;
 }
 
 bool Options::get_bool(const std::string& key, bool default_val) const {
     auto it = values_.find(key);
-    if (it != values_.end())
-        if (const auto* b = std::get_if<bool>(&it->second)) return *b;
-    return default_val;
+    if (it == values_.end()) return default_val;
+    const auto* b = std::get_if<bool>(&it->second);
+    return b ? *b : default_val;
 }
 
 }  // namespace indus
