@@ -13,6 +13,8 @@
 //   indus::presolve::PresolveEngine::apply / postsolve  src/presolve/presolve.cpp:10,649
 //   indus::verifier::verify_solution      src/engine/verifier.cpp:25
 //   indus::Options (set / get_*)          src/engine/options.cpp:6-89
+//   indus::Solution::recompute_quality    src/engine/model.cpp:203-374
+#include "../tests/cpp/audit_dump.hpp"
 #include "../tests/cpp/options_probe.hpp"
 #include "../tests/cpp/presolve_dump.hpp"
 #include "indus/options.hpp"
@@ -258,6 +260,22 @@ int ref_presolve_dump(const char* path, int lp, const char* out_path) {
         auto red = presolve_dump::synthetic_reduced<indus::Solution>(res.presolved_model, indus::SolveStatus::kOptimal);
         presolve_dump::dump_solution(f, indus::presolve::PresolveEngine::postsolve(red, res.stack, model));
     }
+    std::fclose(f);
+    return 0;
+}
+
+// The audits of audit_dump.hpp (verify_solution, recompute_quality, verify_solution) on the
+// point in point_path for the model in the MPS file model_path, written to out_path.
+int ref_audit_dump(const char* model_path, const char* point_path, double tol, const char* out_path) {
+    const indus::Model model = indus::io::read_mps(model_path);
+    audit_dump::Point p;
+    if (!audit_dump::read_point(point_path, p)) return 1;
+    FILE* f = std::fopen(out_path, "w");
+    if (!f) return 1;
+    audit_dump::run<indus::Solution, indus::Model, indus::SolveStatus>(
+        f, model, p, tol, [](const indus::Model& m, const indus::Solution& s, double t) {
+            return indus::verifier::verify_solution(m, s, t);
+        });
     std::fclose(f);
     return 0;
 }

@@ -28,6 +28,7 @@
 #include "indus/pdhg.hpp"
 #include "indus/presolve.hpp"
 #include "indus/verifier.hpp"
+#include "audit_dump.hpp"
 #include "options_probe.hpp"
 #include "presolve_dump.hpp"
 #include "vajra_lpgen.h"
@@ -792,6 +793,20 @@ int solve_mps(int argc, char** argv) {
 }
 
 int main(int argc, char** argv) {
+    if (argc >= 6 && std::string(argv[1]) == "audit_dump") {
+        // audit_dump <mps> <point file> <tol> <out>: the host audits of a given point
+        const indus::Model model = indus::io::read_mps(argv[2]);
+        audit_dump::Point p;
+        if (!audit_dump::read_point(argv[3], p)) return 3;
+        FILE* f = std::fopen(argv[5], "w");
+        if (!f) return 3;
+        audit_dump::run<indus::Solution, indus::Model, indus::SolveStatus>(
+            f, model, p, std::stod(argv[4]), [](const indus::Model& m, const indus::Solution& s, double t) {
+                return indus::verifier::verify_solution(m, s, t);
+            });
+        std::fclose(f);
+        return 0;
+    }
     if (argc >= 2 && std::string(argv[1]) == "options_probe") {
         std::cout << options_probe::run<indus::Options>();
         return 0;

@@ -13,6 +13,9 @@ std::string name_of(const std::vector<std::string>& names, int k, char prefix) {
     if (static_cast<size_t>(k) < names.size() && !names[k].empty()) return names[k];
     return std::string(1, prefix) + std::to_string(k);
 }
+// Values in violation messages: fixed notation, six decimals (the report format the
+// reference's verifier writes).
+std::string num(double v) { return std::to_string(v); }
 }  // namespace
 
 VerificationResult verify_solution(const Model& M, const Solution& sol, double tol) {
@@ -25,7 +28,8 @@ VerificationResult verify_solution(const Model& M, const Solution& sol, double t
     res.reported_objective = sol.objective_value;
     const int m = M.num_rows, n = M.num_cols;
     if (static_cast<int>(sol.col_value.size()) != n) {
-        res.violations.push_back("Primal solution vector size does not match model num_cols");
+        res.violations.push_back("Primal solution vector size (" + std::to_string(sol.col_value.size()) +
+                                 ") does not match model num_cols (" + std::to_string(n) + ")");
         res.summary = "FAIL: Dimension mismatch";
         return res;
     }
@@ -38,11 +42,13 @@ VerificationResult verify_solution(const Model& M, const Solution& sol, double t
         const double xj = sol.col_value[j], lj = M.col_lower[j], uj = M.col_upper[j];
         if (xj < lj - tol) {
             res.max_bound_violation = std::max(res.max_bound_violation, lj - xj);
-            res.violations.push_back("Variable " + name_of(M.col_names, j, 'c') + " below lower bound");
+            res.violations.push_back("Variable " + name_of(M.col_names, j, 'c') + " below lower bound: " + num(xj) +
+                                     " < " + num(lj));
         }
         if (xj > uj + tol) {
             res.max_bound_violation = std::max(res.max_bound_violation, xj - uj);
-            res.violations.push_back("Variable " + name_of(M.col_names, j, 'c') + " above upper bound");
+            res.violations.push_back("Variable " + name_of(M.col_names, j, 'c') + " above upper bound: " + num(xj) +
+                                     " > " + num(uj));
         }
     }
     res.bounds_feasible = res.max_bound_violation <= tol;
@@ -51,11 +57,13 @@ VerificationResult verify_solution(const Model& M, const Solution& sol, double t
     for (int i = 0; i < m; ++i) {
         if (ax[i] < M.row_lower[i] - tol) {
             res.max_row_violation = std::max(res.max_row_violation, M.row_lower[i] - ax[i]);
-            res.violations.push_back("Row " + name_of(M.row_names, i, 'r') + " below lower bound");
+            res.violations.push_back("Row " + name_of(M.row_names, i, 'r') + " below lower bound: " + num(ax[i]) +
+                                     " < " + num(M.row_lower[i]));
         }
         if (ax[i] > M.row_upper[i] + tol) {
             res.max_row_violation = std::max(res.max_row_violation, ax[i] - M.row_upper[i]);
-            res.violations.push_back("Row " + name_of(M.row_names, i, 'r') + " above upper bound");
+            res.violations.push_back("Row " + name_of(M.row_names, i, 'r') + " above upper bound: " + num(ax[i]) +
+                                     " > " + num(M.row_upper[i]));
         }
     }
     res.max_primal_violation = std::max(res.max_bound_violation, res.max_row_violation);
@@ -65,7 +73,9 @@ VerificationResult verify_solution(const Model& M, const Solution& sol, double t
     res.recomputed_objective = obj;
     res.objective_discrepancy = std::fabs(obj - sol.objective_value);
     res.objective_matches = res.objective_discrepancy / std::max(1.0, std::fabs(sol.objective_value)) <= tol;
-    if (!res.objective_matches) res.violations.push_back("Objective mismatch");
+    if (!res.objective_matches)
+        res.violations.push_back("Objective mismatch: reported " + num(res.reported_objective) + " vs recomputed " +
+                                 num(res.recomputed_objective));
     if (!M.has_integers() && static_cast<int>(sol.row_dual.size()) == m) {
         std::vector<double> aty(n);
         M.A.multiply_transpose(sol.row_dual, aty);
@@ -75,11 +85,11 @@ VerificationResult verify_solution(const Model& M, const Solution& sol, double t
             const double xj = sol.col_value[j], lj = M.col_lower[j], uj = M.col_upper[j];
             if (xj < uj - tol && dj < -tol) {
                 res.max_dual_violation = std::max(res.max_dual_violation, -dj);
-                res.violations.push_back("Variable " + name_of(M.col_names, j, 'c') + " can increase but dj < 0");
+                res.violations.push_back("Variable " + name_of(M.col_names, j, 'c') + " can increase but dj < 0: " + num(dj));
             }
             if (xj > lj + tol && dj > tol) {
                 res.max_dual_violation = std::max(res.max_dual_violation, dj);
-                res.violations.push_back("Variable " + name_of(M.col_names, j, 'c') + " can decrease but dj > 0");
+                res.violations.push_back("Variable " + name_of(M.col_names, j, 'c') + " can decrease but dj > 0: " + num(dj));
             }
             if (dj > tol) res.max_complementarity_violation = std::max(res.max_complementarity_violation, dj * std::fabs(xj - lj));
             else if (dj < -tol)
@@ -89,10 +99,12 @@ VerificationResult verify_solution(const Model& M, const Solution& sol, double t
             const double yi = sol.row_dual[i], li = M.row_lower[i], ui = M.row_upper[i];
             if (li <= -1e20 && ui < 1e20 && yi > tol) {
                 res.max_dual_violation = std::max(res.max_dual_violation, yi);
-                res.violations.push_back("Row " + name_of(M.row_names, i, 'r') + " (<= row) has positive dual");
+                res.violations.push_back("Row " + name_of(M.row_names, i, 'r') +
+                                         " (<= row) has positive dual multiplier yi: " + num(yi));
             } else if (li > -1e20 && ui >= 1e20 && yi < -tol) {
                 res.max_dual_violation = std::max(res.max_dual_violation, -yi);
-                res.violations.push_back("Row " + name_of(M.row_names, i, 'r') + " (>= row) has negative dual");
+                res.violations.push_back("Row " + name_of(M.row_names, i, 'r') +
+                                         " (>= row) has negative dual multiplier yi: " + num(yi));
             }
         }
         res.dual_feasible = res.max_dual_violation <= tol && res.max_complementarity_violation <= tol;
