@@ -276,6 +276,18 @@ int vj_op_diagnostics(int m, int n, const double* Ax, const double* row_lower,
                       const double* row_upper, const double* Aty, const double* c,
                       const double* x, const double* col_lower, const double* col_upper,
                       double* out4, void* stream);
+/* vj_op_diagnostics without any allocation: no counterpart in the reference. For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same inputs and the same four results, bit for bit, but
+ * out4_device is a DEVICE pointer and the call is asynchronous on `stream` (nothing is
+ * copied to the host, nothing is waited for). The per-workgroup partials live in a static
+ * device array of 1 MiB per GPU: max(m, n) must be <= 8,388,608, else
+ * VJ_ERR_INVALID_ARGUMENT. Every call on one GPU shares that array, so calls that overlap
+ * on one GPU (different streams or host threads) race; calls ordered on one stream do not. */
+int vj_op_diagnostics_static(int m, int n, const double* Ax, const double* row_lower,
+                             const double* row_upper, const double* Aty, const double* c,
+                             const double* x, const double* col_lower, const double* col_upper,
+                             double* out4_device, void* stream);
 
 #ifdef __cplusplus
 }

@@ -3,11 +3,43 @@
 // reference kernel contracts of src/solvers/gpu/spmv_kernels.cuh one by one.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "vj_device.hpp"
 #include "vj_internal.hpp"
 
 namespace vj {
 namespace {
+
+// Scratch of vj_op_diagnostics_static: the per-workgroup partials (4 doubles each) of at
+// most kDiagMaxBlocks workgroups, so max(m, n) <= kDiagMaxBlocks * kBlock (8,388,608).
+// Each GPU holds its own copy (1 MiB) until the process ends.
+constexpr int kDiagMaxBlocks = 32768;
+static __device__ __align__(256) double scratch[4 * kDiagMaxBlocks];
+
+// Device address of `scratch` on each device, looked up once per device with that device
+// current; host code passes this address to kernels, never the host-side symbol.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+int scratch_on_current_device(double** out) {
+    int dev = 0;
+    VJ_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("device index " + std::to_string(dev) + " beyond the " + std::to_string(kMaxDevices) +
+                  " devices the static scratch table covers");
+        return VJ_ERR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        VJ_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return VJ_OK;
+}
 
 // Row-group SpMV over int64 row pointers (the reference's CSR layout).
 template <int G>
@@ -220,6 +252,29 @@ int vj_op_diagnostics(int m, int n, const double* Ax, const double* row_lower, c
     VJ_LAUNCHED();
     VJ_HIP(hipMemcpyAsync(out4, o.p, 32, hipMemcpyDeviceToHost, s));
     VJ_HIP(hipStreamSynchronize(s));
+    return VJ_OK;
+}
+
+int vj_op_diagnostics_static(int m, int n, const double* Ax, const double* row_lower, const double* row_upper,
+                             const double* Aty, const double* c, const double* x, const double* col_lower,
+                             const double* col_upper, double* out4_device, void* stream) {
+    clear_error();
+    if (m < 0 || n < 0 || !out4_device) return VJ_ERR_INVALID_ARGUMENT;
+    const int nb = grid_for(m > n ? m : n, kBlock);
+    if (nb > kDiagMaxBlocks) {
+        set_error("vj_op_diagnostics_static: max(m, n) = " + std::to_string(m > n ? m : n) + " exceeds the " +
+                  std::to_string(static_cast<int64_t>(kDiagMaxBlocks) * kBlock) +
+                  " entries its static scratch covers; use vj_op_diagnostics");
+        return VJ_ERR_INVALID_ARGUMENT;
+    }
+    double* part = nullptr;
+    VJ_TRY(scratch_on_current_device(&part));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // every partial the final reduction reads is written by this call's first kernel
+    hipLaunchKernelGGL(op_diag_blocks, dim3(nb), dim3(kBlock), 0, s, m, n, Ax, row_lower, row_upper, Aty, c, x, col_lower,
+                       col_upper, part);
+    hipLaunchKernelGGL(op_diag_final, dim3(1), dim3(kBlock), 0, s, nb, part, out4_device);
+    VJ_LAUNCHED();
     return VJ_OK;
 }
 

@@ -98,7 +98,7 @@ EXPORTED = [
     "vj_problem_upload", "vj_params_set", "vj_solve", "vj_download", "vj_reference_iterates",
     "vj_audit_point", "vj_kernel_stats", "vj_benchmark_kernels", "vj_benchmark_blocks", "vj_batch_upload",
     "vj_batch_solve", "vj_batch_download", "vj_op_spmv", "vj_op_dual_update",
-    "vj_op_primal_update", "vj_op_diagnostics", "vj_gen_random_lp", "vj_gen_staircase_lp",
+    "vj_op_primal_update", "vj_op_diagnostics", "vj_op_diagnostics_static", "vj_gen_random_lp", "vj_gen_staircase_lp",
     "vj_gen_uniform_ints",
 ]
 
@@ -112,6 +112,7 @@ _lib.vj_op_spmv.argtypes = [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_
 _lib.vj_op_dual_update.argtypes = [_c_int, _c_dbl, _vp, _vp, _vp, _vp, _vp]
 _lib.vj_op_primal_update.argtypes = [_c_int, _c_dbl, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
 _lib.vj_op_diagnostics.argtypes = [_c_int, _c_int] + [_vp] * 9 + [_vp]
+_lib.vj_op_diagnostics_static.argtypes = [_c_int, _c_int] + [_vp] * 9 + [_vp]
 _lib.vj_reference_iterates.argtypes = [_vp, _c_i64, _vp, _vp, _vp]
 _lib.vj_audit_point.argtypes = [_vp, _vp, _vp, _vp]
 _lib.vj_download.argtypes = [_vp, _vp, _vp, _vp]
@@ -431,3 +432,11 @@ def op_diagnostics(Ax, row_lower, row_upper, Aty, c, x, col_lower, col_upper, st
                                   _dp(x), _dp(col_lower), _dp(col_upper), _p(out), _stream(stream)),
            "vj_op_diagnostics")
     return out
+
+
+def op_diagnostics_static(Ax, row_lower, row_upper, Aty, c, x, col_lower, col_upper, out4, stream=None):
+    """vj_op_diagnostics_static: the same four results written to the device tensor out4,
+    asynchronously, with no allocation (partials in a static device array)."""
+    _check(_lib.vj_op_diagnostics_static(Ax.numel(), x.numel(), _dp(Ax), _dp(row_lower), _dp(row_upper), _dp(Aty),
+                                         _dp(c), _dp(x), _dp(col_lower), _dp(col_upper), _dp(out4), _stream(stream)),
+           "vj_op_diagnostics_static")
