@@ -14,6 +14,8 @@
 //   indus::verifier::verify_solution      src/engine/verifier.cpp:25
 //   indus::Options (set / get_*)          src/engine/options.cpp:6-89
 //   indus::Solution::recompute_quality    src/engine/model.cpp:203-374
+//   indus::io::write_solution / write_json  src/io/writer.cpp:85-264
+//   indus::verifier::verify_files         src/engine/verifier.cpp:352-668
 #include "../tests/cpp/audit_dump.hpp"
 #include "../tests/cpp/options_probe.hpp"
 #include "../tests/cpp/presolve_dump.hpp"
@@ -278,6 +280,30 @@ int ref_audit_dump(const char* model_path, const char* point_path, double tol, c
         });
     std::fclose(f);
     return 0;
+}
+
+// The reference's .sol and JSON writers on the Solution audit_dump::solution_of builds from
+// the point in point_path (src/io/writer.cpp:85-264).
+int ref_write_files(const char* model_path, const char* point_path, const char* sol_path, const char* json_path) {
+    const indus::Model model = indus::io::read_mps(model_path);
+    audit_dump::Point p;
+    if (!audit_dump::read_point(point_path, p)) return 1;
+    const auto s = audit_dump::solution_of<indus::Solution, indus::Model, indus::SolveStatus>(model, p);
+    indus::io::write_solution(s, model, sol_path);
+    indus::io::write_json(s, model, json_path);
+    return 0;
+}
+
+// verify_files (src/engine/verifier.cpp:660-668: the model file read by the reference's
+// reader, the .sol parsed by its parser, then verify_solution), written as the verification
+// dump of audit_dump.hpp; returns 1 when the verification passed, 0 when not, -1 on I/O error.
+int ref_verify_files(const char* model_path, const char* sol_path, double tol, const char* out_path) {
+    const auto r = indus::verifier::verify_files(model_path, sol_path, tol);
+    FILE* f = std::fopen(out_path, "w");
+    if (!f) return -1;
+    audit_dump::verification(f, "files", r);
+    std::fclose(f);
+    return r.passed ? 1 : 0;
 }
 
 // The options probe script of options_probe.hpp run on the reference's Options; returns the

@@ -84,4 +84,22 @@ void run(FILE* f, const Model& model, const Point& p, double tol, Verify verify)
     verification(f, "audited", verify(model, s, tol));
 }
 
+// The Solution both sides' writers are given for a point: what a device PDHG solve reports
+// (status, objective and dual bound, x, row duals, algorithm, iterations) plus the row
+// activities, reduced costs and quality that recompute_quality fills in.
+template <class Solution, class Model, class Status>
+Solution solution_of(const Model& model, const Point& p) {
+    Solution s;
+    s.status = static_cast<Status>(p.status);
+    s.objective_value = p.objective;
+    s.best_dual_bound = p.objective;
+    s.col_value = p.x;
+    s.row_dual = p.row_dual;
+    s.algorithm_used = "pdhg_cuda";
+    s.iterations = 4096;
+    s.status_message = "converged";
+    s.recompute_quality(model);
+    return s;
+}
+
 }  // namespace audit_dump

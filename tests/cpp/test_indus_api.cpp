@@ -769,7 +769,7 @@ int api_solve(int argc, char** argv) {
     return 0;
 }
 
-// solve_mps <file.mps> <fp64|fp32> <tol> <presolve 0|1>: indus::solve on the device (the
+// solve_mps <file.mps> <fp64|fp32> <tol> <presolve 0|1> [out.sol out.json]: indus::solve on the device (the
 // reader, presolve, the PDHG engine, postsolve and the quality audit of the C++ API) on one
 // model file; prints the status, the objective, the reductions and the relative KKT error.
 int solve_mps(int argc, char** argv) {
@@ -784,6 +784,10 @@ int solve_mps(int argc, char** argv) {
     o.iteration_limit = 1000000;
     o.time_limit = 100.0;
     const indus::Solution s = indus::solve(M, o);
+    if (argc >= 8) {   // optional: the answer written by this build's .sol and JSON writers
+        indus::io::write_solution(s, M, argv[6]);
+        indus::io::write_json(s, M, argv[7]);
+    }
     const auto k = indus::verifier::relative_kkt(M, s);
     std::printf("solve_mps status=%d objective=%.17g reductions=%d iterations=%lld rel_kkt=%.6e\n",
                 static_cast<int>(s.status), s.objective_value, s.presolve_total_reductions,
@@ -805,6 +809,17 @@ int main(int argc, char** argv) {
                 return indus::verifier::verify_solution(m, s, t);
             });
         std::fclose(f);
+        return 0;
+    }
+    if (argc >= 6 && std::string(argv[1]) == "write_files") {
+        // write_files <mps> <point file> <out.sol> <out.json>: this build's writers on the
+        // Solution audit_dump::solution_of builds from the point
+        const indus::Model model = indus::io::read_mps(argv[2]);
+        audit_dump::Point p;
+        if (!audit_dump::read_point(argv[3], p)) return 3;
+        const auto s = audit_dump::solution_of<indus::Solution, indus::Model, indus::SolveStatus>(model, p);
+        indus::io::write_solution(s, model, argv[4]);
+        indus::io::write_json(s, model, argv[5]);
         return 0;
     }
     if (argc >= 2 && std::string(argv[1]) == "options_probe") {

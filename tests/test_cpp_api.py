@@ -275,14 +275,19 @@ def test_solve_mps_structured_end_to_end(tmp_path, seed, m, n, prec, presolve):
     look unbounded): OPTIMAL and the HiGHS optimum; the postsolved duals follow the
     reference's postsolve, which does not rebuild duals that meet the KKT conditions (its own
     pipeline, run on the 40 x 40 LP, returns a relative KKT error of 0.99), so they are not
-    checked here."""
+    checked here.
+
+    Without presolve the answer is also written by this build's .sol and JSON writers and the
+    reference's own verify_files (its MPS reader, .sol parser and verifier,
+    src/engine/verifier.cpp:352-668, from oracle/_ref where it is built) passes the .sol."""
     lp = lp_cases.structured_feasible_lp(seed, m, n)
     opt = lp_cases.highs_optimum(lp) - lp.offset
     path = str(tmp_path / f"s{seed}.mps")
     lp_cases.write_mps(lp, path)
     print(f"[case] solve_mps seed={seed} {m}x{n} {prec} presolve={presolve}", flush=True)
-    r = subprocess.run([_driver(), "solve_mps", path, prec, "1e-9", str(presolve)], capture_output=True, text=True,
-                       timeout=300)
+    sol, js = str(tmp_path / "answer.sol"), str(tmp_path / "answer.json")
+    r = subprocess.run([_driver(), "solve_mps", path, prec, "1e-9", str(presolve), sol, js], capture_output=True,
+                       text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
     kv = dict(t.split("=", 1) for t in r.stdout.splitlines()[0].split()[1:])
     assert int(kv["status"]) == 1, r.stdout
@@ -290,6 +295,14 @@ def test_solve_mps_structured_end_to_end(tmp_path, seed, m, n, prec, presolve):
         assert float(kv["rel_kkt"]) <= 1e-9 * 1.0001, r.stdout
     got = float(kv["objective"])
     assert abs(got - opt) <= 1e-6 * (1 + abs(opt)), (got, opt, r.stdout)
+    if not presolve and oracle.have_ref():
+        out = str(tmp_path / "verify.txt")
+        passed = oracle.ref_lib().ref_verify_files(path.encode(), sol.encode(), oracle._c_dbl(1e-6), out.encode())
+        with open(out) as f:
+            report = f.read()
+        assert passed == 1, report
+        with open(js) as f:
+            assert '"verification_status": "PASSED"' in f.read()
 
 
 def test_options_store_equals_reference():

@@ -86,3 +86,61 @@ def test_host_audits_equal_reference(tmp_path, tol):
             assert mine == ref, (name, k, status)
             compared += 1
     assert compared == len(cases) * 13
+
+
+PRODUCER_LINES = ("# Solution produced by ", "# Git Commit: ", '  "git_commit": ')
+
+
+def strip_producer(text):
+    return "\n".join(l for l in text.splitlines() if not l.startswith(PRODUCER_LINES))
+
+
+def write_both(mps, pt, tmp):
+    mine = (os.path.join(tmp, "mine.sol"), os.path.join(tmp, "mine.json"))
+    ref = (os.path.join(tmp, "ref.sol"), os.path.join(tmp, "ref.json"))
+    subprocess.run([DRIVER, "write_files", mps, pt, *mine], check=True, timeout=120)
+    assert O.ref_lib().ref_write_files(mps.encode(), pt.encode(), ref[0].encode(), ref[1].encode()) == 0
+    return mine, ref
+
+
+def verify_files(mps, sol, tol, tmp):
+    out = os.path.join(tmp, "verify.txt")
+    passed = O.ref_lib().ref_verify_files(mps.encode(), sol.encode(), O._c_dbl(tol), out.encode())
+    assert passed in (0, 1)
+    with open(out) as f:
+        return passed, f.read()
+
+
+@pytest.mark.skipif(not O.have_ref(), reason="the reference library (oracle/_ref) is not built here")
+def test_writers_equal_reference_and_parse_in_its_verifier(tmp_path):
+    """The .sol and JSON files this build writes (src/io/writer.cpp:85-264) equal the
+    reference writer's on the same Solution, line for line apart from the producer and commit
+    lines, and the reference's verify_files (its reader and .sol parser,
+    src/engine/verifier.cpp:352-668) reads this build's .sol into the same verification as
+    its own file's. Points of every status that a .sol names; values to 17 digits."""
+    rng = np.random.default_rng(7)
+    tmp = str(tmp_path)
+    cases = [(n, lp) for n, lp in lps() if n.startswith("feasible")][:5]
+    cases.append(("afiro", lp_cases.mps_lp("afiro")))
+    checked = 0
+    for name, lp in cases:
+        mps = os.path.join(lp_cases.GOLDEN, "afiro.mps") if name == "afiro" else os.path.join(tmp, f"{name}.mps")
+        if name != "afiro":
+            lp_cases.write_mps(lp, mps)
+            lp.offset = 0.0
+        for k, (status, obj, x, rd) in enumerate(points(lp, rng)):
+            if status not in (1, 2, 3, 5) or not np.all(np.isfinite(x)) or rd.size == 0:
+                continue   # a .sol names OPTIMAL, FEASIBLE, INFEASIBLE or UNBOUNDED; NaN is no file value
+            pt = os.path.join(tmp, "p.pt")
+            write_point(pt, status, obj, x, rd)
+            mine, ref = write_both(mps, pt, tmp)
+            for a, b in zip(mine, ref):
+                with open(a) as fa, open(b) as fb:
+                    assert strip_producer(fa.read()) == strip_producer(fb.read()), (name, k, a)
+            pm, vm = verify_files(mps, mine[0], 1e-6, tmp)
+            pr, vr = verify_files(mps, ref[0], 1e-6, tmp)
+            assert (pm, vm) == (pr, vr), (name, k)
+            for bad in ("Malformed", "Unrecognized", "Could not open", "Duplicate", "Unknown"):
+                assert bad not in vm, vm
+            checked += 1
+    assert checked >= 30
