@@ -249,9 +249,12 @@ int vj_batch_solve(vj_ctx* ctx, vj_result* out);
 int vj_batch_download(vj_ctx* ctx, double* x, double* y_canonical);
 
 /* ---- kernel-level operators on caller device memory ----------------------------
- * Device pointers, a hipStream_t passed as void* (NULL = default stream). These are the
- * fused HIP kernels of the hot loop, exposed so that each can be checked on its own;
- * they replace the reference kernels of src/solvers/gpu/spmv_kernels.cuh. */
+ * Device pointers, a hipStream_t passed as void* (NULL = default stream). These restate
+ * the reference's separate CUDA kernels (src/solvers/gpu/spmv_kernels.cuh) one by one, with
+ * their contracts, so that each can be checked on its own against the reference's kernel
+ * tests. They are not the restarted engine's hot loop: vj_solve runs fused kernels (SpMV,
+ * projection, extrapolation and Halpern step in one launch per half-step, over tiled,
+ * column-split or sliced-ELL layouts), which these entry points do not expose. */
 /* y = A x over CSR (int64 row pointers as in the reference); lanes = lanes per row.
  * Replaces spmv_csr_warp_kernel (spmv_kernels.cuh:39-76) and, applied to the CSR of A^T,
  * spmv_csc_transpose_warp_kernel (:101-138). exact_order = 1: one lane per row, ascending

@@ -1,0 +1,84 @@
+"""The production fused kernels of the restarted engine, pinned step by step: FP64 solves
+stopped after exactly B check blocks of K = 64 iterations (tolerance 0, max_iterations = B K)
+return the full iterate of the last check (centre + the last T(z) correction, unscaled), and
+that point must equal the numpy model of the engine (tests/engine_model.py: the same setup,
+correction coordinates, reflected Halpern step, restart rule and primal-weight update, written
+with scipy sparse products) to 1e-12 relative, for every kernel variant the autotuner can pick:
+tile-streamed, sliced ELL, lane groups, XCD column splits of A and of A^T, and the batched
+(block-diagonal) kernels. B = 1 checks the fused steps from the initial centre; B = 3 also
+crosses the restart (re-centring and the shifted data) that the first check always takes.
+
+The differences come only from summation order (scipy's products against the kernels' fixed
+orders) and from the setup's rounding, so they stay near 1e-15 relative."""
+import numpy as np
+import pytest
+
+import engine_model as M
+import lp_cases
+import vajra_pdhg as vp
+
+pytestmark = pytest.mark.gpu
+
+K = 64
+VARIANTS = {
+    "tile_streamed": dict(lanes_A=-1, lanes_At=-1, col_splits_A=1, col_splits_At=1),
+    "sliced_ell": dict(lanes_A=-3, lanes_At=-3, col_splits_A=1, col_splits_At=1),
+    "lane_groups_8": dict(lanes_A=8, lanes_At=8, col_splits_A=1, col_splits_At=1),
+    "split_A4_At2_streamed": dict(lanes_A=-1, lanes_At=-1, col_splits_A=4, col_splits_At=2),
+    "split_A2_At4_groups": dict(lanes_A=2, lanes_At=4, col_splits_A=2, col_splits_At=4),
+}
+
+
+def model_point(lp, blocks):
+    r = M.solve(lp, K=K, tol=0.0, max_iters=blocks * K)
+    assert r["iters"] == blocks * K
+    return r["x"], r["y"], r["restarts"]
+
+
+def close(got, want, what):
+    scale = max(1.0, float(np.max(np.abs(want))) if want.size else 1.0)
+    err = float(np.max(np.abs(got - want))) / scale if want.size else 0.0
+    assert err <= 1e-12, (what, err)
+    return err
+
+
+def lps():
+    return [("cfg0", lp_cases.cfg0_lp()), ("structured_5", lp_cases.structured_feasible_lp(5, 60, 150)),
+            ("structured_6", lp_cases.structured_feasible_lp(6, 150, 60))]
+
+
+@pytest.mark.parametrize("blocks", [1, 3])
+@pytest.mark.parametrize("variant", sorted(VARIANTS))
+def test_single_lp_blocks_equal_model(engine, variant, blocks):
+    worst = 0.0
+    for name, lp in lps():
+        print(f"[case] step parity {variant} blocks={blocks} {name}", flush=True)
+        engine.upload(lp)
+        r = engine.solve(vp.default_params(vp.MODE_RESTARTED, precision="fp64", tolerance=0.0,
+                                           max_iterations=blocks * K, check_frequency=K, **VARIANTS[variant]))
+        x, _, y = engine.download()
+        assert r.iterations == blocks * K and r.checks == blocks, r.as_dict()
+        mx, my, restarts = model_point(lp, blocks)
+        assert r.restart_count == restarts, (r.restart_count, restarts)
+        worst = max(worst, close(x, mx, (name, "x")), close(y, my, (name, "y")))
+    print(f"[case] worst relative difference {worst:.2e}", flush=True)
+
+
+@pytest.mark.parametrize("blocks", [1, 3])
+@pytest.mark.parametrize("lanes", [-1, 8])
+def test_batch_blocks_equal_model(engine, lanes, blocks):
+    """The batched kernels (every instance in one launch, per-instance state) on a batch of
+    LPs of different shapes: each instance equals its own model run."""
+    cases = lps()
+    engine.upload_batch([lp for _, lp in cases])
+    print(f"[case] batch step parity lanes={lanes} blocks={blocks}", flush=True)
+    res = engine.solve_batch(vp.default_params(vp.MODE_RESTARTED, precision="fp64", tolerance=0.0,
+                                               max_iterations=blocks * K, check_frequency=K, lanes_A=lanes,
+                                               lanes_At=lanes))
+    xs, ys = engine.download_batch()
+    for (name, lp), r, x, y in zip(cases, res, xs, ys):
+        assert r.iterations == blocks * K, r.as_dict()
+        mx, my, restarts = model_point(lp, blocks)
+        assert r.restart_count == restarts
+        close(x, mx, (name, "x"))
+        close(y, my, (name, "y"))

@@ -157,6 +157,23 @@ __global__ void unscale_out(int len, const double* __restrict__ a, const double*
 
 }  // namespace
 
+// The SpMV layout of one fused step, as the autotuner picks it.
+struct StepCfg {
+    int S, G;
+    bool stream;
+    bool rec;   // FP32 (index, value) records in the tile-streamed kernels
+    bool ell;   // sliced-ELL kernel (S = 1)
+};
+
+// An autotuner decision, kept for the rest of the upload: the same precision and kernel
+// parameters on the same problem reuse it, so repeated default solves run the same kernels
+// (and, the kernels being deterministic, return the same bits).
+struct TuneDecision {
+    int prec, lanes_A, lanes_At, col_splits_A, col_splits_At, check_frequency;
+    StepCfg a, at;
+    int GA, GAt, K_tuned;
+};
+
 struct FastEngine {
     int prec = VJ_PREC_FP64;
     int count = 0, m = 0, n = 0;
@@ -164,6 +181,7 @@ struct FastEngine {
     bool multi = false;
     int GA = 8, GAt = 4, K = 64;
     int K_tuned = 0;   // iterations per check picked by the autotuner (0: not timed)
+    std::vector<TuneDecision> tuned;   // decisions of earlier solves of this upload
     double rho = 1.0;
     DevBuf av_s, tv_s, av_lo, tv_lo;
     DevBuf Rsc, Csc, rstep, cstep, rnorm, cnorm;
@@ -178,8 +196,9 @@ struct FastEngine {
     // of A (gathers y); S = 1 means the fused single-pass kernels are used
     int SA = 1, SAt = 1, GsA = 2, GsAt = 2;
     DevBuf bpA, idxA, valA, psumA, bpAt, idxAt, valAt, psumAt, permA, permAt;
-    // row tiles of the split layouts for split_stream_partials (Gs = 0 selects that kernel)
-    DevBuf tileSA, tileSAt;
+    // row tiles of the split layouts for split_stream_partials (Gs = 0 selects that kernel) and
+    // each tile's first entry in every block, tbase[t * S + k] (t = 0..tiles)
+    DevBuf tileSA, tileSAt, tbaseSA, tbaseSAt;
     int ntSA = 0, ntSAt = 0, tcapSA = 0, tcapSAt = 0;   // tile count; LDS entries the tiles fit
     std::vector<int32_t> h_bpA, h_bpAt;                  // host copies of the block pointers
     // FP32: interleaved (index, value) records of A, A^T and of the split layouts, read by
@@ -248,11 +267,11 @@ int launch_primal(vj_ctx* ctx, FastEngine& E, const V* tv, T* x, T* xbar, const 
     if (E.SAt > 1) {
         VJ_TRY(require({E.bpAt.p, E.idxAt.p, E.valAt.p, E.psumAt.p}, "split primal_step"));
         if (E.GsAt == 0) {
-            VJ_TRY(require({E.tileSAt.p}, "split_stream_partials"));
+            VJ_TRY(require({E.tileSAt.p, E.tbaseSAt.p}, "split_stream_partials"));
             VJ_TRY(with_entries<V>(E.recAt, E.recSAt_buf, E.idxAt.as<int>(), E.valAt.as<V>(), "split records",
                                    [&](auto ent) {
                 hipLaunchKernelGGL((split_stream_partials<decltype(ent), T>), dim3(E.SAt * E.ntSAt), dim3(kBlock), 0,
-                                   ctx->stream, E.n, E.SAt, E.tileSAt.as<int>(), E.bpAt.as<int>(), ent, y,
+                                   ctx->stream, E.n, E.SAt, E.tileSAt.as<int>(), E.tbaseSAt.as<int>(), E.bpAt.as<int>(), ent, y,
                                    E.psumAt.as<T>(), E.st.as<InstState>());
             }));
         } else {
@@ -337,11 +356,11 @@ int launch_dual(vj_ctx* ctx, FastEngine& E, const V* av, const T* xbar, T* y, in
     if (E.SA > 1) {
         VJ_TRY(require({E.bpA.p, E.idxA.p, E.valA.p, E.psumA.p}, "split dual_step"));
         if (E.GsA == 0) {
-            VJ_TRY(require({E.tileSA.p}, "split_stream_partials"));
+            VJ_TRY(require({E.tileSA.p, E.tbaseSA.p}, "split_stream_partials"));
             VJ_TRY(with_entries<V>(E.recA, E.recSA_buf, E.idxA.as<int>(), E.valA.as<V>(), "split records",
                                    [&](auto ent) {
                 hipLaunchKernelGGL((split_stream_partials<decltype(ent), T>), dim3(E.SA * E.ntSA), dim3(kBlock), 0,
-                                   ctx->stream, E.m, E.SA, E.tileSA.as<int>(), E.bpA.as<int>(), ent, xbar,
+                                   ctx->stream, E.m, E.SA, E.tileSA.as<int>(), E.tbaseSA.as<int>(), E.bpA.as<int>(), ent, xbar,
                                    E.psumA.as<T>(), E.st.as<InstState>());
             }));
         } else {
@@ -827,8 +846,8 @@ int setup(vj_ctx* ctx, FastEngine& E) {
 // are gathered on the device from the scaled FP64 matrix in the working precision.
 int setup_one_split(vj_ctx* ctx, FastEngine& E, int S, int rows, int cols, const std::vector<int64_t>& rp,
                     const std::vector<int32_t>& ci, const double* vals_dev, int& built, DevBuf& bp, DevBuf& idx,
-                    DevBuf& perm, DevBuf& val, DevBuf& psum, DevBuf& tiles, int& ntiles, DevBuf& rec,
-                    std::vector<int32_t>& hbp, int& tiles_cap) {
+                    DevBuf& perm, DevBuf& val, DevBuf& psum, DevBuf& tiles, DevBuf& tbase, int& ntiles,
+                    DevBuf& rec, std::vector<int32_t>& hbp, int& tiles_cap) {
     hipStream_t s = ctx->stream;
     const int64_t nnz = rp[rows];
     if (built != S) {
@@ -851,8 +870,14 @@ int setup_one_split(vj_ctx* ctx, FastEngine& E, int S, int rows, int cols, const
         std::vector<int32_t> htile;
         build_split_tiles(rows, S, cap, max_rows, hbp, htile);
         ntiles = static_cast<int>(htile.size()) - 1;
+        std::vector<int32_t> hbase(htile.size() * static_cast<size_t>(S));
+        for (size_t t = 0; t < htile.size(); ++t)
+            for (int k = 0; k < S; ++k)
+                hbase[t * S + k] = hbp[static_cast<size_t>(k) * (rows + 1) + static_cast<size_t>(htile[t])];
         VJ_TRY(tiles.alloc(htile.size() * 4));
+        VJ_TRY(tbase.alloc(hbase.size() * 4));
         VJ_HIP(hipMemcpy(tiles.p, htile.data(), htile.size() * 4, hipMemcpyHostToDevice));
+        VJ_HIP(hipMemcpy(tbase.p, hbase.data(), hbase.size() * 4, hipMemcpyHostToDevice));
         tiles_cap = cap;
     }
     const size_t vsz = E.prec == VJ_PREC_FP64 ? 8 : (E.prec == VJ_PREC_FP32 ? 4 : 2);
@@ -882,11 +907,11 @@ int setup_splits(vj_ctx* ctx, FastEngine& E) {
     E.SAt = prm.col_splits_At ? prm.col_splits_At : choose_splits(E.m);
     if (E.SA > 1)
         VJ_TRY(setup_one_split(ctx, E, E.SA, E.m, E.n, P.h_rp, P.h_ci, E.av_s.as<double>(), E.builtSA, E.bpA, E.idxA,
-                               E.permA, E.valA, E.psumA, E.tileSA, E.ntSA, E.recSA_buf, E.h_bpA,
+                               E.permA, E.valA, E.psumA, E.tileSA, E.tbaseSA, E.ntSA, E.recSA_buf, E.h_bpA,
                                E.tcapSA));
     if (E.SAt > 1)
         VJ_TRY(setup_one_split(ctx, E, E.SAt, E.n, E.m, P.h_tp, P.h_ti, E.tv_s.as<double>(), E.builtSAt, E.bpAt,
-                               E.idxAt, E.permAt, E.valAt, E.psumAt, E.tileSAt, E.ntSAt, E.recSAt_buf,
+                               E.idxAt, E.permAt, E.valAt, E.psumAt, E.tileSAt, E.tbaseSAt, E.ntSAt, E.recSAt_buf,
                                E.h_bpAt, E.tcapSAt));
     // split partial kernels see ~nnz/(rows*S) entries per row: few lanes per row
     E.GsA = (static_cast<double>(E.nnz) / std::max(1, E.m) / E.SA) > 3.0 ? 4 : 2;
@@ -1069,12 +1094,6 @@ int time_kernels(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv, int reps,
 // Autotune, per matrix, the SpMV layout of each fused step on the real data: the
 // tile-streamed kernel, a lane-group kernel, or the XCD column split (when the gathered
 // vector exceeds the L2 budget) with a few lanes per block segment.
-struct StepCfg {
-    int S, G;
-    bool stream;
-    bool rec;   // FP32 (index, value) records in the tile-streamed kernels
-    bool ell;   // sliced-ELL kernel (S = 1)
-};
 
 template <class V, class T>
 int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
@@ -1125,6 +1144,20 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     E.K_tuned = 0;
     if (E.nnz < 200000 || (cA.size() == 1 && cAt.size() == 1))
         return VJ_OK;  // launch-bound sizes: the first candidate is as good as timing
+    for (const TuneDecision& d : E.tuned)
+        if (d.prec == E.prec && d.lanes_A == prm.lanes_A && d.lanes_At == prm.lanes_At &&
+            d.col_splits_A == prm.col_splits_A && d.col_splits_At == prm.col_splits_At &&
+            d.check_frequency == prm.check_frequency) {
+            apply(d.a, E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA);
+            apply(d.at, E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt);
+            E.GA = d.GA;
+            E.GAt = d.GAt;
+            E.K_tuned = d.K_tuned;
+            return VJ_OK;
+        }
+    // A later candidate replaces the best so far only when it is faster by more than this
+    // fraction: near-ties keep the earlier candidate instead of following timing noise.
+    constexpr double kTuneMargin = 0.03;
     const size_t rounds = std::max(cA.size(), cAt.size());
     double bestA = 1e30, bestAt = 1e30;
     StepCfg wA = cA[0], wAt = cAt[0];
@@ -1139,8 +1172,8 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
             std::fprintf(stderr, "[autotune] A: S=%d G=%d stream=%d rec=%d ell=%d %.1f us | A^T: S=%d G=%d stream=%d rec=%d ell=%d %.1f us\n",
                          a.S, a.G, a.stream ? 1 : 0, a.rec ? 1 : 0, a.ell ? 1 : 0, ud, b.S, b.G, b.stream ? 1 : 0,
                          b.rec ? 1 : 0, b.ell ? 1 : 0, up);
-        if (ud < bestA) { bestA = ud; wA = a; }
-        if (up < bestAt) { bestAt = up; wAt = b; }
+        if (r == 0 || (r < cA.size() && ud < (1.0 - kTuneMargin) * bestA)) { bestA = ud; wA = a; }
+        if (r == 0 || (r < cAt.size() && up < (1.0 - kTuneMargin) * bestAt)) { bestAt = up; wAt = b; }
     }
     apply(wA, E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA);
     apply(wAt, E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt);
@@ -1171,6 +1204,8 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
         while (K < 256 && t_check / K > 0.02 * t_iter) K *= 2;
         E.K_tuned = K;
     }
+    E.tuned.push_back(TuneDecision{E.prec, prm.lanes_A, prm.lanes_At, prm.col_splits_A, prm.col_splits_At,
+                                   prm.check_frequency, wA, wAt, E.GA, E.GAt, E.K_tuned});
     return VJ_OK;
 }
 

@@ -349,9 +349,22 @@ struct RecEntries {
     }
 };
 
+// Loads in flight per lane in one pass of the streamed SpMV loops: every lane issues all of
+// its entry loads of a pass, then all of its gathers. Tiles hold a few entries per lane
+// (about 3 per lane and column block on the 1M x 2M LP), so issuing them together, rather
+// than one load-gather-store chain at a time, is what hides the gather latency: the FP32
+// split A x_bar partials took 72.4 us with 8 per pass against 76.2 us with a 4-deep loop
+// and a one-at-a-time remainder; 16 per pass (84 VGPRs, lower occupancy) took 74.2 us.
+template <class T>
+constexpr int stream_depth() { return sizeof(T) == 8 ? 4 : 8; }
+
+// The fused tile-streamed steps (batches, and single LPs without a split) keep a 4-deep
+// loop with a one-at-a-time remainder: their tiles hold 6-15 entries per lane and, in the
+// batches, gather from vectors past the L2 capacity, and there the predicated passes were
+// slower (512 random LPs: 19.87 s against 19.31 s, the same iterations).
 template <class Ent, class T>
-__device__ __forceinline__ void stream_products(int s0, int s1, const Ent& ent, const T* __restrict__ vec,
-                                                T* __restrict__ lds) {
+__device__ __forceinline__ void stream_products_unrolled(int s0, int s1, const Ent& ent, const T* __restrict__ vec,
+                                                         T* __restrict__ lds) {
     constexpr int U = 4;
     int p = s0 + static_cast<int>(threadIdx.x);
     for (; p + (U - 1) * kBlock < s1; p += U * kBlock) {
@@ -367,6 +380,27 @@ __device__ __forceinline__ void stream_products(int s0, int s1, const Ent& ent, 
         T a;
         ent.load(p, c, a);
         lds[p - s0] = a * vec[c];
+    }
+}
+
+template <class Ent, class T>
+__device__ __forceinline__ void stream_products(int s0, int s1, const Ent& ent, const T* __restrict__ vec,
+                                                T* __restrict__ lds) {
+    constexpr int U = stream_depth<T>();
+    for (int p = s0 + static_cast<int>(threadIdx.x); p < s1; p += U * kBlock) {
+        int c[U];
+        T a[U], g[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            c[u] = 0;
+            a[u] = 0;
+            if (p + u * kBlock < s1) ent.load(p + u * kBlock, c[u], a[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) g[u] = p + u * kBlock < s1 ? vec[c[u]] : static_cast<T>(0);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (p + u * kBlock < s1) lds[p + u * kBlock - s0] = a[u] * g[u];
     }
 }
 
@@ -427,7 +461,7 @@ __global__ __launch_bounds__(kBlock) void stream_primal_step(
             q0 = tp[j] - s0;
             q1 = tp[j + 1] - s0;
         }
-        stream_products<Ent, T>(s0, s1, ent, y, lds);
+        stream_products_unrolled<Ent, T>(s0, s1, ent, y, lds);
         __syncthreads();
         if (mine) {
             T acc = 0;
@@ -476,7 +510,7 @@ __global__ __launch_bounds__(kBlock) void stream_dual_step(
             q0 = rp[i] - s0;
             q1 = rp[i + 1] - s0;
         }
-        stream_products<Ent, T>(s0, s1, ent, xbar, lds);
+        stream_products_unrolled<Ent, T>(s0, s1, ent, xbar, lds);
         __syncthreads();
         if (mine) {
             T w = 0;
@@ -538,8 +572,13 @@ constexpr int kSplitLdsBytes = 16384;
 template <class T>
 constexpr int split_cap() { return kSplitLdsBytes / static_cast<int>(sizeof(T)); }
 
+// tile_base[t * S + k] = bp[k][first row of tile t] (t = 0..tiles): the tile's span of block k
+// comes from one load that depends on nothing, so the entry stream starts without waiting
+// for the tile's rows and their pointers (70.9 us against 72.4 us for the FP32 A x_bar
+// partials); the row bounds are loaded after the stream is issued.
 template <class Ent, class T>
 __global__ __launch_bounds__(kBlock) void split_stream_partials(int rows, int S, const int* __restrict__ tile_row,
+                                                                const int* __restrict__ tile_base,
                                                                 const int* __restrict__ bp, Ent ent,
                                                                 const T* __restrict__ vec, T* __restrict__ part,
                                                                 const InstState* __restrict__ st) {
@@ -547,15 +586,16 @@ __global__ __launch_bounds__(kBlock) void split_stream_partials(int rows, int S,
     if (st[0].active == 0) return;
     const int k = static_cast<int>(blockIdx.x) % S;
     const int t = static_cast<int>(blockIdx.x) / S;
+    const int s0 = tile_base[t * S + k], s1 = tile_base[(t + 1) * S + k];
     const int r0 = tile_row[t], r1 = tile_row[t + 1];
     const int* p = bp + static_cast<int64_t>(k) * (rows + 1);
     T* out = part + static_cast<int64_t>(k) * rows;
-    const int s0 = p[r0], s1 = p[r1];
     if (s1 - s0 > split_cap<T>()) {
         const T acc = long_row_sum<Ent, T>(s0, s1, ent, vec, lds);
         if (threadIdx.x == 0) out[r0] = acc;
         return;
     }
+    stream_products<Ent, T>(s0, s1, ent, vec, lds);
     constexpr int RPT = split_rows<T>() / kBlock;
     int lo[RPT], hi[RPT];
 #pragma unroll
@@ -564,7 +604,6 @@ __global__ __launch_bounds__(kBlock) void split_stream_partials(int rows, int S,
         lo[u] = r < r1 ? p[r] - s0 : 0;
         hi[u] = r < r1 ? p[r + 1] - s0 : 0;
     }
-    stream_products<Ent, T>(s0, s1, ent, vec, lds);
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < RPT; ++u) {
@@ -655,21 +694,25 @@ __device__ __forceinline__ T ell_row_sum(int r, const int* __restrict__ off, con
     const int sl = r / kEllSlice;
     const int64_t base = static_cast<int64_t>(off[sl]) + (r % kEllSlice);
     const int w = width[sl];
+    // passes of up to stream_depth<T>() entries: all loads of a pass, then all its gathers
+    // (the FP32 ELL primal step of the 1M x 2M LP, 6 entries per row: 77-79 us against 82 us
+    // for a 4-deep loop with a one-at-a-time remainder)
+    constexpr int U = stream_depth<T>();
     T acc = 0;
-    int q = 0;
-    for (; q + 3 < w; q += 4) {
-        int c[4];
-        T a[4];
+    for (int q0 = 0; q0 < w; q0 += U) {
+        int c[U];
+        T a[U], g[U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) ent.load(base + static_cast<int64_t>(q + u) * kEllSlice, c[u], a[u]);
+        for (int u = 0; u < U; ++u) {
+            c[u] = 0;
+            a[u] = 0;
+            if (q0 + u < w) ent.load(base + static_cast<int64_t>(q0 + u) * kEllSlice, c[u], a[u]);
+        }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc += a[u] * vec[c[u]];
-    }
-    for (; q < w; ++q) {
-        int c;
-        T a;
-        ent.load(base + static_cast<int64_t>(q) * kEllSlice, c, a);
-        acc += a * vec[c];
+        for (int u = 0; u < U; ++u) g[u] = q0 + u < w ? vec[c[u]] : static_cast<T>(0);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (q0 + u < w) acc += a[u] * g[u];
     }
     return acc;
 }
