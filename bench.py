@@ -89,14 +89,13 @@ def main():
     # ---- workload: large random LP -------------------------------------------------------
     t0 = time.perf_counter()
     lp = vp.gen_random_lp(CFG1["seed"], CFG1["m"], CFG1["n"], CFG1["k"], CFG1["frac"], CFG1["rng"])
-    lp.csr()
     t_gen = time.perf_counter() - t0
     log(f"generated {lp.m}x{lp.n} nnz={lp.nnz} in {t_gen:.1f}s")
     eng = vp.Engine(0)
     t0 = time.perf_counter()
     eng.upload(lp)
     t_up = time.perf_counter() - t0
-    log(f"uploaded in {t_up:.2f}s")
+    log(f"uploaded in {t_up:.2f}s (the CSC; the CSR of A is built on the device)")
 
     extra = {}
     flags = []
@@ -105,20 +104,23 @@ def main():
     if not args.skip_fp64:
         r64, wall64 = solve_timed(vp, eng, "fp64", args.tol, args.max_iters, args.time_limit)
         log(f"fp64: {vp.STATUS[r64.status]} iters={r64.iterations} restarts={r64.restart_count} "
-            f"rel_kkt={r64.rel_kkt:.3e} setup={r64.setup_time:.3f}s iter={r64.iteration_time:.3f}s wall={wall64:.3f}s")
+            f"rel_kkt={r64.rel_kkt:.3e} setup={r64.setup_time:.3f}s layout={r64.layout_time:.3f}s "
+            f"iter={r64.iteration_time:.3f}s wall={wall64:.3f}s")
         if r64.status != 1:
             flags.append(f"fp64 solve ended {vp.STATUS[r64.status]}")
         k64 = eng.benchmark_kernels(reps=30)
         extra["fp64"] = dict(status=vp.STATUS[r64.status], iterations=r64.iterations, restarts=r64.restart_count,
                              rel_kkt=r64.rel_kkt, objective=r64.primal_objective, time_to_target_s=r64.time_to_target,
-                             setup_s=r64.setup_time, iteration_s=r64.iteration_time, solve_wall_s=wall64,
+                             setup_s=r64.setup_time, layout_s=r64.layout_time, iteration_s=r64.iteration_time,
+                             solve_wall_s=wall64,
                              lanes=[r64.lanes_A, r64.lanes_At], splits=[r64.col_splits_A, r64.col_splits_At],
                              check_interval=r64.check_interval, kernels=k64)
 
     # ---- headline: FP32 iterates + FP64 refinement, one complete solve --------------------
     res, wall = solve_timed(vp, eng, args.precision, args.tol, args.max_iters, args.time_limit)
     log(f"{args.precision}: {vp.STATUS[res.status]} iters={res.iterations} restarts={res.restart_count} "
-        f"rel_kkt={res.rel_kkt:.3e} setup={res.setup_time:.3f}s iter={res.iteration_time:.3f}s wall={wall:.3f}s")
+        f"rel_kkt={res.rel_kkt:.3e} setup={res.setup_time:.3f}s layout={res.layout_time:.3f}s "
+        f"iter={res.iteration_time:.3f}s wall={wall:.3f}s")
     if res.status != 1:
         flags.append(f"headline solve ended {vp.STATUS[res.status]}")
     x, rd, y = eng.download()
@@ -138,7 +140,8 @@ def main():
     extra["mixed"] = dict(precision=args.precision, status=vp.STATUS[res.status], iterations=res.iterations,
                           restarts=res.restart_count, rel_kkt_device=res.rel_kkt, rel_kkt_oracle=oracle_kkt,
                           objective=res.primal_objective, time_to_target_s=res.time_to_target,
-                          setup_s=res.setup_time, iteration_s=res.iteration_time, solve_wall_s=wall,
+                          setup_s=res.setup_time, layout_s=res.layout_time, iteration_s=res.iteration_time,
+                          solve_wall_s=wall,
                           lanes=[res.lanes_A, res.lanes_At], splits=[res.col_splits_A, res.col_splits_At],
                           check_interval=res.check_interval, block_ms=ms_block, kernels=kmix)
     if r64 is not None and r64.status == 1 and res.status == 1:
@@ -234,8 +237,6 @@ def bench_batch(vp, kind, args, budget_left):
         ts = vp.gen_uniform_ints(CFG4["seed"], cnt, CFG4["lo"], CFG4["hi"])
         lps = [vp.gen_staircase_lp(CFG4["seed"] * 1000 + i, int(t)) for i, t in enumerate(ts)]
         seq_n = args.seq_staircase
-    for lp in lps:
-        lp.csr()
     nnz = sum(lp.nnz for lp in lps)
     log(f"batch {kind}: {cnt} LPs, sum nnz={nnz}, generated in {time.perf_counter() - t0:.1f}s")
     eng = vp.Engine(0)

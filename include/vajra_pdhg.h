@@ -9,10 +9,13 @@
  * indus::gpu::solve_pdhg_gpu (src/solvers/gpu/gpu_context.cpp:65). Each entry point below
  * names the part of that seam it replaces.
  *
- * Matrix layout conventions: the caller passes A both as CSR and as CSR of A^T (which is
- * the CSC of A, the reference Model layout, include/indus/sparse.hpp:20-65). Row
- * pointers are int64 as in the reference; the engine narrows them to int32 on the device
- * (nnz < 2^31 is checked). Column indices must be ascending within each row.
+ * Matrix layout conventions: the caller passes A as CSR of A^T (which is the CSC of A, the
+ * reference Model layout, include/indus/sparse.hpp:20-65) and, optionally, also as CSR; when
+ * the CSR arrays are NULL the engine builds the CSR of A on the device from the CSC (a
+ * stable sort of the entries by row, the order SparseMatrixCSC::to_csr produces,
+ * src/linalg/sparse.cpp:147-184). Row pointers are int64 as in the reference; the engine
+ * narrows them to int32 on the device (nnz < 2^31 is checked). Indices must be strictly
+ * ascending within each row and column.
  * Infinite bounds: any |bound| >= 1e20 (or +-inf) is infinite.
  *
  * Threading: one context is used by one host thread at a time. Calls on different
@@ -149,6 +152,9 @@ typedef struct vj_result {
     int lanes_A, lanes_At, check_interval;
     int64_t checks;
     int col_splits_A, col_splits_At;   /* column blocks used by the XCD-split SpMV (1 = fused) */
+    double layout_time;            /* seconds of this solve spent building kernel layouts on the
+                                      device (column split, sliced ELL, their row tiles); reused
+                                      by later solves of the same upload, then near 0 */
 } vj_result;
 
 /* ---- single-LP context (replaces solve_pdhg_cuda_internal, pdhg_cuda.cu:109-375) */
@@ -156,12 +162,13 @@ typedef struct vj_ctx vj_ctx;
 int vj_ctx_create(int device, vj_ctx** out);
 int vj_ctx_destroy(vj_ctx* ctx);
 
-/* Host LP in the Model layout plus the CSR of A. Arrays are copied; the engine owns its
- * device buffers (replaces the H2D uploads of pdhg_cuda.cu:187-231). */
+/* Host LP in the Model layout, optionally with the CSR of A (csr_* all NULL: built on the
+ * device). Arrays are copied; the engine owns its device buffers (replaces the host CSR build
+ * and the H2D uploads of pdhg_cuda.cu:159,187-231). */
 typedef struct vj_lp {
     int m, n;
     int64_t nnz;
-    const int64_t* csr_row_ptr;    /* m+1 */
+    const int64_t* csr_row_ptr;    /* m+1, or NULL (with csr_col_idx and csr_vals) */
     const int32_t* csr_col_idx;    /* nnz, ascending in each row */
     const double* csr_vals;        /* nnz */
     const int64_t* csc_col_ptr;    /* n+1 (= CSR of A^T) */
@@ -192,6 +199,13 @@ int vj_reference_iterates(vj_ctx* ctx, int64_t k, double* x, double* x_bar, doub
  * computed on the device. out: rel_kkt, rel_p, rel_d, rel_gap, pobj, dobj, |r_p|, |r_d|. */
 int vj_audit_point(vj_ctx* ctx, const double* x, const double* y_canonical, double* out8);
 
+/* The cumulative equilibration factors of the last VJ_MODE_RESTARTED solve, computed on the
+ * device: row_scale[m] and col_scale[n] after the Ruiz passes and the Pock-Chambolle pass
+ * (before the per-instance bound/objective rescaling); A_scaled = diag(row) A diag(col).
+ * Either pointer may be NULL. (Counterpart of RuizScaler's factors, src/linalg/scaling.hpp:11-17,
+ * which the reference computes on the host, src/linalg/scaling.cpp:7-153.) */
+int vj_scaling_factors(vj_ctx* ctx, double* row_scale, double* col_scale);
+
 /* Timing of the dominant kernels of the last solve, measured with HIP events on the
  * context stream: kernel names, average microseconds per launch, launches, and
  * algorithmic bytes per launch. Returns the number of entries written (<= cap). */
@@ -220,7 +234,7 @@ int vj_benchmark_blocks(vj_ctx* ctx, int warmup, int steps, double* ms_per_block
  * keeps its own step sizes, primal weight, restart and convergence state. Arrays are
  * concatenations of the per-instance arrays with per-instance index bases: row/col
  * indices are LOCAL to the instance, pointers are local offsets (each instance's
- * pointer array starts at 0). */
+ * pointer array starts at 0). The CSR arrays may be NULL together, as for one LP. */
 typedef struct vj_batch {
     int count;
     const int32_t* m;              /* count */

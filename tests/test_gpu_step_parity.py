@@ -29,8 +29,8 @@ VARIANTS = {
 }
 
 
-def model_point(lp, blocks):
-    r = M.solve(lp, K=K, tol=0.0, max_iters=blocks * K)
+def model_point(lp, blocks, rho=1.0):
+    r = M.solve(lp, K=K, tol=0.0, max_iters=blocks * K, rho=rho)
     assert r["iters"] == blocks * K
     return r["x"], r["y"], r["restarts"]
 
@@ -79,6 +79,23 @@ def test_batch_blocks_equal_model(engine, lanes, blocks):
     for (name, lp), r, x, y in zip(cases, res, xs, ys):
         assert r.iterations == blocks * K, r.as_dict()
         mx, my, restarts = model_point(lp, blocks)
+        assert r.restart_count == restarts
+        close(x, mx, (name, "x"))
+        close(y, my, (name, "y"))
+
+
+@pytest.mark.parametrize("blocks", [1, 3])
+@pytest.mark.parametrize("variant", ["sliced_ell", "split_A4_At2_streamed", "tile_streamed"])
+def test_reflection_half_equals_model(engine, variant, blocks):
+    """Reflection 0.5: the x iterate then lives in its own vector (with reflection 1 the steps
+    recover it from x_bar); both forms equal the model."""
+    for name, lp in lps()[:2]:
+        print(f"[case] step parity reflection 0.5 {variant} blocks={blocks} {name}", flush=True)
+        engine.upload(lp)
+        r = engine.solve(vp.default_params(vp.MODE_RESTARTED, precision="fp64", tolerance=0.0, reflection=0.5,
+                                           max_iterations=blocks * K, check_frequency=K, **VARIANTS[variant]))
+        x, _, y = engine.download()
+        mx, my, restarts = model_point(lp, blocks, rho=0.5)
         assert r.restart_count == restarts
         close(x, mx, (name, "x"))
         close(y, my, (name, "y"))

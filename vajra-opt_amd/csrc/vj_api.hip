@@ -10,6 +10,7 @@
 #include <string>
 
 #include "vj_context.hpp"
+#include "vj_layout.hpp"
 
 namespace vj {
 namespace {
@@ -43,8 +44,9 @@ int upload_common(vj_ctx* ctx) {
         set_error("nnz must be below 2^31");
         return VJ_ERR_INVALID_ARGUMENT;
     }
-    std::vector<int32_t> rp(P.m + 1), tp(P.n + 1);
-    for (int i = 0; i <= P.m; ++i) rp[i] = static_cast<int32_t>(P.h_rp[i]);
+    const bool csr_given = !P.h_rp.empty();
+    std::vector<int32_t> rp(csr_given ? P.m + 1 : 0), tp(P.n + 1);
+    for (size_t i = 0; i < rp.size(); ++i) rp[i] = static_cast<int32_t>(P.h_rp[i]);
     for (int j = 0; j <= P.n; ++j) tp[j] = static_cast<int32_t>(P.h_tp[j]);
     std::vector<double> ccan(P.n), lc(P.n), uc(P.n), lr(P.m), ur(P.m);
     std::vector<int32_t> rinst(P.m), cinst(P.n), roff(P.count + 1), coff(P.count + 1);
@@ -91,12 +93,18 @@ int upload_common(vj_ctx* ctx) {
         VJ_HIP(hipMemcpyAsync(b.p, h, bytes, hipMemcpyHostToDevice, s));
         return VJ_OK;
     };
-    VJ_TRY(up(P.rp, rp.data(), rp.size() * 4));
-    VJ_TRY(up(P.ci, P.h_ci.data(), P.nnz * 4));
-    VJ_TRY(up(P.av, P.h_av.data(), P.nnz * 8));
     VJ_TRY(up(P.tp, tp.data(), tp.size() * 4));
     VJ_TRY(up(P.ti, P.h_ti.data(), P.nnz * 4));
     VJ_TRY(up(P.tv, P.h_tv.data(), P.nnz * 8));
+    if (csr_given) {
+        VJ_TRY(up(P.rp, rp.data(), rp.size() * 4));
+        VJ_TRY(up(P.ci, P.h_ci.data(), P.nnz * 4));
+        VJ_TRY(up(P.av, P.h_av.data(), P.nnz * 8));
+    } else {
+        // the CSR of A, transposed on the device from the uploaded CSC (to_csr order)
+        VJ_TRY(device_transpose(s, P.n, P.m, P.nnz, P.tp.as<int>(), P.ti.as<int>(), P.tv.as<double>(), P.rp, P.ci,
+                                P.av));
+    }
     VJ_TRY(up(P.c_can, ccan.data(), P.n * 8));
     VJ_TRY(up(P.lc, lc.data(), P.n * 8));
     VJ_TRY(up(P.uc, uc.data(), P.n * 8));
@@ -279,14 +287,16 @@ int vj_problem_upload(vj_ctx* ctx, const vj_lp* lp) {
         set_error("the device engine needs m > 0, n > 0 and nnz > 0 (callers solve empty LPs on the host)");
         return VJ_ERR_INVALID_ARGUMENT;
     }
-    if (!lp->csr_row_ptr || !lp->csr_col_idx || !lp->csr_vals || !lp->csc_col_ptr || !lp->csc_row_idx ||
-        !lp->csc_vals || !lp->c || !lp->col_lower || !lp->col_upper || !lp->row_lower || !lp->row_upper) {
-        set_error("null array in vj_lp");
+    const int csr_parts = (lp->csr_row_ptr ? 1 : 0) + (lp->csr_col_idx ? 1 : 0) + (lp->csr_vals ? 1 : 0);
+    if ((csr_parts != 0 && csr_parts != 3) || !lp->csc_col_ptr || !lp->csc_row_idx || !lp->csc_vals || !lp->c ||
+        !lp->col_lower || !lp->col_upper || !lp->row_lower || !lp->row_upper) {
+        set_error("null array in vj_lp (the CSR arrays may be NULL together: the CSR is then built on the device)");
         return VJ_ERR_INVALID_ARGUMENT;
     }
-    if (!check_ptr_array(lp->csr_row_ptr, lp->m, lp->nnz) || !check_ptr_array(lp->csc_col_ptr, lp->n, lp->nnz) ||
-        !check_idx(lp->csr_row_ptr, lp->csr_col_idx, lp->m, lp->n) ||
-        !check_idx(lp->csc_col_ptr, lp->csc_row_idx, lp->n, lp->m)) {
+    const bool csr = csr_parts == 3;
+    if (!check_ptr_array(lp->csc_col_ptr, lp->n, lp->nnz) || !check_idx(lp->csc_col_ptr, lp->csc_row_idx, lp->n, lp->m) ||
+        (csr && (!check_ptr_array(lp->csr_row_ptr, lp->m, lp->nnz) ||
+                 !check_idx(lp->csr_row_ptr, lp->csr_col_idx, lp->m, lp->n)))) {
         set_error("malformed CSR/CSC arrays (pointers must be monotone from 0 to nnz, indices in range and strictly ascending per row)");
         return VJ_ERR_INVALID_ARGUMENT;
     }
@@ -304,9 +314,11 @@ int vj_problem_upload(vj_ctx* ctx, const vj_lp* lp) {
     P.col_off = {0, lp->n};
     P.maximize_i = {lp->maximize ? 1 : 0};
     P.offset_i = {(lp->maximize ? -1.0 : 1.0) * lp->objective_offset};
-    P.h_rp.assign(lp->csr_row_ptr, lp->csr_row_ptr + lp->m + 1);
-    P.h_ci.assign(lp->csr_col_idx, lp->csr_col_idx + lp->nnz);
-    P.h_av.assign(lp->csr_vals, lp->csr_vals + lp->nnz);
+    if (csr) {
+        P.h_rp.assign(lp->csr_row_ptr, lp->csr_row_ptr + lp->m + 1);
+        P.h_ci.assign(lp->csr_col_idx, lp->csr_col_idx + lp->nnz);
+        P.h_av.assign(lp->csr_vals, lp->csr_vals + lp->nnz);
+    }
     P.h_tp.assign(lp->csc_col_ptr, lp->csc_col_ptr + lp->n + 1);
     P.h_ti.assign(lp->csc_row_idx, lp->csc_row_idx + lp->nnz);
     P.h_tv.assign(lp->csc_vals, lp->csc_vals + lp->nnz);
@@ -330,11 +342,13 @@ int vj_problem_upload(vj_ctx* ctx, const vj_lp* lp) {
 int vj_batch_upload(vj_ctx* ctx, const vj_batch* b) {
     clear_error();
     if (!ctx || !b || b->count <= 0) return VJ_ERR_INVALID_ARGUMENT;
-    if (!b->m || !b->n || !b->nnz || !b->csr_row_ptr || !b->csr_col_idx || !b->csr_vals || !b->csc_col_ptr ||
-        !b->csc_row_idx || !b->csc_vals || !b->c || !b->col_lower || !b->col_upper || !b->row_lower || !b->row_upper) {
-        set_error("null array in vj_batch");
+    const int csr_parts = (b->csr_row_ptr ? 1 : 0) + (b->csr_col_idx ? 1 : 0) + (b->csr_vals ? 1 : 0);
+    if (!b->m || !b->n || !b->nnz || (csr_parts != 0 && csr_parts != 3) || !b->csc_col_ptr || !b->csc_row_idx ||
+        !b->csc_vals || !b->c || !b->col_lower || !b->col_upper || !b->row_lower || !b->row_upper) {
+        set_error("null array in vj_batch (the CSR arrays may be NULL together: the CSR is then built on the device)");
         return VJ_ERR_INVALID_ARGUMENT;
     }
+    const bool csr = csr_parts == 3;
     VJ_HIP(hipSetDevice(ctx->device));
     Problem& P = ctx->prob;
     P.reset();
@@ -374,29 +388,31 @@ int vj_batch_upload(vj_ctx* ctx, const vj_batch* b) {
         P.maximize_i[q] = mx ? 1 : 0;
         P.offset_i[q] = (mx ? -1.0 : 1.0) * off;   // canonical (minimisation) sign, as for one LP
     }
-    P.h_rp.resize(P.m + 1);
+    if (csr) {
+        P.h_rp.resize(P.m + 1);
+        P.h_ci.resize(P.nnz);
+        P.h_av.assign(b->csr_vals, b->csr_vals + P.nnz);
+    }
     P.h_tp.resize(P.n + 1);
-    P.h_ci.resize(P.nnz);
     P.h_ti.resize(P.nnz);
-    P.h_av.assign(b->csr_vals, b->csr_vals + P.nnz);
     P.h_tv.assign(b->csc_vals, b->csc_vals + P.nnz);
     for (int q = 0; q < b->count; ++q) {
         const int64_t r0 = P.row_off[q], c0 = P.col_off[q], z0 = nnz_off[q];
-        const int64_t* rp = b->csr_row_ptr + r0 + q;  // each instance has m+1 pointers
+        const int64_t* rp = csr ? b->csr_row_ptr + r0 + q : nullptr;  // each instance has m+1 pointers
         const int64_t* tp = b->csc_col_ptr + c0 + q;
-        if (!check_ptr_array(rp, b->m[q], b->nnz[q]) || !check_ptr_array(tp, b->n[q], b->nnz[q]) ||
-            !check_idx(rp, b->csr_col_idx + z0, b->m[q], b->n[q]) || !check_idx(tp, b->csc_row_idx + z0, b->n[q], b->m[q])) {
+        if (!check_ptr_array(tp, b->n[q], b->nnz[q]) || !check_idx(tp, b->csc_row_idx + z0, b->n[q], b->m[q]) ||
+            (csr && (!check_ptr_array(rp, b->m[q], b->nnz[q]) || !check_idx(rp, b->csr_col_idx + z0, b->m[q], b->n[q])))) {
             set_error("malformed CSR/CSC arrays in batch instance " + std::to_string(q));
             return VJ_ERR_INVALID_ARGUMENT;
         }
-        for (int i = 0; i < b->m[q]; ++i) P.h_rp[r0 + i] = z0 + rp[i];
         for (int j = 0; j < b->n[q]; ++j) P.h_tp[c0 + j] = z0 + tp[j];
-        for (int64_t k = 0; k < b->nnz[q]; ++k) {
-            P.h_ci[z0 + k] = static_cast<int32_t>(c0 + b->csr_col_idx[z0 + k]);
-            P.h_ti[z0 + k] = static_cast<int32_t>(r0 + b->csc_row_idx[z0 + k]);
+        for (int64_t k = 0; k < b->nnz[q]; ++k) P.h_ti[z0 + k] = static_cast<int32_t>(r0 + b->csc_row_idx[z0 + k]);
+        if (csr) {
+            for (int i = 0; i < b->m[q]; ++i) P.h_rp[r0 + i] = z0 + rp[i];
+            for (int64_t k = 0; k < b->nnz[q]; ++k) P.h_ci[z0 + k] = static_cast<int32_t>(c0 + b->csr_col_idx[z0 + k]);
         }
     }
-    P.h_rp[P.m] = P.nnz;
+    if (csr) P.h_rp[P.m] = P.nnz;
     P.h_tp[P.n] = P.nnz;
     P.h_c.assign(b->c, b->c + P.n);
     P.h_lc.assign(b->col_lower, b->col_lower + P.n);
@@ -546,6 +562,13 @@ int vj_benchmark_kernels(vj_ctx* ctx, int reps, vj_kernel_stat* out, int cap) {
         ctx->timers.push_back(t);
     }
     return written;
+}
+
+int vj_scaling_factors(vj_ctx* ctx, double* row_scale, double* col_scale) {
+    clear_error();
+    if (!ctx) return VJ_ERR_INVALID_ARGUMENT;
+    VJ_HIP(hipSetDevice(ctx->device));
+    return fast_scaling(ctx, row_scale, col_scale);
 }
 
 int vj_benchmark_blocks(vj_ctx* ctx, int warmup, int steps, double* ms_per_block, int* iterations_per_block) {

@@ -89,6 +89,7 @@ int reference_solve(vj_ctx* ctx, vj_result* out, int64_t stop_at_iter, double* t
                     double* traj_xbar, double* traj_y);
 int fast_solve(vj_ctx* ctx, vj_result* out_array);
 int fast_benchmark(vj_ctx* ctx, int reps, vj_kernel_stat* out, int cap, int* written);
+int fast_scaling(vj_ctx* ctx, double* row_scale, double* col_scale);
 int fast_benchmark_blocks(vj_ctx* ctx, int warmup, int steps, double* ms_per_block, int* iters_per_block);
 // FP64 rel-KKT audit of device vectors (original space, canonical dual) for all
 // instances; out8 has count*8 entries.

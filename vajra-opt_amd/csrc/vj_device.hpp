@@ -10,6 +10,7 @@ namespace vj {
 
 constexpr int kBlock = 256;          // threads per workgroup (4 wavefronts of 64)
 constexpr int kWave = 64;
+constexpr int kEllSlice = 64;        // rows per slice of the sliced-ELL layout (one wavefront)
 
 // BF16 matrix values are stored as raw 16-bit patterns (upper half of an FP32).
 struct bf16_t { uint16_t bits; };

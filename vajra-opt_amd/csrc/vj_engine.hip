@@ -20,6 +20,7 @@
 
 #include "vj_context.hpp"
 #include "vj_kernels.hpp"
+#include "vj_layout.hpp"
 
 namespace vj {
 
@@ -200,7 +201,6 @@ struct FastEngine {
     // each tile's first entry in every block, tbase[t * S + k] (t = 0..tiles)
     DevBuf tileSA, tileSAt, tbaseSA, tbaseSAt;
     int ntSA = 0, ntSAt = 0, tcapSA = 0, tcapSAt = 0;   // tile count; LDS entries the tiles fit
-    std::vector<int32_t> h_bpA, h_bpAt;                  // host copies of the block pointers
     // FP32: interleaved (index, value) records of A, A^T and of the split layouts, read by
     // the tile-streamed kernels when the autotuner picks them (recA / recAt)
     DevBuf recA_buf, recAt_buf, recSA_buf, recSAt_buf;
@@ -500,134 +500,6 @@ int choose_lanes(double avg_nnz) {
     return 32;
 }
 
-// Split layout of a CSR (rows x cols) into S column blocks of width ceil(cols/S):
-// block-major arrays with absolute row pointers bp[k*(rows+1) + r], column indices and the
-// permutation perm[q] = position of split entry q in the original CSR.
-void build_split(int rows, int cols, const std::vector<int64_t>& rp, const std::vector<int32_t>& ci, int S,
-                 std::vector<int32_t>& bp, std::vector<int32_t>& idx, std::vector<int32_t>& perm) {
-    const int64_t nnz = rp[rows];
-    const int64_t B = (static_cast<int64_t>(cols) + S - 1) / S;
-    bp.assign(static_cast<size_t>(S) * (rows + 1), 0);
-    for (int r = 0; r < rows; ++r)
-        for (int64_t p = rp[r]; p < rp[r + 1]; ++p) bp[static_cast<size_t>(ci[p] / B) * (rows + 1) + r + 1]++;
-    int64_t base = 0;
-    for (int k = 0; k < S; ++k) {
-        int32_t* b = bp.data() + static_cast<size_t>(k) * (rows + 1);
-        b[0] = static_cast<int32_t>(base);
-        for (int r = 0; r < rows; ++r) b[r + 1] += b[r];
-        base = b[rows];
-    }
-    idx.resize(nnz);
-    perm.resize(nnz);
-    std::vector<int32_t> cur(bp);
-    for (int r = 0; r < rows; ++r)
-        for (int64_t p = rp[r]; p < rp[r + 1]; ++p) {
-            const int k = static_cast<int>(ci[p] / B);
-            const int32_t q = cur[static_cast<size_t>(k) * (rows + 1) + r]++;
-            idx[q] = ci[p];
-            perm[q] = static_cast<int32_t>(p);
-        }
-}
-
-// Row tiles for the tile-streamed kernels: at most 256 rows and kStreamCap nonzeros per
-// tile; a row longer than kStreamCap is a tile of its own. first/last: per instance, the
-// tiles holding its first and last row.
-void build_stream_tiles(int rows, const std::vector<int64_t>& rp, const std::vector<int64_t>& off, int count,
-                        std::vector<int32_t>& tile_row, std::vector<int32_t>& first, std::vector<int32_t>& last) {
-    tile_row.assign(1, 0);
-    int r = 0;
-    while (r < rows) {
-        const int64_t start = rp[r];
-        int e = r + 1;
-        if (rp[r + 1] - start <= kStreamCap)
-            while (e < rows && e - r < kBlock && rp[e + 1] - start <= kStreamCap) ++e;
-        tile_row.push_back(e);
-        r = e;
-    }
-    first.resize(count);
-    last.resize(count);
-    auto tile_of = [&](int64_t row) {
-        return static_cast<int32_t>(std::upper_bound(tile_row.begin(), tile_row.end(), row) - tile_row.begin() - 1);
-    };
-    for (int q = 0; q < count; ++q) {
-        first[q] = tile_of(off[q]);
-        last[q] = tile_of(off[q + 1] - 1);
-    }
-}
-
-// Row tiles of a split layout (S blocks, absolute pointers bp[k*(rows+1) + r]) for the
-// tile-streamed split kernel: at most max_rows rows and at most `cap` entries in every
-// block; a row longer than `cap` in some block is a tile of its own.
-void build_split_tiles(int rows, int S, int cap, int max_rows, const std::vector<int32_t>& bp,
-                       std::vector<int32_t>& tile_row) {
-    tile_row.assign(1, 0);
-    auto fits = [&](int r, int e) {
-        for (int k = 0; k < S; ++k) {
-            const int32_t* b = bp.data() + static_cast<size_t>(k) * (rows + 1);
-            if (b[e] - b[r] > cap) return false;
-        }
-        return true;
-    };
-    int r = 0;
-    while (r < rows) {
-        int e = r + 1;
-        if (fits(r, e))
-            while (e < rows && e - r < max_rows && fits(r, e + 1)) ++e;
-        tile_row.push_back(e);
-        r = e;
-    }
-}
-
-// Sliced-ELL layout of a CSR (rows x cols), slices of kEllSlice rows: per slice the offset
-// of its first entry and its width (its longest row); entry q of row r sits at
-// off[s] + kEllSlice*q + (r mod kEllSlice). perm holds each entry's CSR position, or -1 for
-// padding, whose index repeats the row's first column (column 0 for an empty row). Returns
-// false when the layout would not fit 32-bit offsets.
-bool build_ell(int rows, const std::vector<int64_t>& rp, const std::vector<int32_t>& ci, std::vector<int32_t>& off,
-               std::vector<int32_t>& width, std::vector<int32_t>& idx, std::vector<int32_t>& perm) {
-    const int slices = (rows + kEllSlice - 1) / kEllSlice;
-    off.assign(slices, 0);
-    width.assign(slices, 0);
-    int64_t total = 0;
-    for (int sl = 0; sl < slices; ++sl) {
-        int64_t w = 0;
-        for (int r = sl * kEllSlice; r < std::min(rows, (sl + 1) * kEllSlice); ++r) w = std::max(w, rp[r + 1] - rp[r]);
-        if (total + kEllSlice * w > 2147483647LL) return false;
-        off[sl] = static_cast<int32_t>(total);
-        width[sl] = static_cast<int32_t>(w);
-        total += kEllSlice * w;
-    }
-    idx.assign(static_cast<size_t>(total), 0);
-    perm.assign(static_cast<size_t>(total), -1);
-    for (int sl = 0; sl < slices; ++sl)
-        for (int lane = 0; lane < kEllSlice; ++lane) {
-            const int r = sl * kEllSlice + lane;
-            const int64_t len = r < rows ? rp[r + 1] - rp[r] : 0;
-            const int32_t first = len > 0 ? ci[rp[r]] : 0;
-            for (int q = 0; q < width[sl]; ++q) {
-                const size_t e = static_cast<size_t>(off[sl]) + static_cast<size_t>(q) * kEllSlice + lane;
-                if (q < len) {
-                    idx[e] = ci[rp[r] + q];
-                    perm[e] = static_cast<int32_t>(rp[r] + q);
-                } else {
-                    idx[e] = first;
-                }
-            }
-        }
-    return true;
-}
-
-// ELL entries / nonzeros of a CSR in slices of kEllSlice rows (the padding the layout adds).
-double ell_padding(int rows, const std::vector<int64_t>& rp) {
-    int64_t total = 0;
-    for (int sl = 0; sl * kEllSlice < rows; ++sl) {
-        int64_t w = 0;
-        for (int r = sl * kEllSlice; r < std::min(rows, (sl + 1) * kEllSlice); ++r) w = std::max(w, rp[r + 1] - rp[r]);
-        total += kEllSlice * w;
-    }
-    return rp[rows] > 0 ? static_cast<double>(total) / static_cast<double>(rp[rows]) : 0.0;
-}
-
 // Column blocks of at most 512K entries of the gathered vector (2 MB in FP32, 4 MB in FP64):
 // on the 1M x 2M LP this block size was the fastest split for both precisions (S = 2, 4, 8
 // measured; FP64 at S = 4 beat S = 8 by 6%, FP32 at S = 4 beat S = 2 and S = 8).
@@ -658,24 +530,11 @@ int setup(vj_ctx* ctx, FastEngine& E) {
     for (DevBuf* b : {&E.x, &E.xbar, &E.dhat, &E.lcs, &E.ucs}) VJ_TRY(b->alloc(static_cast<size_t>(n) * T));
     for (DevBuf* b : {&E.y, &E.lrs, &E.urs}) VJ_TRY(b->alloc(static_cast<size_t>(m) * T));
     if (E.prec != VJ_PREC_FP64) VJ_TRY(E.yhat_t.alloc(static_cast<size_t>(m) * T));
-    if (!E.tiles_built) {
-        std::vector<int32_t> tr, f, l;
-        build_stream_tiles(m, P.h_rp, P.row_off, cnt, tr, f, l);
-        E.ntA = static_cast<int>(tr.size()) - 1;
-        VJ_TRY(E.tileA.alloc(tr.size() * 4));
-        VJ_TRY(E.tfA.alloc(f.size() * 4));
-        VJ_TRY(E.tlA.alloc(l.size() * 4));
-        VJ_HIP(hipMemcpy(E.tileA.p, tr.data(), tr.size() * 4, hipMemcpyHostToDevice));
-        VJ_HIP(hipMemcpy(E.tfA.p, f.data(), f.size() * 4, hipMemcpyHostToDevice));
-        VJ_HIP(hipMemcpy(E.tlA.p, l.data(), l.size() * 4, hipMemcpyHostToDevice));
-        build_stream_tiles(n, P.h_tp, P.col_off, cnt, tr, f, l);
-        E.ntAt = static_cast<int>(tr.size()) - 1;
-        VJ_TRY(E.tileAt.alloc(tr.size() * 4));
-        VJ_TRY(E.tfAt.alloc(f.size() * 4));
-        VJ_TRY(E.tlAt.alloc(l.size() * 4));
-        VJ_HIP(hipMemcpy(E.tileAt.p, tr.data(), tr.size() * 4, hipMemcpyHostToDevice));
-        VJ_HIP(hipMemcpy(E.tfAt.p, f.data(), f.size() * 4, hipMemcpyHostToDevice));
-        VJ_HIP(hipMemcpy(E.tlAt.p, l.data(), l.size() * 4, hipMemcpyHostToDevice));
+    if (!E.tiles_built) {   // row tiles of A and A^T for the tile-streamed kernels, built on the device
+        VJ_TRY(device_row_tiles(s, m, 1, P.rp.as<int>(), 0, kStreamCap, kBlock, E.tileA, &E.ntA));
+        VJ_TRY(device_instance_tiles(s, cnt, E.ntA, E.tileA.as<int>(), P.d_row_off.as<int>(), E.tfA, E.tlA));
+        VJ_TRY(device_row_tiles(s, n, 1, P.tp.as<int>(), 0, kStreamCap, kBlock, E.tileAt, &E.ntAt));
+        VJ_TRY(device_instance_tiles(s, cnt, E.ntAt, E.tileAt.as<int>(), P.d_col_off.as<int>(), E.tfAt, E.tlAt));
         E.tiles_built = true;
     }
     const int px_slots = std::max(partial_slots(n, cnt), E.ntAt + cnt + 1);
@@ -842,24 +701,16 @@ int setup(vj_ctx* ctx, FastEngine& E) {
 }
 
 // Column-split copies of the scaled matrix (single LP). The split pattern depends only on
-// the sparsity pattern and S, so it is built on the host once per upload and S; the values
+// the sparsity pattern and S, so it is built on the device once per upload and S; the values
 // are gathered on the device from the scaled FP64 matrix in the working precision.
-int setup_one_split(vj_ctx* ctx, FastEngine& E, int S, int rows, int cols, const std::vector<int64_t>& rp,
-                    const std::vector<int32_t>& ci, const double* vals_dev, int& built, DevBuf& bp, DevBuf& idx,
-                    DevBuf& perm, DevBuf& val, DevBuf& psum, DevBuf& tiles, DevBuf& tbase, int& ntiles,
-                    DevBuf& rec, std::vector<int32_t>& hbp, int& tiles_cap) {
+int setup_one_split(vj_ctx* ctx, FastEngine& E, int S, int rows, int cols, const int* rp, const int* ci,
+                    const double* vals_dev, int& built, DevBuf& bp, DevBuf& idx, DevBuf& perm, DevBuf& val,
+                    DevBuf& psum, DevBuf& tiles, DevBuf& tbase, int& ntiles, DevBuf& rec, int& tiles_cap) {
     hipStream_t s = ctx->stream;
-    const int64_t nnz = rp[rows];
+    const int64_t nnz = E.nnz;
     if (built != S) {
-        std::vector<int32_t> hidx, hperm;
-        build_split(rows, cols, rp, ci, S, hbp, hidx, hperm);
+        VJ_TRY(device_split(s, rows, cols, nnz, rp, ci, S, bp, idx, perm));
         tiles_cap = 0;
-        VJ_TRY(bp.alloc(hbp.size() * 4));
-        VJ_TRY(idx.alloc(static_cast<size_t>(nnz) * 4));
-        VJ_TRY(perm.alloc(static_cast<size_t>(nnz) * 4));
-        VJ_HIP(hipMemcpy(bp.p, hbp.data(), hbp.size() * 4, hipMemcpyHostToDevice));
-        VJ_HIP(hipMemcpy(idx.p, hidx.data(), static_cast<size_t>(nnz) * 4, hipMemcpyHostToDevice));
-        VJ_HIP(hipMemcpy(perm.p, hperm.data(), static_cast<size_t>(nnz) * 4, hipMemcpyHostToDevice));
         built = S;
     }
     // tiles depend on the LDS capacity in entries and on the tile height of the working
@@ -867,17 +718,9 @@ int setup_one_split(vj_ctx* ctx, FastEngine& E, int S, int rows, int cols, const
     const int cap = E.prec == VJ_PREC_FP64 ? split_cap<double>() : split_cap<float>();
     const int max_rows = E.prec == VJ_PREC_FP64 ? split_rows<double>() : split_rows<float>();
     if (tiles_cap != cap) {
-        std::vector<int32_t> htile;
-        build_split_tiles(rows, S, cap, max_rows, hbp, htile);
-        ntiles = static_cast<int>(htile.size()) - 1;
-        std::vector<int32_t> hbase(htile.size() * static_cast<size_t>(S));
-        for (size_t t = 0; t < htile.size(); ++t)
-            for (int k = 0; k < S; ++k)
-                hbase[t * S + k] = hbp[static_cast<size_t>(k) * (rows + 1) + static_cast<size_t>(htile[t])];
-        VJ_TRY(tiles.alloc(htile.size() * 4));
-        VJ_TRY(tbase.alloc(hbase.size() * 4));
-        VJ_HIP(hipMemcpy(tiles.p, htile.data(), htile.size() * 4, hipMemcpyHostToDevice));
-        VJ_HIP(hipMemcpy(tbase.p, hbase.data(), hbase.size() * 4, hipMemcpyHostToDevice));
+        const int64_t stride = static_cast<int64_t>(rows) + 1;
+        VJ_TRY(device_row_tiles(s, rows, S, bp.as<int>(), stride, cap, max_rows, tiles, &ntiles));
+        VJ_TRY(device_tile_bases(s, ntiles, S, tiles.as<int>(), bp.as<int>(), stride, tbase));
         tiles_cap = cap;
     }
     const size_t vsz = E.prec == VJ_PREC_FP64 ? 8 : (E.prec == VJ_PREC_FP32 ? 4 : 2);
@@ -906,13 +749,13 @@ int setup_splits(vj_ctx* ctx, FastEngine& E) {
     E.SA = prm.col_splits_A ? prm.col_splits_A : choose_splits(E.n);
     E.SAt = prm.col_splits_At ? prm.col_splits_At : choose_splits(E.m);
     if (E.SA > 1)
-        VJ_TRY(setup_one_split(ctx, E, E.SA, E.m, E.n, P.h_rp, P.h_ci, E.av_s.as<double>(), E.builtSA, E.bpA, E.idxA,
-                               E.permA, E.valA, E.psumA, E.tileSA, E.tbaseSA, E.ntSA, E.recSA_buf, E.h_bpA,
+        VJ_TRY(setup_one_split(ctx, E, E.SA, E.m, E.n, P.rp.as<int>(), P.ci.as<int>(), E.av_s.as<double>(), E.builtSA,
+                               E.bpA, E.idxA, E.permA, E.valA, E.psumA, E.tileSA, E.tbaseSA, E.ntSA, E.recSA_buf,
                                E.tcapSA));
     if (E.SAt > 1)
-        VJ_TRY(setup_one_split(ctx, E, E.SAt, E.n, E.m, P.h_tp, P.h_ti, E.tv_s.as<double>(), E.builtSAt, E.bpAt,
-                               E.idxAt, E.permAt, E.valAt, E.psumAt, E.tileSAt, E.tbaseSAt, E.ntSAt, E.recSAt_buf,
-                               E.h_bpAt, E.tcapSAt));
+        VJ_TRY(setup_one_split(ctx, E, E.SAt, E.n, E.m, P.tp.as<int>(), P.ti.as<int>(), E.tv_s.as<double>(),
+                               E.builtSAt, E.bpAt, E.idxAt, E.permAt, E.valAt, E.psumAt, E.tileSAt, E.tbaseSAt, E.ntSAt,
+                               E.recSAt_buf, E.tcapSAt));
     // split partial kernels see ~nnz/(rows*S) entries per row: few lanes per row
     E.GsA = (static_cast<double>(E.nnz) / std::max(1, E.m) / E.SA) > 3.0 ? 4 : 2;
     E.GsAt = (static_cast<double>(E.nnz) / std::max(1, E.n) / E.SAt) > 3.0 ? 4 : 2;
@@ -929,29 +772,13 @@ constexpr double kEllMaxPadding = 1.25;
 // their rounding noise moves the rel-KKT by more than this factor from check to check.)
 constexpr double kRefineDrift = 2.0;
 
-int setup_one_ell(vj_ctx* ctx, FastEngine& E, int rows, const std::vector<int64_t>& rp,
-                  const std::vector<int32_t>& ci, const double* vals_dev, bool forced, bool& built, double& pad,
-                  int64_t& len, DevBuf& off, DevBuf& width, DevBuf& idx, DevBuf& perm, DevBuf& val, DevBuf& rec) {
+int setup_one_ell(vj_ctx* ctx, FastEngine& E, int rows, const int* rp, const int* ci, const double* vals_dev,
+                  bool forced, bool& built, double& pad, int64_t& len, DevBuf& off, DevBuf& width, DevBuf& idx,
+                  DevBuf& perm, DevBuf& val, DevBuf& rec) {
     hipStream_t s = ctx->stream;
-    if (!built) {
-        pad = ell_padding(rows, rp);
-        if (pad > kEllMaxPadding && !forced) return VJ_OK;
-        std::vector<int32_t> hoff, hw, hidx, hperm;
-        if (!build_ell(rows, rp, ci, hoff, hw, hidx, hperm)) {
-            pad = 0.0;   // too large for 32-bit offsets: no ELL copy
-            return VJ_OK;
-        }
-        len = static_cast<int64_t>(hidx.size());
-        VJ_TRY(off.alloc(hoff.size() * 4));
-        VJ_TRY(width.alloc(hw.size() * 4));
-        VJ_TRY(idx.alloc(std::max<size_t>(hidx.size(), 1) * 4));
-        VJ_TRY(perm.alloc(std::max<size_t>(hperm.size(), 1) * 4));
-        VJ_HIP(hipMemcpy(off.p, hoff.data(), hoff.size() * 4, hipMemcpyHostToDevice));
-        VJ_HIP(hipMemcpy(width.p, hw.data(), hw.size() * 4, hipMemcpyHostToDevice));
-        if (len > 0) {
-            VJ_HIP(hipMemcpy(idx.p, hidx.data(), hidx.size() * 4, hipMemcpyHostToDevice));
-            VJ_HIP(hipMemcpy(perm.p, hperm.data(), hperm.size() * 4, hipMemcpyHostToDevice));
-        }
+    if (!built) {   // the layout, on the device; no copy when it pads too much (unless forced)
+        VJ_TRY(device_ell(s, rows, E.nnz, rp, ci, kEllMaxPadding, forced, &pad, &len, off, width, idx, perm));
+        if (len == 0) return VJ_OK;
         built = true;
     }
     if (len == 0) return VJ_OK;
@@ -976,9 +803,11 @@ int setup_ell(vj_ctx* ctx, FastEngine& E) {
     if (E.count != 1) return VJ_OK;
     Problem& P = ctx->prob;
     const vj_params& prm = ctx->params;
-    VJ_TRY(setup_one_ell(ctx, E, E.m, P.h_rp, P.h_ci, E.av_s.as<double>(), prm.lanes_A <= -3, E.ellBuiltA, E.ellPadA,
+    VJ_TRY(setup_one_ell(ctx, E, E.m, P.rp.as<int>(), P.ci.as<int>(), E.av_s.as<double>(), prm.lanes_A <= -3, E.ellBuiltA,
+                         E.ellPadA,
                          E.ellLenA, E.ellOffA, E.ellWA, E.ellIdxA, E.ellPermA, E.ellValA, E.ellRecA));
-    VJ_TRY(setup_one_ell(ctx, E, E.n, P.h_tp, P.h_ti, E.tv_s.as<double>(), prm.lanes_At <= -3, E.ellBuiltAt, E.ellPadAt,
+    VJ_TRY(setup_one_ell(ctx, E, E.n, P.tp.as<int>(), P.ti.as<int>(), E.tv_s.as<double>(), prm.lanes_At <= -3,
+                         E.ellBuiltAt, E.ellPadAt,
                          E.ellLenAt, E.ellOffAt, E.ellWAt, E.ellIdxAt, E.ellPermAt, E.ellValAt, E.ellRecAt));
     return VJ_OK;
 }
@@ -988,7 +817,7 @@ int launch_restart(vj_ctx* ctx, FastEngine& E, int force) {
     Problem& P = ctx->prob;
     hipStream_t s = ctx->stream;
     T* yhat_t = yhat_working<T>(E);
-    VJ_TRY(require({E.st.p, E.xfull.p, E.aty_full.p, E.c_s.p, E.lc_s.p, E.uc_s.p, E.xhat.p, E.x.p, E.dhat.p,
+    VJ_TRY(require({E.st.p, E.xfull.p, E.aty_full.p, E.c_s.p, E.lc_s.p, E.uc_s.p, E.xhat.p, E.x.p, E.xbar.p, E.dhat.p,
                     E.lcs.p, E.ucs.p, E.yfull.p, E.ax_full.p, E.lr_s.p, E.ur_s.p, E.yhat.p, yhat_t, E.y.p,
                     E.lrs.p, E.urs.p, E.multi ? P.row_inst.p : E.st.p, E.multi ? P.col_inst.p : E.st.p},
                    "restart kernels"));
@@ -996,7 +825,7 @@ int launch_restart(vj_ctx* ctx, FastEngine& E, int force) {
         hipLaunchKernelGGL((restart_cols<T, true>), dim3(grid_for(E.n, kBlock)), dim3(kBlock), 0, s, E.n,
                            E.st.as<InstState>(), P.col_inst.as<int>(), force, E.xfull.as<double>(), E.aty_full.as<double>(),
                            E.c_s.as<double>(), E.lc_s.as<double>(), E.uc_s.as<double>(), E.xhat.as<double>(), E.x.as<T>(),
-                           E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>());
+                           E.xbar.as<T>(), E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>());
         hipLaunchKernelGGL((restart_rows<T, true>), dim3(grid_for(E.m, kBlock)), dim3(kBlock), 0, s, E.m,
                            E.st.as<InstState>(), P.row_inst.as<int>(), force, E.yfull.as<double>(), E.ax_full.as<double>(),
                            E.lr_s.as<double>(), E.ur_s.as<double>(), E.yhat.as<double>(), yhat_t, E.y.as<T>(),
@@ -1005,7 +834,7 @@ int launch_restart(vj_ctx* ctx, FastEngine& E, int force) {
         hipLaunchKernelGGL((restart_cols<T, false>), dim3(grid_for(E.n, kBlock)), dim3(kBlock), 0, s, E.n,
                            E.st.as<InstState>(), P.col_inst.as<int>(), force, E.xfull.as<double>(), E.aty_full.as<double>(),
                            E.c_s.as<double>(), E.lc_s.as<double>(), E.uc_s.as<double>(), E.xhat.as<double>(), E.x.as<T>(),
-                           E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>());
+                           E.xbar.as<T>(), E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>());
         hipLaunchKernelGGL((restart_rows<T, false>), dim3(grid_for(E.m, kBlock)), dim3(kBlock), 0, s, E.m,
                            E.st.as<InstState>(), P.row_inst.as<int>(), force, E.yfull.as<double>(), E.ax_full.as<double>(),
                            E.lr_s.as<double>(), E.ur_s.as<double>(), E.yhat.as<double>(), yhat_t, E.y.as<T>(),
@@ -1113,19 +942,23 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
             else c.push_back({Sm, lanes, false, false, false});
             return c;
         }
+        // FP32 (index, value) records come before the separate arrays of the same kernel: with
+        // the tie-break margin below an earlier candidate wins near-ties, and records were
+        // faster on every gather-bound step measured (e.g. 77.7 against 82.0 us for the split
+        // A x_bar step of the 1M x 2M LP)
         if (forced <= 1 || Sm == 1) {   // unsplit candidates
-            c.push_back({1, choose_lanes(avg), true, false, false});
             if (f32) c.push_back({1, choose_lanes(avg), true, true, false});
+            c.push_back({1, choose_lanes(avg), true, false, false});
             for (int g : {4, 8, 16, 32}) c.push_back({1, g, false, false, false});
             if (ell) {
-                c.push_back({1, choose_lanes(avg), false, false, true});
                 if (f32) c.push_back({1, choose_lanes(avg), false, true, true});
+                c.push_back({1, choose_lanes(avg), false, false, true});
             }
         }
         if (Sm > 1) {
+            if (f32) c.push_back({Sm, 0, false, true, false});   // tile-streamed split
+            c.push_back({Sm, 0, false, false, false});
             for (int g : {1, 2, 4}) c.push_back({Sm, g, false, false, false});
-            c.push_back({Sm, 0, false, false, false});   // tile-streamed split
-            if (f32) c.push_back({Sm, 0, false, true, false});
         }
         return c;
     };
@@ -1158,6 +991,7 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     // A later candidate replaces the best so far only when it is faster by more than this
     // fraction: near-ties keep the earlier candidate instead of following timing noise.
     constexpr double kTuneMargin = 0.03;
+    constexpr int kTuneReps = 10;   // timed launches per candidate and step
     const size_t rounds = std::max(cA.size(), cAt.size());
     double bestA = 1e30, bestAt = 1e30;
     StepCfg wA = cA[0], wAt = cAt[0];
@@ -1167,7 +1001,7 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
         apply(a, E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA);
         apply(b, E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt);
         double up = 0, ud = 0;
-        VJ_TRY((time_kernels<V, T>(ctx, E, av, tv, 3, &up, &ud)));
+        VJ_TRY((time_kernels<V, T>(ctx, E, av, tv, kTuneReps, &up, &ud)));
         if (prm.verbose >= 2)
             std::fprintf(stderr, "[autotune] A: S=%d G=%d stream=%d rec=%d ell=%d %.1f us | A^T: S=%d G=%d stream=%d rec=%d ell=%d %.1f us\n",
                          a.S, a.G, a.stream ? 1 : 0, a.rec ? 1 : 0, a.ell ? 1 : 0, ud, b.S, b.G, b.stream ? 1 : 0,
@@ -1222,8 +1056,11 @@ int run_typed(vj_ctx* ctx, FastEngine& E, vj_result* out, clk::time_point t_star
         av = reinterpret_cast<const V*>(E.av_lo.p);
         tv = reinterpret_cast<const V*>(E.tv_lo.p);
     }
+    const auto t_layout = clk::now();
     VJ_TRY(setup_splits(ctx, E));
     VJ_TRY(setup_ell(ctx, E));
+    VJ_HIP(hipStreamSynchronize(s));
+    const double layout_time = secs(t_layout, clk::now());
     VJ_TRY(launch_restart<T>(ctx, E, 1));  // initial centring
     VJ_TRY((autotune<V, T>(ctx, E, av, tv)));
     // after autotuning the scratch runs left the real iterates untouched; recentre anyway
@@ -1357,6 +1194,7 @@ int run_typed(vj_ctx* ctx, FastEngine& E, vj_result* out, clk::time_point t_star
         r.col_splits_At = E.SAt;
         r.check_interval = E.K;
         r.checks = z.checks;
+        r.layout_time = layout_time;
         if (z.nan_flag) r.status = VJ_STATUS_NUMERICAL_ERROR;
         else if (z.converged) r.status = VJ_STATUS_OPTIMAL;
         else if (timed_out) r.status = VJ_STATUS_TIME_LIMIT;
@@ -1507,6 +1345,20 @@ int fast_benchmark_blocks(vj_ctx* ctx, int warmup, int steps, double* ms_per_blo
     return VJ_OK;
 }
 
+int fast_scaling(vj_ctx* ctx, double* row_scale, double* col_scale) {
+    if (!ctx->fast || !ctx->fast->Rsc.p || !ctx->fast->Csc.p) {
+        set_error("no restarted-mode solve has run on this context");
+        return VJ_ERR_NOT_READY;
+    }
+    FastEngine& E = *ctx->fast;
+    if (row_scale)
+        VJ_HIP(hipMemcpyAsync(row_scale, E.Rsc.p, static_cast<size_t>(E.m) * 8, hipMemcpyDeviceToHost, ctx->stream));
+    if (col_scale)
+        VJ_HIP(hipMemcpyAsync(col_scale, E.Csc.p, static_cast<size_t>(E.n) * 8, hipMemcpyDeviceToHost, ctx->stream));
+    VJ_HIP(hipStreamSynchronize(ctx->stream));
+    return VJ_OK;
+}
+
 int fast_benchmark(vj_ctx* ctx, int reps, vj_kernel_stat* out, int cap, int* written) {
     *written = 0;
     if (!ctx->fast) {
@@ -1540,8 +1392,9 @@ int fast_benchmark(vj_ctx* ctx, int reps, vj_kernel_stat* out, int cap, int* wri
     const double sx = E.prec == VJ_PREC_FP64 ? 8 : 4;
     const double m = E.m, n = E.n, nnz = static_cast<double>(E.nnz);
     // algorithmic bytes: matrix once, pointers once, gathered vector once, per-row streams
-    // primal_step rows of A^T: reads y (gather, m), x, dhat, lcs, ucs (4n); writes x, xbar (2n)
-    const double b_primal = nnz * (sv + 4) + (n + 1) * 4 + m * sx + 6 * n * sx;
+    // primal_step rows of A^T: reads y (gather, m), x (or xbar), dhat, lcs, ucs (4n); writes
+    // xbar, and x unless the reflection is 1 (x is then recovered from xbar)
+    const double b_primal = nnz * (sv + 4) + (n + 1) * 4 + m * sx + (E.rho == 1.0 ? 5 : 6) * n * sx;
     // dual_step rows of A: reads xbar (gather, n), y, yhat, lrs, urs (4m); writes y (m)
     const double b_dual = nnz * (sv + 4) + (m + 1) * 4 + n * sx + 5 * m * sx;
     // names carry the kernel variant the autotuner chose, as the profiler lists it

@@ -160,10 +160,27 @@ template <class T>
 struct PrimalOps {
     T xk, dh, lo, hi;
 };
+// The x iterate at the start of iteration `it`. With reflection rho = 1 the Halpern update
+// is x+ = lam (2 xt - x) = lam xbar, exactly (2 xt is exact and both round 2 xt - x once), so
+// the step writes only xbar and the next iteration recovers x as lam_prev * xbar, bit for
+// bit; lam_prev is the previous iteration's lam (the same expression of s.k + it - 1), 0 at
+// the start of an epoch, where the restart kernels have zeroed x and xbar. This saves one
+// vector write per primal step. Other rho keep x in its own vector.
 template <class T>
-__device__ __forceinline__ PrimalOps<T> primal_ops(int j, const T* __restrict__ x, const T* __restrict__ dhat,
-                                                   const T* __restrict__ lcs, const T* __restrict__ ucs) {
-    return PrimalOps<T>{x[j], dhat[j], lcs[j], ucs[j]};
+__device__ __forceinline__ T x_start(int j, const T* __restrict__ x, const T* __restrict__ xbar, const InstState& s,
+                                     int it, double rho) {
+    if (rho != 1.0) return x[j];
+    const double kp = static_cast<double>(s.k + it) - 1.0;
+    const T lam_prev = static_cast<T>((kp + 1.0) / (kp + 2.0));
+    return lam_prev * xbar[j];
+}
+
+template <class T>
+__device__ __forceinline__ PrimalOps<T> primal_ops(int j, const T* __restrict__ x, const T* __restrict__ xbar,
+                                                   const T* __restrict__ dhat, const T* __restrict__ lcs,
+                                                   const T* __restrict__ ucs, const InstState& s, int it,
+                                                   double rho) {
+    return PrimalOps<T>{x_start<T>(j, x, xbar, s, it, rho), dhat[j], lcs[j], ucs[j]};
 }
 
 template <class T>
@@ -179,7 +196,7 @@ __device__ __forceinline__ void primal_update(T acc, const PrimalOps<T>& o, int 
     xt = xt < o.lo ? o.lo : xt;
     xt = xt > o.hi ? o.hi : xt;
     xbar[j] = static_cast<T>(2) * xt - xk;
-    x[j] = lam * ((static_cast<T>(1) + static_cast<T>(rho)) * xt - static_cast<T>(rho) * xk);
+    if (rho != 1.0) x[j] = lam * ((static_cast<T>(1) + static_cast<T>(rho)) * xt - static_cast<T>(rho) * xk);
     const double d = static_cast<double>(xt) - static_cast<double>(xk);
     if (flags & 1) c[0] = d * d;
     if (flags & 2) {
@@ -195,7 +212,8 @@ __device__ __forceinline__ void primal_epilogue(T acc, int j, const InstState& s
                                                 const T* __restrict__ ucs, T* __restrict__ x, T* __restrict__ xbar,
                                                 const double* __restrict__ xhat, double* __restrict__ xfull,
                                                 double (&c)[3]) {
-    primal_update<T>(acc, primal_ops<T>(j, x, dhat, lcs, ucs), j, s, it, flags, rho, x, xbar, xhat, xfull, c);
+    primal_update<T>(acc, primal_ops<T>(j, x, xbar, dhat, lcs, ucs, s, it, rho), j, s, it, flags, rho, x, xbar, xhat,
+                     xfull, c);
 }
 
 // Dual epilogue for row i given w = (A x_bar)_i: prox of the row-bound support function
@@ -457,7 +475,7 @@ __global__ __launch_bounds__(kBlock) void stream_primal_step(
         PrimalOps<T> o{};
         int q0 = 0, q1 = 0;
         if (mine) {
-            o = primal_ops<T>(j, x, dhat, lcs, ucs);
+            o = primal_ops<T>(j, x, xbar, dhat, lcs, ucs, st[my], it, rho);
             q0 = tp[j] - s0;
             q1 = tp[j + 1] - s0;
         }
@@ -686,7 +704,6 @@ __global__ void gather_split_vals(int64_t nnz, const int* __restrict__ perm, con
 // off[s] + 64 q + r: one coalesced load per q, no row pointers, no LDS. The row sum runs in
 // index order. The autotuner offers it when padding adds at most a quarter to the entries,
 // as when every row has the same length.
-constexpr int kEllSlice = 64;
 
 template <class Ent, class T>
 __device__ __forceinline__ T ell_row_sum(int r, const int* __restrict__ off, const int* __restrict__ width,
@@ -727,7 +744,7 @@ __global__ __launch_bounds__(kBlock) void ell_primal_step(
     const int j = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
     double c[3] = {0.0, 0.0, 0.0};
     if (j < n) {
-        const PrimalOps<T> o = primal_ops<T>(j, x, dhat, lcs, ucs);   // issued before the gathers
+        const PrimalOps<T> o = primal_ops<T>(j, x, xbar, dhat, lcs, ucs, st[0], it, rho);   // before the gathers
         const T acc = ell_row_sum<Ent, T>(j, off, width, ent, y);
         primal_update<T>(acc, o, j, st[0], it, flags, rho, x, xbar, xhat, xfull, c);
     }
@@ -1000,8 +1017,8 @@ __global__ void restart_cols(int n, const InstState* __restrict__ st, const int*
                              int force, const double* __restrict__ xfull,
                              const double* __restrict__ aty_full, const double* __restrict__ c_s,
                              const double* __restrict__ lc_s, const double* __restrict__ uc_s,
-                             double* __restrict__ xhat, T* __restrict__ x, T* __restrict__ dhat,
-                             T* __restrict__ lcs, T* __restrict__ ucs) {
+                             double* __restrict__ xhat, T* __restrict__ x, T* __restrict__ xbar,
+                             T* __restrict__ dhat, T* __restrict__ lcs, T* __restrict__ ucs) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     const int q = MULTI ? col_inst[j] : 0;
@@ -1009,6 +1026,7 @@ __global__ void restart_cols(int n, const InstState* __restrict__ st, const int*
     const double xf = xfull[j];
     xhat[j] = xf;
     x[j] = static_cast<T>(0);
+    xbar[j] = static_cast<T>(0);
     dhat[j] = static_cast<T>(c_s[j] - aty_full[j]);
     lcs[j] = static_cast<T>(lc_s[j] - xf);
     ucs[j] = static_cast<T>(uc_s[j] - xf);

@@ -61,23 +61,20 @@ std::string kernel_mode(const vj_params& p, const vj_result& r) {
 }
 
 struct HostLp {
-    la::SparseMatrixCSR csr;
-    std::vector<int32_t> csr_ci, csc_ri;
-    std::vector<int64_t> csr_rp;
+    std::vector<int32_t> csc_ri;
 };
 
+// The Model's CSC only: the engine builds the CSR of A on the device.
 vj_lp to_vj(const Model& M, HostLp& h) {
-    h.csr = M.A.to_csr();
-    h.csr_ci.assign(h.csr.col_idx.begin(), h.csr.col_idx.end());
     h.csc_ri.assign(M.A.row_idx.begin(), M.A.row_idx.end());
     vj_lp lp;
     std::memset(&lp, 0, sizeof(lp));
     lp.m = M.num_rows;
     lp.n = M.num_cols;
     lp.nnz = M.A.nnz();
-    lp.csr_row_ptr = h.csr.row_ptr.data();
-    lp.csr_col_idx = h.csr_ci.data();
-    lp.csr_vals = h.csr.values.data();
+    lp.csr_row_ptr = nullptr;
+    lp.csr_col_idx = nullptr;
+    lp.csr_vals = nullptr;
     lp.csc_col_ptr = M.A.col_ptr.data();
     lp.csc_row_idx = h.csc_ri.data();
     lp.csc_vals = M.A.values.data();
@@ -244,9 +241,9 @@ std::vector<Solution> solve_pdhg_gpu_batch(const std::vector<Model>& models, con
     const auto t0 = clk::now();
     const int cnt = static_cast<int>(models.size());
     std::vector<int32_t> ms(cnt), ns(cnt);
-    std::vector<int64_t> zs(cnt), rp, tp;
-    std::vector<int32_t> ci, ti;
-    std::vector<double> av, tv, c, lc, uc, lr, ur, off(cnt);
+    std::vector<int64_t> zs(cnt), tp;
+    std::vector<int32_t> ti;
+    std::vector<double> tv, c, lc, uc, lr, ur, off(cnt);
     std::vector<int32_t> mx(cnt);
     for (int q = 0; q < cnt; ++q) {
         const Model& M = models[q];
@@ -255,10 +252,6 @@ std::vector<Solution> solve_pdhg_gpu_batch(const std::vector<Model>& models, con
         ms[q] = M.num_rows;
         ns[q] = M.num_cols;
         zs[q] = M.A.nnz();
-        const la::SparseMatrixCSR csr = M.A.to_csr();
-        rp.insert(rp.end(), csr.row_ptr.begin(), csr.row_ptr.end());
-        ci.insert(ci.end(), csr.col_idx.begin(), csr.col_idx.end());
-        av.insert(av.end(), csr.values.begin(), csr.values.end());
         tp.insert(tp.end(), M.A.col_ptr.begin(), M.A.col_ptr.end());
         ti.insert(ti.end(), M.A.row_idx.begin(), M.A.row_idx.end());
         tv.insert(tv.end(), M.A.values.begin(), M.A.values.end());
@@ -270,7 +263,8 @@ std::vector<Solution> solve_pdhg_gpu_batch(const std::vector<Model>& models, con
         lr.insert(lr.end(), M.row_lower.begin(), M.row_lower.end());
         ur.insert(ur.end(), M.row_upper.begin(), M.row_upper.end());
     }
-    vj_batch b{cnt, ms.data(), ns.data(), zs.data(), rp.data(), ci.data(), av.data(), tp.data(), ti.data(), tv.data(),
+    // CSCs only: the engine builds the CSRs on the device
+    vj_batch b{cnt, ms.data(), ns.data(), zs.data(), nullptr, nullptr, nullptr, tp.data(), ti.data(), tv.data(),
                c.data(), lc.data(), uc.data(), lr.data(), ur.data(), mx.data(), off.data()};
     Ctx ctx(0);
     check(vj_batch_upload(ctx.c, &b), "vj_batch_upload");

@@ -65,7 +65,7 @@ class Result(ctypes.Structure):
                 ("download_time", _c_dbl), ("time_to_target", _c_dbl),
                 ("host_device_transfers", _c_i64), ("refinements", _c_int), ("lanes_A", _c_int),
                 ("lanes_At", _c_int), ("check_interval", _c_int), ("checks", _c_i64),
-                ("col_splits_A", _c_int), ("col_splits_At", _c_int)]
+                ("col_splits_A", _c_int), ("col_splits_At", _c_int), ("layout_time", _c_dbl)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
@@ -96,7 +96,8 @@ class KernelStat(ctypes.Structure):
 EXPORTED = [
     "vj_last_error", "vj_device_probe", "vj_params_default", "vj_ctx_create", "vj_ctx_destroy",
     "vj_problem_upload", "vj_params_set", "vj_solve", "vj_download", "vj_reference_iterates",
-    "vj_audit_point", "vj_kernel_stats", "vj_benchmark_kernels", "vj_benchmark_blocks", "vj_batch_upload",
+    "vj_audit_point", "vj_kernel_stats", "vj_benchmark_kernels", "vj_benchmark_blocks", "vj_scaling_factors",
+    "vj_batch_upload",
     "vj_batch_solve", "vj_batch_download", "vj_op_spmv", "vj_op_dual_update",
     "vj_op_primal_update", "vj_op_diagnostics", "vj_op_diagnostics_static", "vj_gen_random_lp", "vj_gen_staircase_lp",
     "vj_gen_uniform_ints",
@@ -120,6 +121,7 @@ _lib.vj_batch_download.argtypes = [_vp, _vp, _vp]
 _lib.vj_benchmark_kernels.argtypes = [_vp, _c_int, _vp, _c_int]
 _lib.vj_kernel_stats.argtypes = [_vp, _vp, _c_int]
 _lib.vj_benchmark_blocks.argtypes = [_vp, _c_int, _c_int, _vp, _vp]
+_lib.vj_scaling_factors.argtypes = [_vp, _vp, _vp]
 
 
 def lib():
@@ -285,8 +287,10 @@ class Engine:
         self.close()
 
     # -- single LP --
-    def upload(self, lp: LP):
-        rp, ci, av = lp.csr()
+    def upload(self, lp: LP, host_csr=False):
+        """Upload lp (its CSC). host_csr=True also passes the CSR built on the host; by default
+        the engine builds it on the device."""
+        rp, ci, av = lp.csr() if host_csr else (None, None, None)
         self._keep = (rp, ci, av)
         s = VjLp(lp.m, lp.n, lp.nnz, _p(rp), _p(ci), _p(av), _p(lp.col_ptr), _p(lp.row_idx), _p(lp.vals),
                  _p(lp.c), _p(lp.col_lower), _p(lp.col_upper), _p(lp.row_lower), _p(lp.row_upper),
@@ -340,17 +344,30 @@ class Engine:
                "vj_benchmark_blocks")
         return ms.value, k.value
 
+    def scaling_factors(self, m, n):
+        """(row_scale[m], col_scale[n]): the device equilibration factors of the last restarted
+        solve (vj_scaling_factors)."""
+        r, c = np.zeros(m), np.zeros(n)
+        _check(_lib.vj_scaling_factors(self._ctx, _p(r), _p(c)), "vj_scaling_factors")
+        return r, c
+
     # -- batches --
-    def upload_batch(self, lps):
+    def upload_batch(self, lps, host_csr=False):
+        """Upload the LPs as one block-diagonal batch (their CSCs; host_csr=True also passes
+        CSRs built on the host, else the engine builds them on the device)."""
         cnt = len(lps)
         ms = np.array([lp.m for lp in lps], np.int32)
         ns = np.array([lp.n for lp in lps], np.int32)
         zs = np.array([lp.nnz for lp in lps], np.int64)
-        csr = [lp.csr() for lp in lps]
         cat = lambda arrs, dt: np.ascontiguousarray(np.concatenate(arrs).astype(dt, copy=False))
+        if host_csr:
+            csr = [lp.csr() for lp in lps]
+            host = dict(rp=cat([c[0] for c in csr], np.int64), ci=cat([c[1] for c in csr], np.int32),
+                        av=cat([c[2] for c in csr], np.float64))
+        else:
+            host = dict(rp=None, ci=None, av=None)
         parts = dict(
-            rp=cat([c[0] for c in csr], np.int64), ci=cat([c[1] for c in csr], np.int32),
-            av=cat([c[2] for c in csr], np.float64), tp=cat([lp.col_ptr for lp in lps], np.int64),
+            **host, tp=cat([lp.col_ptr for lp in lps], np.int64),
             ti=cat([lp.row_idx for lp in lps], np.int32), tv=cat([lp.vals for lp in lps], np.float64),
             c=cat([lp.c for lp in lps], np.float64),
             mx=np.array([int(lp.maximize) for lp in lps], np.int32),
