@@ -40,10 +40,17 @@ def log(*a):
     print(f"[bench {time.perf_counter() - T0:6.1f}s]", *a, file=sys.stderr, flush=True)
 
 
-def solve_timed(vp, eng, prec, tol, max_iters, time_limit):
+USE_GRAPH = 1
+
+
+def solve_timed(vp, eng, prec, tol, max_iters, time_limit, force=None):
     # verbose=1: the engine prints a progress line to stderr every 10 s of a long solve
+    kw = dict(use_graph=USE_GRAPH)
+    if force:
+        la, lat, sa, sat = (int(v) for v in force.split(","))
+        kw.update(lanes_A=la, lanes_At=lat, col_splits_A=sa, col_splits_At=sat)
     p = vp.default_params(vp.MODE_RESTARTED, precision=prec, tolerance=tol, max_iterations=max_iters,
-                          time_limit=time_limit, verbose=1)
+                          time_limit=time_limit, verbose=1, **kw)
     t0 = time.perf_counter()
     r = eng.solve(p)
     wall = time.perf_counter() - t0
@@ -71,6 +78,12 @@ def main():
                     help="staircase instances solved one by one (all 128 by default)")
     ap.add_argument("--cpu-iters", type=int, default=300)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp64"])
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch each check block's kernels directly instead of replaying a hipGraph (profiled "
+                         "runs: graph replays under rocprofv3 kernel tracing have ended in a host fault)")
+    ap.add_argument("--force-kernels", default=None, metavar="LANES_A,LANES_AT,SPLITS_A,SPLITS_AT",
+                    help="fix the headline solve's kernel variants (vj_params lanes_A, lanes_At, col_splits_A, "
+                         "col_splits_At) instead of autotuning them, e.g. to profile the variants a timed run chose")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design note "
@@ -78,6 +91,8 @@ def main():
         sys.exit(2)
     if args.steps < 1 or args.warmup < 0:
         raise SystemExit("--steps must be >= 1 and --warmup >= 0")
+    global USE_GRAPH
+    USE_GRAPH = 0 if args.no_graph else 1
 
     import vajra_pdhg as vp
 
@@ -117,7 +132,7 @@ def main():
                              check_interval=r64.check_interval, kernels=k64)
 
     # ---- headline: FP32 iterates + FP64 refinement, one complete solve --------------------
-    res, wall = solve_timed(vp, eng, args.precision, args.tol, args.max_iters, args.time_limit)
+    res, wall = solve_timed(vp, eng, args.precision, args.tol, args.max_iters, args.time_limit, args.force_kernels)
     log(f"{args.precision}: {vp.STATUS[res.status]} iters={res.iterations} restarts={res.restart_count} "
         f"rel_kkt={res.rel_kkt:.3e} setup={res.setup_time:.3f}s layout={res.layout_time:.3f}s "
         f"iter={res.iteration_time:.3f}s wall={wall:.3f}s")
@@ -242,7 +257,7 @@ def bench_batch(vp, kind, args, budget_left):
     eng = vp.Engine(0)
     eng.upload_batch(lps)
     p = vp.default_params(vp.MODE_RESTARTED, precision=args.precision, tolerance=args.tol,
-                          max_iterations=args.max_iters, time_limit=args.time_limit, verbose=1)
+                          max_iterations=args.max_iters, time_limit=args.time_limit, verbose=1, use_graph=USE_GRAPH)
     t0 = time.perf_counter()
     res = eng.solve_batch(p)
     t_batch = time.perf_counter() - t0
@@ -265,7 +280,7 @@ def bench_batch(vp, kind, args, budget_left):
         e.upload(lp)
         t0 = time.perf_counter()
         r = e.solve(vp.default_params(vp.MODE_RESTARTED, precision="fp64", tolerance=args.tol,
-                                      max_iterations=args.max_iters, time_limit=args.time_limit))
+                                      max_iterations=args.max_iters, time_limit=args.time_limit, use_graph=USE_GRAPH))
         seq_t += time.perf_counter() - t0
         seq_ok += r.status == 1
         seq_it += r.iterations
