@@ -89,7 +89,7 @@ def solve(lp, S=None, K=64, tol=1e-6, max_iters=200000, rho=1.0, artificial=0.5,
             x = lam * ((1 + rho) * xt - rho * x)
             w = A @ xbar
             full = yh + y
-            yt = np.where(sig * (w - lrs) < full, y + sig * (lrs - w),
+            yt = np.where(sig * (w - lrs) <= full, y + sig * (lrs - w),
                           np.where(sig * (w - urs) > full, y + sig * (urs - w), -yh))
             if it == 0:
                 yq0 = np.dot(yt - y, yt - y)

@@ -69,9 +69,11 @@ int upload_common(vj_ctx* ctx) {
         lc[j] = fix_lo(P.h_lc[j]);
         uc[j] = fix_hi(P.h_uc[j]);
     }
+    P.all_eq = P.m > 0;
     for (int i = 0; i < P.m; ++i) {
         lr[i] = fix_lo(P.h_lr[i]);
         ur[i] = fix_hi(P.h_ur[i]);
+        P.all_eq = P.all_eq && lr[i] == ur[i] && std::isfinite(lr[i]);
     }
     // original-space norms for the rel-KKT denominators (finite-bound vector and c)
     P.norm_b.assign(P.count, 0.0);

@@ -37,6 +37,7 @@ struct Problem {
     DevBuf row_inst, col_inst;             // instance id per row / column (count > 1)
     DevBuf d_row_off, d_col_off;           // int32 count+1
     std::vector<double> norm_b, norm_c;    // original-space norms per instance
+    bool all_eq = false;                   // every row of every instance has l = u (finite)
 
     // Back to the empty state, releasing device memory (DevBuf is not copyable).
     void reset() {
@@ -46,6 +47,7 @@ struct Problem {
         for (auto* v : {&m_i, &n_i, &maximize_i}) v->clear();
         for (auto* v : {&nnz_i, &row_off, &col_off, &h_rp, &h_tp}) v->clear();
         for (auto* v : {&offset_i, &h_av, &h_tv, &h_c, &h_lc, &h_uc, &h_lr, &h_ur, &norm_b, &norm_c}) v->clear();
+        all_eq = false;
         h_ci.clear();
         h_ti.clear();
         for (DevBuf* b : {&rp, &ci, &av, &tp, &ti, &tv, &c_can, &lc, &uc, &lr, &ur, &row_inst, &col_inst,

@@ -197,9 +197,10 @@ struct FastEngine {
     // of A (gathers y); S = 1 means the fused single-pass kernels are used
     int SA = 1, SAt = 1, GsA = 2, GsAt = 2;
     DevBuf bpA, idxA, valA, psumA, bpAt, idxAt, valAt, psumAt, permA, permAt;
-    // row tiles of the split layouts for split_stream_partials (Gs = 0 selects that kernel) and
-    // each tile's first entry in every block, tbase[t * S + k] (t = 0..tiles)
-    DevBuf tileSA, tileSAt, tbaseSA, tbaseSAt;
+    // row tiles of the split layouts for split_stream_partials (Gs = 0 selects that kernel),
+    // each tile's first entry in every block, tbase[t * S + k] (t = 0..tiles), and each row's
+    // first entry in every block relative to its tile's (toff*, 16-bit)
+    DevBuf tileSA, tileSAt, tbaseSA, tbaseSAt, toffSA, toffSAt;
     int ntSA = 0, ntSAt = 0, tcapSA = 0, tcapSAt = 0;   // tile count; LDS entries the tiles fit
     // FP32: interleaved (index, value) records of A, A^T and of the split layouts, read by
     // the tile-streamed kernels when the autotuner picks them (recA / recAt)
@@ -267,12 +268,12 @@ int launch_primal(vj_ctx* ctx, FastEngine& E, const V* tv, T* x, T* xbar, const 
     if (E.SAt > 1) {
         VJ_TRY(require({E.bpAt.p, E.idxAt.p, E.valAt.p, E.psumAt.p}, "split primal_step"));
         if (E.GsAt == 0) {
-            VJ_TRY(require({E.tileSAt.p, E.tbaseSAt.p}, "split_stream_partials"));
+            VJ_TRY(require({E.tileSAt.p, E.tbaseSAt.p, E.toffSAt.p}, "split_stream_partials"));
             VJ_TRY(with_entries<V>(E.recAt, E.recSAt_buf, E.idxAt.as<int>(), E.valAt.as<V>(), "split records",
                                    [&](auto ent) {
                 hipLaunchKernelGGL((split_stream_partials<decltype(ent), T>), dim3(E.SAt * E.ntSAt), dim3(kBlock), 0,
-                                   ctx->stream, E.n, E.SAt, E.tileSAt.as<int>(), E.tbaseSAt.as<int>(), E.bpAt.as<int>(), ent, y,
-                                   E.psumAt.as<T>(), E.st.as<InstState>());
+                                   ctx->stream, E.n, E.SAt, E.tileSAt.as<int>(), E.tbaseSAt.as<int>(),
+                                   E.toffSAt.as<uint16_t>(), ent, y, E.psumAt.as<T>(), E.st.as<InstState>());
             }));
         } else {
             VJ_DISPATCH_G(E.GsAt, hipLaunchKernelGGL((split_partials<V, T, GG>), dim3(E.SAt * tiles_for(E.n, GG)),
@@ -351,17 +352,21 @@ int launch_dual(vj_ctx* ctx, FastEngine& E, const V* av, const T* xbar, T* y, in
     Problem& P = ctx->prob;
     const int tiles = tiles_for(E.m, E.GA);
     const T* yht = yhat_working<T>(E);
+    // all rows equalities: the shifted bounds are equal, and passing one vector for both
+    // selects the kernels' equality-row update (no upper bound, no centre dual read)
+    const T* lrs = E.lrs.as<T>();
+    const T* urs = P.all_eq ? lrs : E.urs.as<T>();
     VJ_TRY(require({P.rp.p, P.ci.p, av, xbar, yht, E.lrs.p, E.urs.p, y, E.yhat.p, E.yfull.p, E.st.p, E.part_y.p,
                     E.multi ? P.row_inst.p : E.st.p}, "dual_step"));
     if (E.SA > 1) {
         VJ_TRY(require({E.bpA.p, E.idxA.p, E.valA.p, E.psumA.p}, "split dual_step"));
         if (E.GsA == 0) {
-            VJ_TRY(require({E.tileSA.p, E.tbaseSA.p}, "split_stream_partials"));
+            VJ_TRY(require({E.tileSA.p, E.tbaseSA.p, E.toffSA.p}, "split_stream_partials"));
             VJ_TRY(with_entries<V>(E.recA, E.recSA_buf, E.idxA.as<int>(), E.valA.as<V>(), "split records",
                                    [&](auto ent) {
                 hipLaunchKernelGGL((split_stream_partials<decltype(ent), T>), dim3(E.SA * E.ntSA), dim3(kBlock), 0,
-                                   ctx->stream, E.m, E.SA, E.tileSA.as<int>(), E.tbaseSA.as<int>(), E.bpA.as<int>(), ent, xbar,
-                                   E.psumA.as<T>(), E.st.as<InstState>());
+                                   ctx->stream, E.m, E.SA, E.tileSA.as<int>(), E.tbaseSA.as<int>(),
+                                   E.toffSA.as<uint16_t>(), ent, xbar, E.psumA.as<T>(), E.st.as<InstState>());
             }));
         } else {
             VJ_DISPATCH_G(E.GsA, hipLaunchKernelGGL((split_partials<V, T, GG>), dim3(E.SA * tiles_for(E.m, GG)),
@@ -369,7 +374,7 @@ int launch_dual(vj_ctx* ctx, FastEngine& E, const V* av, const T* xbar, T* y, in
                 E.psumA.as<T>(), E.st.as<InstState>()));
         }
         hipLaunchKernelGGL((dual_finish<T>), dim3(grid_for(E.m, kBlock)), dim3(kBlock), 0, ctx->stream, E.m, E.SA,
-                           E.psumA.as<T>(), yht, E.lrs.as<T>(), E.urs.as<T>(), y, E.yhat.as<double>(),
+                           E.psumA.as<T>(), yht, lrs, urs, y, E.yhat.as<double>(),
                            E.yfull.as<double>(), E.st.as<InstState>(), E.part_y.as<double>(), it, flags, E.rho);
         VJ_LAUNCHED();
         return VJ_OK;
@@ -380,7 +385,7 @@ int launch_dual(vj_ctx* ctx, FastEngine& E, const V* av, const T* xbar, T* y, in
                                [&](auto ent) {
             hipLaunchKernelGGL((ell_dual_step<decltype(ent), T>), dim3(grid_for(E.m, kBlock)), dim3(kBlock), 0,
                                ctx->stream, E.m, E.ellOffA.as<int>(), E.ellWA.as<int>(), ent, xbar, yht,
-                               E.lrs.as<T>(), E.urs.as<T>(), y, E.yhat.as<double>(), E.yfull.as<double>(),
+                               lrs, urs, y, E.yhat.as<double>(), E.yfull.as<double>(),
                                E.st.as<InstState>(), E.part_y.as<double>(), it, flags, E.rho);
         }));
         VJ_LAUNCHED();
@@ -392,12 +397,12 @@ int launch_dual(vj_ctx* ctx, FastEngine& E, const V* av, const T* xbar, T* y, in
             using Ent = decltype(ent);
             if (E.multi)
                 hipLaunchKernelGGL((stream_dual_step<Ent, T, true>), dim3(E.ntA), dim3(kBlock), 0, ctx->stream, E.m,
-                                   E.tileA.as<int>(), P.rp.as<int>(), ent, xbar, yht, E.lrs.as<T>(), E.urs.as<T>(), y,
+                                   E.tileA.as<int>(), P.rp.as<int>(), ent, xbar, yht, lrs, urs, y,
                                    E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(),
                                    P.row_inst.as<int>(), E.part_y.as<double>(), it, flags, E.rho);
             else
                 hipLaunchKernelGGL((stream_dual_step<Ent, T, false>), dim3(E.ntA), dim3(kBlock), 0, ctx->stream, E.m,
-                                   E.tileA.as<int>(), P.rp.as<int>(), ent, xbar, yht, E.lrs.as<T>(), E.urs.as<T>(), y,
+                                   E.tileA.as<int>(), P.rp.as<int>(), ent, xbar, yht, lrs, urs, y,
                                    E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(),
                                    P.row_inst.as<int>(), E.part_y.as<double>(), it, flags, E.rho);
         }));
@@ -406,12 +411,12 @@ int launch_dual(vj_ctx* ctx, FastEngine& E, const V* av, const T* xbar, T* y, in
     }
     if (E.multi) {
         VJ_DISPATCH_G(E.GA, hipLaunchKernelGGL((dual_step<V, T, GG, true>), dim3(tiles), dim3(kBlock), 0, ctx->stream,
-            E.m, P.rp.as<int>(), P.ci.as<int>(), av, xbar, yht, E.lrs.as<T>(), E.urs.as<T>(), y,
+            E.m, P.rp.as<int>(), P.ci.as<int>(), av, xbar, yht, lrs, urs, y,
             E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(), P.row_inst.as<int>(),
             E.part_y.as<double>(), it, flags, E.rho));
     } else {
         VJ_DISPATCH_G(E.GA, hipLaunchKernelGGL((dual_step<V, T, GG, false>), dim3(tiles), dim3(kBlock), 0, ctx->stream,
-            E.m, P.rp.as<int>(), P.ci.as<int>(), av, xbar, yht, E.lrs.as<T>(), E.urs.as<T>(), y,
+            E.m, P.rp.as<int>(), P.ci.as<int>(), av, xbar, yht, lrs, urs, y,
             E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(), P.row_inst.as<int>(),
             E.part_y.as<double>(), it, flags, E.rho));
     }
@@ -705,7 +710,8 @@ int setup(vj_ctx* ctx, FastEngine& E) {
 // are gathered on the device from the scaled FP64 matrix in the working precision.
 int setup_one_split(vj_ctx* ctx, FastEngine& E, int S, int rows, int cols, const int* rp, const int* ci,
                     const double* vals_dev, int& built, DevBuf& bp, DevBuf& idx, DevBuf& perm, DevBuf& val,
-                    DevBuf& psum, DevBuf& tiles, DevBuf& tbase, int& ntiles, DevBuf& rec, int& tiles_cap) {
+                    DevBuf& psum, DevBuf& tiles, DevBuf& tbase, int& ntiles, DevBuf& rec, int& tiles_cap,
+                    DevBuf& toff) {
     hipStream_t s = ctx->stream;
     const int64_t nnz = E.nnz;
     if (built != S) {
@@ -721,6 +727,7 @@ int setup_one_split(vj_ctx* ctx, FastEngine& E, int S, int rows, int cols, const
         const int64_t stride = static_cast<int64_t>(rows) + 1;
         VJ_TRY(device_row_tiles(s, rows, S, bp.as<int>(), stride, cap, max_rows, tiles, &ntiles));
         VJ_TRY(device_tile_bases(s, ntiles, S, tiles.as<int>(), bp.as<int>(), stride, tbase));
+        VJ_TRY(device_tile_offsets(s, rows, S, ntiles, tiles.as<int>(), bp.as<int>(), stride, toff));
         tiles_cap = cap;
     }
     const size_t vsz = E.prec == VJ_PREC_FP64 ? 8 : (E.prec == VJ_PREC_FP32 ? 4 : 2);
@@ -751,18 +758,18 @@ int setup_splits(vj_ctx* ctx, FastEngine& E) {
     if (E.SA > 1)
         VJ_TRY(setup_one_split(ctx, E, E.SA, E.m, E.n, P.rp.as<int>(), P.ci.as<int>(), E.av_s.as<double>(), E.builtSA,
                                E.bpA, E.idxA, E.permA, E.valA, E.psumA, E.tileSA, E.tbaseSA, E.ntSA, E.recSA_buf,
-                               E.tcapSA));
+                               E.tcapSA, E.toffSA));
     if (E.SAt > 1)
         VJ_TRY(setup_one_split(ctx, E, E.SAt, E.n, E.m, P.tp.as<int>(), P.ti.as<int>(), E.tv_s.as<double>(),
                                E.builtSAt, E.bpAt, E.idxAt, E.permAt, E.valAt, E.psumAt, E.tileSAt, E.tbaseSAt, E.ntSAt,
-                               E.recSAt_buf, E.tcapSAt));
+                               E.recSAt_buf, E.tcapSAt, E.toffSAt));
     // split partial kernels see ~nnz/(rows*S) entries per row: few lanes per row
     E.GsA = (static_cast<double>(E.nnz) / std::max(1, E.m) / E.SA) > 3.0 ? 4 : 2;
     E.GsAt = (static_cast<double>(E.nnz) / std::max(1, E.n) / E.SAt) > 3.0 ? 4 : 2;
     return VJ_OK;
 }
 
-// Sliced-ELL copies for a single LP: the layout is built on the host once per upload for a
+// Sliced-ELL copies for a single LP: the layout is built on the device once per upload for a
 // matrix whose padding stays within a quarter of its nonzeros (or whose ELL kernel the
 // caller forces); the values are gathered per solve from the scaled FP64 matrix.
 constexpr double kEllMaxPadding = 1.25;
@@ -942,18 +949,20 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
             else c.push_back({Sm, lanes, false, false, false});
             return c;
         }
-        // FP32 (index, value) records come before the separate arrays of the same kernel: with
-        // the tie-break margin below an earlier candidate wins near-ties, and records were
-        // faster on every gather-bound step measured (e.g. 77.7 against 82.0 us for the split
-        // A x_bar step of the 1M x 2M LP)
+        // With the tie-break margin below an earlier candidate wins near-ties, so the order is
+        // that of the measured preference: FP32 (index, value) records before the separate
+        // arrays of the same kernel (faster on every gather-bound step measured, e.g. 77.7
+        // against 82.0 us for the split A x_bar step of the 1M x 2M LP), and the sliced ELL,
+        // whose copy exists only where it pads little, before the tile-streamed kernel (A^T y
+        // of that LP: 79.4 against 81.0 us, which the margin alone does not separate)
         if (forced <= 1 || Sm == 1) {   // unsplit candidates
-            if (f32) c.push_back({1, choose_lanes(avg), true, true, false});
-            c.push_back({1, choose_lanes(avg), true, false, false});
-            for (int g : {4, 8, 16, 32}) c.push_back({1, g, false, false, false});
             if (ell) {
                 if (f32) c.push_back({1, choose_lanes(avg), false, true, true});
                 c.push_back({1, choose_lanes(avg), false, false, true});
             }
+            if (f32) c.push_back({1, choose_lanes(avg), true, true, false});
+            c.push_back({1, choose_lanes(avg), true, false, false});
+            for (int g : {4, 8, 16, 32}) c.push_back({1, g, false, false, false});
         }
         if (Sm > 1) {
             if (f32) c.push_back({Sm, 0, false, true, false});   // tile-streamed split

@@ -219,14 +219,24 @@ __device__ __forceinline__ void primal_epilogue(T acc, int j, const InstState& s
 // Dual epilogue for row i given w = (A x_bar)_i: prox of the row-bound support function
 // written per active side, y+ = y + sigma*(proj_[l,u](w - y/sigma) - w) in full
 // coordinates, then the reflected Halpern step.
+// Equality rows: when every row of the problem has l = u, the host passes the same vector
+// for both shifted bounds (lrs == urs), and the update needs neither the upper bound nor the
+// centre's dual: with l = u the prox is y + sigma (l - w) on every branch below (the first
+// branch takes the tie sigma (w - l) = y_full, so no branch depends on y_full), which saves
+// two of the four row operands a dual step reads.
 template <class T>
 struct DualOps {
     T yk, yh, lo, hi;
+    bool eq;
 };
 template <class T>
 __device__ __forceinline__ DualOps<T> dual_ops(int i, const T* __restrict__ y, const T* __restrict__ yhat_t,
                                                const T* __restrict__ lrs, const T* __restrict__ urs) {
-    return DualOps<T>{y[i], yhat_t[i], lrs[i], urs[i]};
+    if (lrs == urs) {
+        const T b = lrs[i];
+        return DualOps<T>{y[i], static_cast<T>(0), b, b, true};
+    }
+    return DualOps<T>{y[i], yhat_t[i], lrs[i], urs[i], false};
 }
 
 template <class T>
@@ -237,11 +247,15 @@ __device__ __forceinline__ void dual_update(T w, const DualOps<T>& o, int i, con
     const double kk = static_cast<double>(s.k + it);
     const T lam = static_cast<T>((kk + 1.0) / (kk + 2.0));
     const T yk = o.yk;
-    const T sfull = o.yh + yk;
     T yt;
-    if (sig * (w - o.lo) < sfull) yt = yk + sig * (o.lo - w);
-    else if (sig * (w - o.hi) > sfull) yt = yk + sig * (o.hi - w);
-    else yt = -o.yh;
+    if (o.eq) {
+        yt = yk + sig * (o.lo - w);
+    } else {
+        const T sfull = o.yh + yk;
+        if (sig * (w - o.lo) <= sfull) yt = yk + sig * (o.lo - w);
+        else if (sig * (w - o.hi) > sfull) yt = yk + sig * (o.hi - w);
+        else yt = -o.yh;
+    }
     y[i] = lam * ((static_cast<T>(1) + static_cast<T>(rho)) * yt - static_cast<T>(rho) * yk);
     const double d = static_cast<double>(yt) - static_cast<double>(yk);
     if (flags & 1) c[0] = d * d;
@@ -571,8 +585,8 @@ __global__ __launch_bounds__(kBlock) void split_partials(int rows, int S, const 
 }
 
 // Tile-streamed column-split partials. Workgroup w handles block k = w % S (the same XCD
-// mapping as split_partials) and row tile t = w / S. The host builds tiles common to all
-// blocks: at most split_rows<T>() rows, and at most split_cap<T>() entries in every block; a
+// mapping as split_partials) and row tile t = w / S. The tiles (built on the device) are common
+// to all blocks: at most split_rows<T>() rows, and at most split_cap<T>() entries in every block; a
 // single row longer than that in some block is a tile of its own, reduced by the whole
 // workgroup.
 // The block's span of values and indices is streamed with coalesced loads independent of
@@ -590,14 +604,19 @@ constexpr int kSplitLdsBytes = 16384;
 template <class T>
 constexpr int split_cap() { return kSplitLdsBytes / static_cast<int>(sizeof(T)); }
 
-// tile_base[t * S + k] = bp[k][first row of tile t] (t = 0..tiles): the tile's span of block k
-// comes from one load that depends on nothing, so the entry stream starts without waiting
-// for the tile's rows and their pointers (70.9 us against 72.4 us for the FP32 A x_bar
-// partials); the row bounds are loaded after the stream is issued.
+// tile_base[t * S + k] = first entry of tile t in block k (t = 0..tiles): the tile's span of
+// block k comes from one load that depends on nothing, so the entry stream starts without
+// waiting for the tile's rows (70.9 us against 72.4 us for the FP32 A x_bar partials when
+// the tile's span came from the block pointers of its first and last rows).
+// toff[k * rows + r] = first entry of row r in block k relative to its tile's first entry
+// (at most split_cap<T>() in a tile that fits LDS; a long-row tile's single row has 0): the
+// row bounds cost two bytes per row and block instead of a 4-byte block pointer. They are
+// loaded after the stream is issued: loaded before it, they made the stream wait for the
+// tile's row range (83.6 us against 77.9 us for the FP32 A x_bar pair).
 template <class Ent, class T>
 __global__ __launch_bounds__(kBlock) void split_stream_partials(int rows, int S, const int* __restrict__ tile_row,
                                                                 const int* __restrict__ tile_base,
-                                                                const int* __restrict__ bp, Ent ent,
+                                                                const uint16_t* __restrict__ toff, Ent ent,
                                                                 const T* __restrict__ vec, T* __restrict__ part,
                                                                 const InstState* __restrict__ st) {
     __shared__ T lds[split_cap<T>()];
@@ -606,7 +625,6 @@ __global__ __launch_bounds__(kBlock) void split_stream_partials(int rows, int S,
     const int t = static_cast<int>(blockIdx.x) / S;
     const int s0 = tile_base[t * S + k], s1 = tile_base[(t + 1) * S + k];
     const int r0 = tile_row[t], r1 = tile_row[t + 1];
-    const int* p = bp + static_cast<int64_t>(k) * (rows + 1);
     T* out = part + static_cast<int64_t>(k) * rows;
     if (s1 - s0 > split_cap<T>()) {
         const T acc = long_row_sum<Ent, T>(s0, s1, ent, vec, lds);
@@ -614,13 +632,14 @@ __global__ __launch_bounds__(kBlock) void split_stream_partials(int rows, int S,
         return;
     }
     stream_products<Ent, T>(s0, s1, ent, vec, lds);
+    const uint16_t* o = toff + static_cast<int64_t>(k) * rows;
     constexpr int RPT = split_rows<T>() / kBlock;
     int lo[RPT], hi[RPT];
 #pragma unroll
     for (int u = 0; u < RPT; ++u) {
         const int r = r0 + static_cast<int>(threadIdx.x) + u * kBlock;
-        lo[u] = r < r1 ? p[r] - s0 : 0;
-        hi[u] = r < r1 ? p[r + 1] - s0 : 0;
+        lo[u] = r < r1 ? static_cast<int>(o[r]) : 0;
+        hi[u] = r + 1 < r1 ? static_cast<int>(o[r + 1]) : (r < r1 ? s1 - s0 : 0);
     }
     __syncthreads();
 #pragma unroll

@@ -120,6 +120,23 @@ __global__ void tile_bases(int ntiles, int S, const int* __restrict__ tile_row, 
     tbase[i] = ptrs[k * stride + tile_row[t]];
 }
 
+// toff[i] for row r = i mod rows of block k = i / rows: its first entry in the block minus the
+// first entry of its tile in the block.
+__global__ void tile_offsets(int rows, int S, int ntiles, const int* __restrict__ tile_row, const int* __restrict__ ptrs,
+                             int64_t stride, uint16_t* __restrict__ toff) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= static_cast<int64_t>(S) * rows) return;
+    const int64_t k = i / rows;
+    const int r = static_cast<int>(i % rows);
+    int lo = 0, hi = ntiles;   // the last tile t with tile_row[t] <= r
+    while (hi - lo > 1) {
+        const int mid = lo + (hi - lo) / 2;
+        if (tile_row[mid] <= r) lo = mid; else hi = mid;
+    }
+    const int v = ptrs[k * stride + r] - ptrs[k * stride + tile_row[lo]];
+    toff[i] = static_cast<uint16_t>(v > 65535 ? 65535 : v);
+}
+
 // Column split: F[k*(rows+1) + r + 1] = entries of row r in column block k, F[k*(rows+1)] = 0;
 // G lanes per row.
 constexpr int kSplitLanes = 8;
@@ -329,6 +346,16 @@ int device_tile_bases(hipStream_t s, int ntiles, int S, const int* tile_row, con
     VJ_TRY(tbase.alloc(static_cast<size_t>(len) * 4));
     hipLaunchKernelGGL(tile_bases, dim3(grid_for(len, kBlock)), dim3(kBlock), 0, s, ntiles, S, tile_row, ptrs, stride,
                        tbase.as<int>());
+    VJ_LAUNCHED();
+    return VJ_OK;
+}
+
+int device_tile_offsets(hipStream_t s, int rows, int S, int ntiles, const int* tile_row, const int* ptrs,
+                        int64_t stride, DevBuf& toff) {
+    const int64_t len = static_cast<int64_t>(S) * rows;
+    VJ_TRY(toff.alloc(std::max<size_t>(static_cast<size_t>(len), 1) * 2));
+    hipLaunchKernelGGL(tile_offsets, dim3(grid_for(len, kBlock)), dim3(kBlock), 0, s, rows, S, ntiles, tile_row, ptrs,
+                       stride, toff.as<uint16_t>());
     VJ_LAUNCHED();
     return VJ_OK;
 }

@@ -32,6 +32,12 @@ int device_tile_bases(hipStream_t s, int ntiles, int S, const int* tile_row, con
 int device_split(hipStream_t s, int rows, int cols, int64_t nnz, const int* rp, const int* ci, int S, DevBuf& bp,
                  DevBuf& idx, DevBuf& perm);
 
+// Tile-relative row offsets of a column split: toff[k*rows + r] = ptrs[k*stride + r] -
+// ptrs[k*stride + tile_row[t]] for the tile t holding row r (clamped to 65535: only a row of
+// a long-row tile, which is its tile's first row with offset 0, could exceed it otherwise).
+int device_tile_offsets(hipStream_t s, int rows, int S, int ntiles, const int* tile_row, const int* ptrs,
+                        int64_t stride, DevBuf& toff);
+
 // Sliced ELL of a CSR in slices of kEllSlice rows: per-slice entry offsets (int32) and widths
 // (the slice's longest row); entry q of row r at off[s] + kEllSlice*q + (r mod kEllSlice);
 // perm = CSR position or -1 for padding, whose index repeats the row's first column (0 for an
