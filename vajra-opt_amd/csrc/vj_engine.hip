@@ -280,7 +280,7 @@ int launch_primal(vj_ctx* ctx, FastEngine& E, const V* tv, T* x, T* xbar, const 
                 dim3(kBlock), 0, ctx->stream, E.n, E.SAt, E.bpAt.as<int>(), E.idxAt.as<int>(), E.valAt.as<V>(), y,
                 E.psumAt.as<T>(), E.st.as<InstState>()));
         }
-        hipLaunchKernelGGL((primal_finish<T>), dim3(grid_for(E.n, kBlock)), dim3(kBlock), 0, ctx->stream, E.n, E.SAt,
+        hipLaunchKernelGGL((primal_finish<T>), dim3(grid_for(E.n, kBlock * finish_rows<T>())), dim3(kBlock), 0, ctx->stream, E.n, E.SAt,
                            E.psumAt.as<T>(), E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), x, xbar, E.xhat.as<double>(),
                            E.xfull.as<double>(), E.st.as<InstState>(), E.part_x.as<double>(), it, flags, E.rho);
         VJ_LAUNCHED();
@@ -373,7 +373,7 @@ int launch_dual(vj_ctx* ctx, FastEngine& E, const V* av, const T* xbar, T* y, in
                 dim3(kBlock), 0, ctx->stream, E.m, E.SA, E.bpA.as<int>(), E.idxA.as<int>(), E.valA.as<V>(), xbar,
                 E.psumA.as<T>(), E.st.as<InstState>()));
         }
-        hipLaunchKernelGGL((dual_finish<T>), dim3(grid_for(E.m, kBlock)), dim3(kBlock), 0, ctx->stream, E.m, E.SA,
+        hipLaunchKernelGGL((dual_finish<T>), dim3(grid_for(E.m, kBlock * finish_rows<T>())), dim3(kBlock), 0, ctx->stream, E.m, E.SA,
                            E.psumA.as<T>(), yht, lrs, urs, y, E.yhat.as<double>(),
                            E.yfull.as<double>(), E.st.as<InstState>(), E.part_y.as<double>(), it, flags, E.rho);
         VJ_LAUNCHED();
@@ -732,7 +732,7 @@ int setup_one_split(vj_ctx* ctx, FastEngine& E, int S, int rows, int cols, const
     }
     const size_t vsz = E.prec == VJ_PREC_FP64 ? 8 : (E.prec == VJ_PREC_FP32 ? 4 : 2);
     VJ_TRY(val.alloc(static_cast<size_t>(nnz) * vsz));
-    VJ_TRY(psum.alloc(static_cast<size_t>(S) * rows * E.tsize()));
+    VJ_TRY(psum.alloc(static_cast<size_t>(S) * static_cast<size_t>(split_pstride(rows)) * E.tsize()));
     const int g = grid_for(nnz, kBlock);
     if (E.prec == VJ_PREC_FP64)
         hipLaunchKernelGGL(gather_split_vals<double>, dim3(g), dim3(kBlock), 0, s, nnz, perm.as<int>(), vals_dev, val.as<double>());
@@ -865,9 +865,10 @@ int enqueue_block(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
                       E.part_kr.as<double>(), E.part_kc.as<double>(), E.kkt8.as<double>()));
     ControlArgs a;
     a.px = E.part_x.as<double>();
-    a.Rx = (E.SAt > 1 || E.ellAt) ? kBlock : kBlock / E.GAt;   // tile height of the kernel writing the partials
+    // rows per workgroup of the kernel writing the partials
+    a.Rx = E.SAt > 1 ? kBlock * finish_rows<T>() : (E.ellAt ? kBlock : kBlock / E.GAt);
     a.py = E.part_y.as<double>();
-    a.Ry = (E.SA > 1 || E.ellA) ? kBlock : kBlock / E.GA;
+    a.Ry = E.SA > 1 ? kBlock * finish_rows<T>() : (E.ellA ? kBlock : kBlock / E.GA);
     const bool sx = E.SAt == 1 && E.streamAt && !E.ellAt, sy = E.SA == 1 && E.streamA && !E.ellA;
     a.xt_first = sx ? E.tfAt.as<int>() : nullptr;
     a.xt_last = sx ? E.tlAt.as<int>() : nullptr;

@@ -183,27 +183,45 @@ __device__ __forceinline__ PrimalOps<T> primal_ops(int j, const T* __restrict__ 
     return PrimalOps<T>{x_start<T>(j, x, xbar, s, it, rho), dhat[j], lcs[j], ucs[j]};
 }
 
+// The projected point xt of column j (the values every primal kernel writes derive from it).
+template <class T>
+__device__ __forceinline__ T primal_point(T acc, const PrimalOps<T>& o, T tau) {
+    T xt = o.xk - tau * (o.dh - acc);
+    xt = xt < o.lo ? o.lo : xt;
+    xt = xt > o.hi ? o.hi : xt;
+    return xt;
+}
+// Residual contributions of one column at the measured iterations, accumulated into c (one
+// column per thread starts from zero, so the sums equal the single terms), and the full FP64
+// point at the block's last iteration.
+template <class T>
+__device__ __forceinline__ void primal_residuals(T xt, T xk, int j, int flags, const double* __restrict__ xhat,
+                                                 double* __restrict__ xfull, double (&c)[3]) {
+    const double d = static_cast<double>(xt) - static_cast<double>(xk);
+    if (flags & 1) c[0] += d * d;
+    if (flags & 2) {
+        c[1] += d * d;
+        c[2] += static_cast<double>(xt) * static_cast<double>(xt);
+        xfull[j] = xhat[j] + static_cast<double>(xt);
+    }
+}
+template <class T>
+__device__ __forceinline__ T anchor_weight(const InstState& s, int it) {
+    const double kk = static_cast<double>(s.k + it);
+    return static_cast<T>((kk + 1.0) / (kk + 2.0));
+}
+
 template <class T>
 __device__ __forceinline__ void primal_update(T acc, const PrimalOps<T>& o, int j, const InstState& s, int it,
                                               int flags, double rho, T* __restrict__ x, T* __restrict__ xbar,
                                               const double* __restrict__ xhat, double* __restrict__ xfull,
                                               double (&c)[3]) {
-    const T tau = static_cast<T>(s.tau);
-    const double kk = static_cast<double>(s.k + it);
-    const T lam = static_cast<T>((kk + 1.0) / (kk + 2.0));
     const T xk = o.xk;
-    T xt = xk - tau * (o.dh - acc);
-    xt = xt < o.lo ? o.lo : xt;
-    xt = xt > o.hi ? o.hi : xt;
+    const T xt = primal_point<T>(acc, o, static_cast<T>(s.tau));
     xbar[j] = static_cast<T>(2) * xt - xk;
-    if (rho != 1.0) x[j] = lam * ((static_cast<T>(1) + static_cast<T>(rho)) * xt - static_cast<T>(rho) * xk);
-    const double d = static_cast<double>(xt) - static_cast<double>(xk);
-    if (flags & 1) c[0] = d * d;
-    if (flags & 2) {
-        c[1] = d * d;
-        c[2] = static_cast<double>(xt) * static_cast<double>(xt);
-        xfull[j] = xhat[j] + static_cast<double>(xt);
-    }
+    if (rho != 1.0)
+        x[j] = anchor_weight<T>(s, it) * ((static_cast<T>(1) + static_cast<T>(rho)) * xt - static_cast<T>(rho) * xk);
+    primal_residuals<T>(xt, xk, j, flags, xhat, xfull, c);
 }
 
 template <class T>
@@ -239,31 +257,39 @@ __device__ __forceinline__ DualOps<T> dual_ops(int i, const T* __restrict__ y, c
     return DualOps<T>{y[i], yhat_t[i], lrs[i], urs[i], false};
 }
 
+// The prox point yt of row i (the value the dual kernels write derives from it).
+template <class T>
+__device__ __forceinline__ T dual_point(T w, const DualOps<T>& o, T sig) {
+    const T yk = o.yk;
+    if (o.eq) return yk + sig * (o.lo - w);
+    const T sfull = o.yh + yk;
+    if (sig * (w - o.lo) <= sfull) return yk + sig * (o.lo - w);
+    if (sig * (w - o.hi) > sfull) return yk + sig * (o.hi - w);
+    return -o.yh;
+}
+template <class T>
+__device__ __forceinline__ T dual_next(T yt, T yk, T lam, double rho) {
+    return lam * ((static_cast<T>(1) + static_cast<T>(rho)) * yt - static_cast<T>(rho) * yk);
+}
+template <class T>
+__device__ __forceinline__ void dual_residuals(T yt, T yk, int i, int flags, const double* __restrict__ yhat,
+                                               double* __restrict__ yfull, double (&c)[3]) {
+    const double d = static_cast<double>(yt) - static_cast<double>(yk);
+    if (flags & 1) c[0] += d * d;
+    if (flags & 2) {
+        c[1] += d * d;
+        c[2] += static_cast<double>(yt) * static_cast<double>(yt);
+        yfull[i] = yhat[i] + static_cast<double>(yt);
+    }
+}
+
 template <class T>
 __device__ __forceinline__ void dual_update(T w, const DualOps<T>& o, int i, const InstState& s, int it, int flags,
                                             double rho, T* __restrict__ y, const double* __restrict__ yhat,
                                             double* __restrict__ yfull, double (&c)[3]) {
-    const T sig = static_cast<T>(s.sigma);
-    const double kk = static_cast<double>(s.k + it);
-    const T lam = static_cast<T>((kk + 1.0) / (kk + 2.0));
-    const T yk = o.yk;
-    T yt;
-    if (o.eq) {
-        yt = yk + sig * (o.lo - w);
-    } else {
-        const T sfull = o.yh + yk;
-        if (sig * (w - o.lo) <= sfull) yt = yk + sig * (o.lo - w);
-        else if (sig * (w - o.hi) > sfull) yt = yk + sig * (o.hi - w);
-        else yt = -o.yh;
-    }
-    y[i] = lam * ((static_cast<T>(1) + static_cast<T>(rho)) * yt - static_cast<T>(rho) * yk);
-    const double d = static_cast<double>(yt) - static_cast<double>(yk);
-    if (flags & 1) c[0] = d * d;
-    if (flags & 2) {
-        c[1] = d * d;
-        c[2] = static_cast<double>(yt) * static_cast<double>(yt);
-        yfull[i] = yhat[i] + static_cast<double>(yt);
-    }
+    const T yt = dual_point<T>(w, o, static_cast<T>(s.sigma));
+    y[i] = dual_next<T>(yt, o.yk, anchor_weight<T>(s, it), rho);
+    dual_residuals<T>(yt, o.yk, i, flags, yhat, yfull, c);
 }
 
 template <class T>
@@ -557,11 +583,15 @@ __global__ __launch_bounds__(kBlock) void stream_dual_step(
 }
 
 // ---------------------------------------------------------------------------------
+// Row stride of the S partial-sum vectors of a column split: a multiple of 64 entries, so that
+// every vector starts 256-byte aligned and the finish kernels read it 16 bytes at a time.
+__host__ __device__ constexpr int64_t split_pstride(int rows) { return (static_cast<int64_t>(rows) + 63) / 64 * 64; }
+
 // XCD-aware column-split SpMV (single LP). The matrix is stored as S column blocks,
 // each a CSR over all rows with absolute pointers bp[k*(rows+1) + r]. Workgroup w works on
 // block w % S: consecutive workgroup ids are dispatched round-robin over the 8 XCDs, so
 // each XCD (with S dividing 8) only ever gathers from one block of the vector, which stays
-// resident in that XCD's 4 MB L2. Block partial sums go to part[k*rows + r]; the finish
+// resident in that XCD's 4 MB L2. Block partial sums go to part[k*pstride + r]; the finish
 // kernels add the S partials in block order (deterministic) and run the epilogue.
 template <class V, class T, int G>
 __global__ __launch_bounds__(kBlock) void split_partials(int rows, int S, const int* __restrict__ bp,
@@ -581,7 +611,7 @@ __global__ __launch_bounds__(kBlock) void split_partials(int rows, int S, const 
         for (int q = p[r] + lane; q < q1; q += G) acc += static_cast<T>(load_val(bval, q)) * vec[load_idx(bidx, q)];
     }
     acc = group_sum<G>(acc);
-    if (r < rows && lane == 0) part[static_cast<int64_t>(k) * rows + r] = acc;
+    if (r < rows && lane == 0) part[static_cast<int64_t>(k) * split_pstride(rows) + r] = acc;
 }
 
 // Tile-streamed column-split partials. Workgroup w handles block k = w % S (the same XCD
@@ -625,7 +655,7 @@ __global__ __launch_bounds__(kBlock) void split_stream_partials(int rows, int S,
     const int t = static_cast<int>(blockIdx.x) / S;
     const int s0 = tile_base[t * S + k], s1 = tile_base[(t + 1) * S + k];
     const int r0 = tile_row[t], r1 = tile_row[t + 1];
-    T* out = part + static_cast<int64_t>(k) * rows;
+    T* out = part + static_cast<int64_t>(k) * split_pstride(rows);
     if (s1 - s0 > split_cap<T>()) {
         const T acc = long_row_sum<Ent, T>(s0, s1, ent, vec, lds);
         if (threadIdx.x == 0) out[r0] = acc;
@@ -653,6 +683,30 @@ __global__ __launch_bounds__(kBlock) void split_stream_partials(int rows, int S,
     }
 }
 
+// The finish kernels add the S block partials of finish_rows<T>() consecutive rows per thread
+// with 16-byte loads (one per partial vector and operand) and run the epilogue row by row,
+// with the same arithmetic as the one-row kernels; a ragged end is done row by row.
+template <class T>
+constexpr int finish_rows() { return static_cast<int>(16 / sizeof(T)); }
+template <class T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    typedef float type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct Vec16<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+template <class T>
+__device__ __forceinline__ typename Vec16<T>::type load16(const T* p) {
+    return *reinterpret_cast<const typename Vec16<T>::type*>(p);
+}
+template <class T>
+__device__ __forceinline__ void store16(T* p, const typename Vec16<T>::type& v) {
+    *reinterpret_cast<typename Vec16<T>::type*>(p) = v;
+}
+
 template <class T>
 __global__ __launch_bounds__(kBlock) void primal_finish(
     int n, int S, const T* __restrict__ psum, const T* __restrict__ dhat, const T* __restrict__ lcs,
@@ -660,12 +714,40 @@ __global__ __launch_bounds__(kBlock) void primal_finish(
     double* __restrict__ xfull, const InstState* __restrict__ st, double* __restrict__ part, int it, int flags,
     double rho) {
     if (st[0].active == 0) return;
-    const int j = blockIdx.x * kBlock + threadIdx.x;
+    constexpr int V = finish_rows<T>();
+    typedef typename Vec16<T>::type VT;
+    const int64_t ps = split_pstride(n);
+    const int j0 = (static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x)) * V;
+    const InstState& s = st[0];
     double c[3] = {0.0, 0.0, 0.0};
-    if (j < n) {
-        T acc = psum[j];
-        for (int k = 1; k < S; ++k) acc += psum[static_cast<int64_t>(k) * n + j];
-        primal_epilogue<T>(acc, j, st[0], it, flags, rho, dhat, lcs, ucs, x, xbar, xhat, xfull, c);
+    if (j0 + V <= n) {
+        VT acc = load16<T>(psum + j0);
+        for (int k = 1; k < S; ++k) acc += load16<T>(psum + k * ps + j0);
+        const VT xb = load16<T>(xbar + j0), dh = load16<T>(dhat + j0), lo = load16<T>(lcs + j0),
+                 hi = load16<T>(ucs + j0);
+        VT xk;
+        if (rho == 1.0) {
+            const T lam_prev = anchor_weight<T>(s, it - 1);   // x recovered from x_bar, as x_start
+            for (int v = 0; v < V; ++v) xk[v] = lam_prev * xb[v];
+        } else {
+            xk = load16<T>(x + j0);
+        }
+        const T tau = static_cast<T>(s.tau), lam = anchor_weight<T>(s, it);
+        VT xb_new, x_new;
+        for (int v = 0; v < V; ++v) {
+            const T xt = primal_point<T>(acc[v], PrimalOps<T>{xk[v], dh[v], lo[v], hi[v]}, tau);
+            xb_new[v] = static_cast<T>(2) * xt - xk[v];
+            x_new[v] = lam * ((static_cast<T>(1) + static_cast<T>(rho)) * xt - static_cast<T>(rho) * xk[v]);
+            primal_residuals<T>(xt, xk[v], j0 + v, flags, xhat, xfull, c);
+        }
+        store16<T>(xbar + j0, xb_new);
+        if (rho != 1.0) store16<T>(x + j0, x_new);
+    } else {
+        for (int j = j0; j < n; ++j) {
+            T acc = psum[j];
+            for (int k = 1; k < S; ++k) acc += psum[k * ps + j];
+            primal_epilogue<T>(acc, j, s, it, flags, rho, dhat, lcs, ucs, x, xbar, xhat, xfull, c);
+        }
     }
     if (flags) {
         const int mask = ((flags & 1) ? 1 : 0) | ((flags & 2) ? 6 : 0);
@@ -679,12 +761,36 @@ __global__ __launch_bounds__(kBlock) void dual_finish(
     const T* __restrict__ urs, T* __restrict__ y, const double* __restrict__ yhat, double* __restrict__ yfull,
     const InstState* __restrict__ st, double* __restrict__ part, int it, int flags, double rho) {
     if (st[0].active == 0) return;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    constexpr int V = finish_rows<T>();
+    typedef typename Vec16<T>::type VT;
+    const int64_t ps = split_pstride(m);
+    const int i0 = (static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x)) * V;
+    const InstState& s = st[0];
     double c[3] = {0.0, 0.0, 0.0};
-    if (i < m) {
-        T w = psum[i];
-        for (int k = 1; k < S; ++k) w += psum[static_cast<int64_t>(k) * m + i];
-        dual_epilogue<T>(w, i, st[0], it, flags, rho, yhat_t, lrs, urs, y, yhat, yfull, c);
+    if (i0 + V <= m) {
+        VT w = load16<T>(psum + i0);
+        for (int k = 1; k < S; ++k) w += load16<T>(psum + k * ps + i0);
+        const bool eq = lrs == urs;
+        const VT yk = load16<T>(y + i0), lo = load16<T>(lrs + i0);
+        VT hi = lo, yh = lo;
+        if (!eq) {
+            hi = load16<T>(urs + i0);
+            yh = load16<T>(yhat_t + i0);
+        }
+        const T sig = static_cast<T>(s.sigma), lam = anchor_weight<T>(s, it);
+        VT y_new;
+        for (int v = 0; v < V; ++v) {
+            const T yt = dual_point<T>(w[v], DualOps<T>{yk[v], eq ? static_cast<T>(0) : yh[v], lo[v], hi[v], eq}, sig);
+            y_new[v] = dual_next<T>(yt, yk[v], lam, rho);
+            dual_residuals<T>(yt, yk[v], i0 + v, flags, yhat, yfull, c);
+        }
+        store16<T>(y + i0, y_new);
+    } else {
+        for (int i = i0; i < m; ++i) {
+            T w = psum[i];
+            for (int k = 1; k < S; ++k) w += psum[k * ps + i];
+            dual_epilogue<T>(w, i, s, it, flags, rho, yhat_t, lrs, urs, y, yhat, yfull, c);
+        }
     }
     if (flags) {
         const int mask = ((flags & 1) ? 1 : 0) | ((flags & 2) ? 6 : 0);
