@@ -41,16 +41,18 @@ def log(*a):
 
 
 USE_GRAPH = 1
+VERBOSE = 1
 
 
 def solve_timed(vp, eng, prec, tol, max_iters, time_limit, force=None):
-    # verbose=1: the engine prints a progress line to stderr every 10 s of a long solve
+    # verbose=1: the engine prints a progress line to stderr every 10 s of a long solve; 2 also
+    # prints the autotuner's timings
     kw = dict(use_graph=USE_GRAPH)
     if force:
         la, lat, sa, sat = (int(v) for v in force.split(","))
         kw.update(lanes_A=la, lanes_At=lat, col_splits_A=sa, col_splits_At=sat)
     p = vp.default_params(vp.MODE_RESTARTED, precision=prec, tolerance=tol, max_iterations=max_iters,
-                          time_limit=time_limit, verbose=1, **kw)
+                          time_limit=time_limit, verbose=VERBOSE, **kw)
     t0 = time.perf_counter()
     r = eng.solve(p)
     wall = time.perf_counter() - t0
@@ -84,6 +86,8 @@ def main():
     ap.add_argument("--force-kernels", default=None, metavar="LANES_A,LANES_AT,SPLITS_A,SPLITS_AT",
                     help="fix the headline solve's kernel variants (vj_params lanes_A, lanes_At, col_splits_A, "
                          "col_splits_At) instead of autotuning them, e.g. to profile the variants a timed run chose")
+    ap.add_argument("--verbose", type=int, default=1, choices=[1, 2],
+                    help="2: the engine also prints the autotuner's candidate timings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design note "
@@ -91,8 +95,9 @@ def main():
         sys.exit(2)
     if args.steps < 1 or args.warmup < 0:
         raise SystemExit("--steps must be >= 1 and --warmup >= 0")
-    global USE_GRAPH
+    global USE_GRAPH, VERBOSE
     USE_GRAPH = 0 if args.no_graph else 1
+    VERBOSE = args.verbose
 
     import vajra_pdhg as vp
 

@@ -951,11 +951,18 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
             return c;
         }
         // With the tie-break margin below an earlier candidate wins near-ties, so the order is
-        // that of the measured preference: FP32 (index, value) records before the separate
-        // arrays of the same kernel (faster on every gather-bound step measured, e.g. 77.7
-        // against 82.0 us for the split A x_bar step of the 1M x 2M LP), and the sliced ELL,
-        // whose copy exists only where it pads little, before the tile-streamed kernel (A^T y
-        // of that LP: 79.4 against 81.0 us, which the margin alone does not separate)
+        // that of the measured preference: the column split first where it exists (the gathered
+        // vector exceeds the L2 budget; on the 1M x 2M LP it beat the sliced ELL of A^T by 2%,
+        // 77.7 against 79.3 us, too close for the margin alone to separate), FP32 (index, value)
+        // records before the separate arrays of the same kernel (faster on every gather-bound
+        // step measured, e.g. 77.7 against 82.0 us for the split A x_bar step of that LP), and
+        // the sliced ELL, whose copy exists only where it pads little, before the tile-streamed
+        // kernel (79.4 against 81.0 us for A^T y unsplit)
+        if (Sm > 1) {
+            if (f32) c.push_back({Sm, 0, false, true, false});   // tile-streamed split
+            c.push_back({Sm, 0, false, false, false});
+            for (int g : {1, 2, 4}) c.push_back({Sm, g, false, false, false});
+        }
         if (forced <= 1 || Sm == 1) {   // unsplit candidates
             if (ell) {
                 if (f32) c.push_back({1, choose_lanes(avg), false, true, true});
@@ -964,11 +971,6 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
             if (f32) c.push_back({1, choose_lanes(avg), true, true, false});
             c.push_back({1, choose_lanes(avg), true, false, false});
             for (int g : {4, 8, 16, 32}) c.push_back({1, g, false, false, false});
-        }
-        if (Sm > 1) {
-            if (f32) c.push_back({Sm, 0, false, true, false});   // tile-streamed split
-            c.push_back({Sm, 0, false, false, false});
-            for (int g : {1, 2, 4}) c.push_back({Sm, g, false, false, false});
         }
         return c;
     };
@@ -1005,6 +1007,13 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     const size_t rounds = std::max(cA.size(), cAt.size());
     double bestA = 1e30, bestAt = 1e30;
     StepCfg wA = cA[0], wAt = cAt[0];
+    {   // one untimed pass first: the first candidate is otherwise timed on colder clocks and
+        // caches than the rest, which biases the comparison against it
+        double up = 0, ud = 0;
+        apply(cA[0], E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA);
+        apply(cAt[0], E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt);
+        VJ_TRY((time_kernels<V, T>(ctx, E, av, tv, kTuneReps, &up, &ud)));
+    }
     for (size_t r = 0; r < rounds; ++r) {
         const StepCfg& a = cA[std::min(r, cA.size() - 1)];
         const StepCfg& b = cAt[std::min(r, cAt.size() - 1)];
