@@ -262,16 +262,19 @@ def bench_batch(vp, kind, args, budget_left):
     eng = vp.Engine(0)
     eng.upload_batch(lps)
     p = vp.default_params(vp.MODE_RESTARTED, precision=args.precision, tolerance=args.tol,
-                          max_iterations=args.max_iters, time_limit=args.time_limit, verbose=1, use_graph=USE_GRAPH)
+                          max_iterations=args.max_iters, time_limit=args.time_limit, verbose=VERBOSE,
+                          use_graph=USE_GRAPH)
     t0 = time.perf_counter()
     res = eng.solve_batch(p)
     t_batch = time.perf_counter() - t0
     ok = sum(1 for r in res if r.status == 1)
     worst = max(r.rel_kkt for r in res)
     log(f"batch {kind}: {ok}/{cnt} optimal in {t_batch:.2f}s (max iters {max(r.iterations for r in res)}), "
-        f"worst rel_kkt {worst:.2e}, check interval {res[0].check_interval}")
+        f"worst rel_kkt {worst:.2e}, check interval {res[0].check_interval}, "
+        f"lanes {res[0].lanes_A},{res[0].lanes_At}")
     eng.close()
     out = dict(count=cnt, sum_nnz=nnz, optimal=ok, worst_rel_kkt=worst, check_interval=res[0].check_interval,
+               lanes=[res[0].lanes_A, res[0].lanes_At],
                max_iterations=max(r.iterations for r in res), batch_s=t_batch, batch_lps_per_s=cnt / t_batch,
                precision=args.precision)
     if seq_n <= 0 or not budget_left(f"sequential {kind}"):

@@ -109,9 +109,17 @@ typedef struct vj_params {
                                       -3 = sliced ELL (slices of 64 rows, single LP; where no
                                       ELL copy exists, as -1); -4 = sliced ELL reading (index,
                                       value) records (FP32; other precisions as -3; no ELL copy:
-                                      as -2). The result reports the kernel that ran with the
+                                      as -2); -5 = LDS-staged (tiles of one instance copy the
+                                      window of the gathered vector they read into LDS; built
+                                      for batches, and for a single LP when forced); -6 = the
+                                      same reading (index, value) records (FP32; otherwise as
+                                      -5); -7 = LDS-staged reading 16-bit column offsets from
+                                      each tile's window start (built where every window
+                                      spans at most 65,536 columns; otherwise as -5). The
+                                      result reports the kernel that ran with the
                                       same codes; the autotuner tries ELL where its padding is
-                                      at most 25% of the nonzeros */
+                                      at most 25% of the nonzeros, and the staged kernels for
+                                      batches */
     int lanes_At;                  /* same for rows of A^T */
     int refine_rounds;             /* FP32/BF16: max FP64 refinement rounds (0 = 64) */
     int verbose;

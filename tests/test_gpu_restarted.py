@@ -258,25 +258,54 @@ def test_structured_lps_against_highs(engine, seed, m, n, prec, lanes):
     assert abs(got - opt) <= 1e-6 * (1 + abs(opt)), (got, opt)
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_structured_batch_against_highs(engine, prec):
+@pytest.mark.parametrize("prec,lanes", [("fp64", 0), ("fp32", 0), ("fp64", -5), ("fp32", -6), ("fp32", -7)])
+def test_structured_batch_against_highs(engine, prec, lanes):
     """The eight structured LPs (1x9 and 9x1 among them) solved as one block-diagonal batch,
     twice over in mixed order, so that tiles span several instances of different shapes:
-    every instance reaches rel-KKT 1e-8 (oracle-checked) and its HiGHS objective to 1e-6."""
+    every instance reaches rel-KKT 1e-8 (oracle-checked) and its HiGHS objective to 1e-6.
+    Also with the LDS-staged kernels forced (their tiles hold one instance each)."""
     lps = [lp_cases.structured_feasible_lp(*s) for s in STRUCTURED]
     lps = lps + lps[::-1]
     opts = [lp_cases.highs_optimum(lp) for lp in lps]
-    print(f"[case] structured batch {prec} ({len(lps)} instances)", flush=True)
+    print(f"[case] structured batch {prec} lanes={lanes} ({len(lps)} instances)", flush=True)
     engine.upload_batch(lps)
     res = engine.solve_batch(vp.default_params(vp.MODE_RESTARTED, precision=prec, tolerance=1e-8,
-                                               max_iterations=200000))
+                                               max_iterations=200000, lanes_A=lanes, lanes_At=lanes))
     xs, ys = engine.download_batch()
+    if lanes:
+        assert all((r.lanes_A, r.lanes_At) == (lanes, lanes) for r in res), res[0].as_dict()
     for lp, opt, r, x, y in zip(lps, opts, res, xs, ys):
         assert r.status == 1, r.as_dict()
         k = O.rel_kkt(lp, x, y)
         assert k[0] <= 1e-8 and abs(k[0] - r.rel_kkt) <= 1e-11 + 1e-6 * k[0], (k, r.rel_kkt)
         got = float(np.dot(lp.c, x)) + lp.offset
         assert abs(got - opt) <= 1e-6 * (1 + abs(opt)), (got, opt)
+
+
+@pytest.mark.parametrize("prec,stream,staged", [("fp64", -1, -5), ("fp32", -2, -6), ("bf16", -1, -5),
+                                                ("fp64", -1, -7), ("fp32", -2, -7), ("bf16", -1, -7)])
+def test_staged_equals_tile_streamed(engine, prec, stream, staged):
+    """The LDS-staged steps form the same products and add each row in index order, as the
+    tile-streamed steps do, so one check block of a batch (before the first restart decision,
+    whose residual partials the two kernels sum in different tile orders) returns the same bits.
+    The batch mixes random and structured LPs, with windows wider than one LDS chunk (the
+    7,000 x 17,500 instance copies its x_bar window in two chunks in FP32, three in FP64).
+    Lanes -7 reads 16-bit column offsets from each tile's window start."""
+    lps = [vp.gen_random_lp(4100 + i, m, int(2.5 * m)) for i, m in enumerate([500, 7000, 1300])]
+    lps += [lp_cases.structured_feasible_lp(*s) for s in STRUCTURED[:3]]
+    engine.upload_batch(lps)
+    out = []
+    for lanes in (stream, staged):
+        print(f"[case] staged vs tile-streamed {prec} lanes={lanes}", flush=True)
+        res = engine.solve_batch(vp.default_params(vp.MODE_RESTARTED, precision=prec, tolerance=0.0,
+                                                   max_iterations=64, check_frequency=64, lanes_A=lanes,
+                                                   lanes_At=lanes))
+        want = 0 if lanes == stream else staged   # the result reports the tile-streamed kernel as 0
+        assert all((r.lanes_A, r.lanes_At) == (want, want) and r.iterations == 64 for r in res), res[0].as_dict()
+        out.append(engine.download_batch())
+    (xs0, ys0), (xs1, ys1) = out
+    for x0, y0, x1, y1 in zip(xs0, ys0, xs1, ys1):
+        assert np.array_equal(x0, x1) and np.array_equal(y0, y1)
 
 
 def test_infeasible_lps_never_reported_optimal(engine):

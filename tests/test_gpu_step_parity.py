@@ -4,8 +4,8 @@ return the full iterate of the last check (centre + the last T(z) correction, un
 that point must equal the numpy model of the engine (tests/engine_model.py: the same setup,
 correction coordinates, reflected Halpern step, restart rule and primal-weight update, written
 with scipy sparse products) to 1e-12 relative, for every kernel variant the autotuner can pick:
-tile-streamed, sliced ELL, lane groups, XCD column splits of A and of A^T, and the batched
-(block-diagonal) kernels. B = 1 checks the fused steps from the initial centre; B = 3 also
+tile-streamed, sliced ELL, lane groups, XCD column splits of A and of A^T, LDS-staged, and the
+batched (block-diagonal) kernels. B = 1 checks the fused steps from the initial centre; B = 3 also
 crosses the restart (re-centring and the shifted data) that the first check always takes.
 
 The differences come only from summation order (scipy's products against the kernels' fixed
@@ -26,6 +26,8 @@ VARIANTS = {
     "lane_groups_8": dict(lanes_A=8, lanes_At=8, col_splits_A=1, col_splits_At=1),
     "split_A4_At2_streamed": dict(lanes_A=-1, lanes_At=-1, col_splits_A=4, col_splits_At=2),
     "split_A2_At4_groups": dict(lanes_A=2, lanes_At=4, col_splits_A=2, col_splits_At=4),
+    "lds_staged": dict(lanes_A=-5, lanes_At=-5, col_splits_A=1, col_splits_At=1),
+    "lds_staged_w16": dict(lanes_A=-7, lanes_At=-7, col_splits_A=1, col_splits_At=1),
 }
 
 
@@ -65,7 +67,7 @@ def test_single_lp_blocks_equal_model(engine, variant, blocks):
 
 
 @pytest.mark.parametrize("blocks", [1, 3])
-@pytest.mark.parametrize("lanes", [-1, 8])
+@pytest.mark.parametrize("lanes", [-1, 8, -5, -7])
 def test_batch_blocks_equal_model(engine, lanes, blocks):
     """The batched kernels (every instance in one launch, per-instance state) on a batch of
     LPs of different shapes: each instance equals its own model run."""

@@ -252,7 +252,7 @@ int vj_params_set(vj_ctx* ctx, const vj_params* p) {
         set_error("unknown precision");
         return VJ_ERR_INVALID_ARGUMENT;
     }
-    auto lanes_ok = [](int g) { return (g >= -4 && g <= 0) || g == 2 || g == 4 || g == 8 || g == 16 || g == 32; };
+    auto lanes_ok = [](int g) { return (g >= -7 && g <= 0) || g == 2 || g == 4 || g == 8 || g == 16 || g == 32; };
     auto splits_ok = [](int k) { return k == 0 || k == 1 || k == 2 || k == 4 || k == 8; };
     if (!splits_ok(p->col_splits_A) || !splits_ok(p->col_splits_At)) {
         set_error("column splits must be 0 (auto), 1, 2, 4 or 8");
@@ -269,7 +269,8 @@ int vj_params_set(vj_ctx* ctx, const vj_params* p) {
     }
     if (!lanes_ok(p->lanes_A) || !lanes_ok(p->lanes_At)) {
         set_error("lanes per row must be 0 (autotune), -1 (tile-streamed), -2 (tile-streamed, FP32 records), -3 "
-                  "(sliced ELL), -4 (sliced ELL, FP32 records) or one of 2, 4, 8, 16, 32");
+                  "(sliced ELL), -4 (sliced ELL, FP32 records), -5 (LDS-staged), -6 (LDS-staged, FP32 records), -7 "
+                  "(LDS-staged, 16-bit window offsets) or one of 2, 4, 8, 16, 32");
         return VJ_ERR_INVALID_ARGUMENT;
     }
     if (p->max_iterations < 0 || p->check_frequency < 0 || !(p->tolerance >= 0.0) ||

@@ -164,6 +164,8 @@ struct StepCfg {
     bool stream;
     bool rec;   // FP32 (index, value) records in the tile-streamed kernels
     bool ell;   // sliced-ELL kernel (S = 1)
+    bool stage = false;   // LDS-staged kernel (S = 1)
+    bool w16 = false;     // ... reading 16-bit window offsets
 };
 
 // An autotuner decision, kept for the rest of the upload: the same precision and kernel
@@ -220,6 +222,17 @@ struct FastEngine {
     bool streamA = true, streamAt = true, tiles_built = false;
     int ntA = 0, ntAt = 0;
     DevBuf tileA, tileAt, tfA, tlA, tfAt, tlAt;
+    // LDS-staged kernels (batches): row tiles that never span two instances, at most
+    // kStageThreads rows and stage_cap entries (gcap*: the cap they were built for, which
+    // differs between 8-byte and 4-byte iterates), each tile's column window, per instance the
+    // first / last tile, and the sum of the windows' widths (gwin*); stageA / stageAt select them
+    // with every window at most 65,536 columns wide, 16-bit window offsets of the entries
+    // (off16*, built when w16ok*) replace the 4-byte column indices (w16A / w16At select them)
+    // instG*: each staged tile's instance
+    DevBuf tileGA, tileGAt, winGA, winGAt, tfGA, tlGA, tfGAt, tlGAt, off16A, off16At, instGA, instGAt;
+    int ntGA = 0, ntGAt = 0, gcapA = 0, gcapAt = 0;
+    int64_t gwinA = 0, gwinAt = 0;
+    bool stageA = false, stageAt = false, w16okA = false, w16okAt = false, w16A = false, w16At = false;
     int hist_cap = 0;
     hipGraphExec_t gexec = nullptr;
     InstState* h_state[2] = {nullptr, nullptr};   // pinned, h_state_cap entries each
@@ -295,6 +308,31 @@ int launch_primal(vj_ctx* ctx, FastEngine& E, const V* tv, T* x, T* xbar, const 
                                E.lcs.as<T>(), E.ucs.as<T>(), x, xbar, E.xhat.as<double>(), E.xfull.as<double>(),
                                E.st.as<InstState>(), E.part_x.as<double>(), it, flags, E.rho);
         }));
+        VJ_LAUNCHED();
+        return VJ_OK;
+    }
+    if (E.stageAt) {
+        VJ_TRY(require({E.tileGAt.p, E.winGAt.p, E.instGAt.p}, "staged primal_step"));
+        auto go = [&](auto ent) {
+            using Ent = decltype(ent);
+            const dim3 grid(E.ntGAt);
+            if (E.multi)
+                hipLaunchKernelGGL((staged_primal_step<Ent, T, true>), grid, dim3(kStageThreads), 0, ctx->stream,
+                                   E.tileGAt.as<int>(), E.instGAt.as<int>(), E.winGAt.as<int>(), P.tp.as<int>(), ent, y,
+                                   E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), x, xbar, E.xhat.as<double>(),
+                                   E.xfull.as<double>(), E.st.as<InstState>(), E.part_x.as<double>(), it, flags, E.rho);
+            else
+                hipLaunchKernelGGL((staged_primal_step<Ent, T, false>), grid, dim3(kStageThreads), 0, ctx->stream,
+                                   E.tileGAt.as<int>(), E.instGAt.as<int>(), E.winGAt.as<int>(), P.tp.as<int>(), ent, y,
+                                   E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), x, xbar, E.xhat.as<double>(),
+                                   E.xfull.as<double>(), E.st.as<InstState>(), E.part_x.as<double>(), it, flags, E.rho);
+        };
+        if (E.w16At) {
+            VJ_TRY(require({E.off16At.p}, "staged primal_step (16-bit offsets)"));
+            go(Win16Entries<V>{E.off16At.as<uint16_t>(), tv});
+        } else {
+            VJ_TRY(with_entries<V>(E.recAt, E.recAt_buf, P.ti.as<int>(), tv, "staged records", go));
+        }
         VJ_LAUNCHED();
         return VJ_OK;
     }
@@ -388,6 +426,31 @@ int launch_dual(vj_ctx* ctx, FastEngine& E, const V* av, const T* xbar, T* y, in
                                lrs, urs, y, E.yhat.as<double>(), E.yfull.as<double>(),
                                E.st.as<InstState>(), E.part_y.as<double>(), it, flags, E.rho);
         }));
+        VJ_LAUNCHED();
+        return VJ_OK;
+    }
+    if (E.stageA) {
+        VJ_TRY(require({E.tileGA.p, E.winGA.p, E.instGA.p}, "staged dual_step"));
+        auto go = [&](auto ent) {
+            using Ent = decltype(ent);
+            const dim3 grid(E.ntGA);
+            if (E.multi)
+                hipLaunchKernelGGL((staged_dual_step<Ent, T, true>), grid, dim3(kStageThreads), 0, ctx->stream,
+                                   E.tileGA.as<int>(), E.instGA.as<int>(), E.winGA.as<int>(), P.rp.as<int>(), ent, xbar,
+                                   yht, lrs, urs, y, E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(),
+                                   E.part_y.as<double>(), it, flags, E.rho);
+            else
+                hipLaunchKernelGGL((staged_dual_step<Ent, T, false>), grid, dim3(kStageThreads), 0, ctx->stream,
+                                   E.tileGA.as<int>(), E.instGA.as<int>(), E.winGA.as<int>(), P.rp.as<int>(), ent, xbar,
+                                   yht, lrs, urs, y, E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(),
+                                   E.part_y.as<double>(), it, flags, E.rho);
+        };
+        if (E.w16A) {
+            VJ_TRY(require({E.off16A.p}, "staged dual_step (16-bit offsets)"));
+            go(Win16Entries<V>{E.off16A.as<uint16_t>(), av});
+        } else {
+            VJ_TRY(with_entries<V>(E.recA, E.recA_buf, P.ci.as<int>(), av, "staged records", go));
+        }
         VJ_LAUNCHED();
         return VJ_OK;
     }
@@ -515,6 +578,49 @@ int choose_splits(int64_t vec_len) {
     return S;
 }
 
+// Row tiles of the LDS-staged kernels, built on the device once per upload and entry size:
+// for batches (the kernels are autotuner candidates there), and for a single LP only when
+// the caller forces them (lanes -5 / -6): a single LP's tiles read windows about as wide as
+// the whole vector, which only a small LP can afford.
+int setup_stage(vj_ctx* ctx, FastEngine& E) {
+    Problem& P = ctx->prob;
+    const vj_params& prm = ctx->params;
+    hipStream_t s = ctx->stream;
+    const int cap = E.prec == VJ_PREC_FP64 ? stage_cap<double>() : stage_cap<float>();
+    const int* rinst = E.multi ? P.row_inst.as<int>() : nullptr;
+    const int* cinst = E.multi ? P.col_inst.as<int>() : nullptr;
+    const int* roff = E.multi ? P.d_row_off.as<int>() : nullptr;
+    const int* coff = E.multi ? P.d_col_off.as<int>() : nullptr;
+    if ((E.multi || prm.lanes_A <= -5) && E.gcapA != cap) {
+        int widest = 0;
+        VJ_TRY(device_row_tiles(s, E.m, 1, P.rp.as<int>(), 0, cap, kStageThreads, E.tileGA, &E.ntGA, rinst, roff));
+        VJ_TRY(device_tile_windows(s, E.ntGA, E.tileGA.as<int>(), P.rp.as<int>(), P.ci.as<int>(), E.winGA, &E.gwinA,
+                                   &widest));
+        VJ_TRY(device_instance_tiles(s, E.count, E.ntGA, E.tileGA.as<int>(), P.d_row_off.as<int>(), E.tfGA, E.tlGA));
+        E.w16okA = widest <= 65536;
+        if (E.w16okA)
+            VJ_TRY(device_window_offsets(s, E.ntGA, E.tileGA.as<int>(), P.rp.as<int>(), P.ci.as<int>(),
+                                         E.winGA.as<int>(), E.nnz, E.off16A));
+        VJ_TRY(device_tile_instances(s, E.ntGA, E.tileGA.as<int>(), rinst, E.instGA));
+        E.gcapA = cap;
+    }
+    if ((E.multi || prm.lanes_At <= -5) && E.gcapAt != cap) {
+        int widest = 0;
+        VJ_TRY(device_row_tiles(s, E.n, 1, P.tp.as<int>(), 0, cap, kStageThreads, E.tileGAt, &E.ntGAt, cinst, coff));
+        VJ_TRY(device_tile_windows(s, E.ntGAt, E.tileGAt.as<int>(), P.tp.as<int>(), P.ti.as<int>(), E.winGAt,
+                                   &E.gwinAt, &widest));
+        VJ_TRY(device_instance_tiles(s, E.count, E.ntGAt, E.tileGAt.as<int>(), P.d_col_off.as<int>(), E.tfGAt,
+                                     E.tlGAt));
+        E.w16okAt = widest <= 65536;
+        if (E.w16okAt)
+            VJ_TRY(device_window_offsets(s, E.ntGAt, E.tileGAt.as<int>(), P.tp.as<int>(), P.ti.as<int>(),
+                                         E.winGAt.as<int>(), E.nnz, E.off16At));
+        VJ_TRY(device_tile_instances(s, E.ntGAt, E.tileGAt.as<int>(), cinst, E.instGAt));
+        E.gcapAt = cap;
+    }
+    return VJ_OK;
+}
+
 // ---- setup: scaling, step sizes, initial centre -------------------------------------
 int setup(vj_ctx* ctx, FastEngine& E) {
     Problem& P = ctx->prob;
@@ -542,8 +648,9 @@ int setup(vj_ctx* ctx, FastEngine& E) {
         VJ_TRY(device_instance_tiles(s, cnt, E.ntAt, E.tileAt.as<int>(), P.d_col_off.as<int>(), E.tfAt, E.tlAt));
         E.tiles_built = true;
     }
-    const int px_slots = std::max(partial_slots(n, cnt), E.ntAt + cnt + 1);
-    const int py_slots = std::max(partial_slots(m, cnt), E.ntA + cnt + 1);
+    VJ_TRY(setup_stage(ctx, E));
+    const int px_slots = std::max({partial_slots(n, cnt), E.ntAt + cnt + 1, E.ntGAt + cnt + 1});
+    const int py_slots = std::max({partial_slots(m, cnt), E.ntA + cnt + 1, E.ntGA + cnt + 1});
     VJ_TRY(E.part_x.alloc(static_cast<size_t>(px_slots) * 3 * 8));
     VJ_TRY(E.part_y.alloc(static_cast<size_t>(py_slots) * 3 * 8));
     VJ_TRY(E.part_kr.alloc(static_cast<size_t>(py_slots) * 3 * 8));
@@ -870,10 +977,10 @@ int enqueue_block(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     a.py = E.part_y.as<double>();
     a.Ry = E.SA > 1 ? kBlock * finish_rows<T>() : (E.ellA ? kBlock : kBlock / E.GA);
     const bool sx = E.SAt == 1 && E.streamAt && !E.ellAt, sy = E.SA == 1 && E.streamA && !E.ellA;
-    a.xt_first = sx ? E.tfAt.as<int>() : nullptr;
-    a.xt_last = sx ? E.tlAt.as<int>() : nullptr;
-    a.yt_first = sy ? E.tfA.as<int>() : nullptr;
-    a.yt_last = sy ? E.tlA.as<int>() : nullptr;
+    a.xt_first = E.stageAt ? E.tfGAt.as<int>() : (sx ? E.tfAt.as<int>() : nullptr);
+    a.xt_last = E.stageAt ? E.tlGAt.as<int>() : (sx ? E.tlAt.as<int>() : nullptr);
+    a.yt_first = E.stageA ? E.tfGA.as<int>() : (sy ? E.tfA.as<int>() : nullptr);
+    a.yt_last = E.stageA ? E.tlGA.as<int>() : (sy ? E.tlA.as<int>() : nullptr);
     a.kkt = E.kkt8.as<double>();
     a.row_off = ctx->prob.d_row_off.as<int>();
     a.col_off = ctx->prob.d_col_off.as<int>();
@@ -942,10 +1049,12 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     // forced: the caller's col_splits (0 = free choice, 1 = unsplit only, >1 = split only)
     const bool f32 = E.prec == VJ_PREC_FP32;
     // ell: the sliced-ELL copy exists (single LP, little padding, or forced by lanes -3 / -4)
-    auto base = [&](bool tune, int lanes, double avg, int Sm, int forced, bool ell) {
+    auto base = [&](bool tune, int lanes, double avg, int Sm, int forced, bool ell, bool stage, bool w16) {
         std::vector<StepCfg> c;
         if (!tune) {
-            if (lanes <= -3 && ell) c.push_back({1, choose_lanes(avg), false, lanes == -4 && f32, true});
+            if (lanes == -7 && w16) c.push_back(StepCfg{1, choose_lanes(avg), false, false, false, true, true});
+            else if (lanes <= -5) c.push_back(StepCfg{1, choose_lanes(avg), false, lanes == -6 && f32, false, true});
+            else if (lanes <= -3 && ell) c.push_back({1, choose_lanes(avg), false, lanes == -4 && f32, true});
             else if (lanes < 0) c.push_back({Sm, Sm > 1 ? 0 : choose_lanes(avg), Sm == 1, (lanes == -2 || lanes == -4) && f32, false});
             else c.push_back({Sm, lanes, false, false, false});
             return c;
@@ -970,22 +1079,37 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
             }
             if (f32) c.push_back({1, choose_lanes(avg), true, true, false});
             c.push_back({1, choose_lanes(avg), true, false, false});
+            if (stage) {   // LDS-staged (batches whose tiles read narrow windows)
+                if (w16) c.push_back(StepCfg{1, choose_lanes(avg), false, false, false, true, true});
+                if (f32) c.push_back(StepCfg{1, choose_lanes(avg), false, true, false, true});
+                c.push_back(StepCfg{1, choose_lanes(avg), false, false, false, true});
+            }
             for (int g : {4, 8, 16, 32}) c.push_back({1, g, false, false, false});
         }
         return c;
     };
-    std::vector<StepCfg> cA = base(tuneA, prm.lanes_A, avgA, SmA, prm.col_splits_A, E.ellLenA > 0);
-    std::vector<StepCfg> cAt = base(tuneAt, prm.lanes_At, avgAt, SmAt, prm.col_splits_At, E.ellLenAt > 0);
-    auto apply = [](const StepCfg& c, int& S, int& G, int& Gs, bool& stream, bool& rec, bool& ell) {
+    // the staged kernels are candidates where their tiles exist (batches) and the windows they
+    // copy to LDS add up to at most kStageMaxWindow times the entries they serve
+    constexpr int64_t kStageMaxWindow = 8;
+    const int scap = E.prec == VJ_PREC_FP64 ? stage_cap<double>() : stage_cap<float>();
+    const bool stgA = E.gcapA == scap && E.gwinA <= kStageMaxWindow * E.nnz;
+    const bool stgAt = E.gcapAt == scap && E.gwinAt <= kStageMaxWindow * E.nnz;
+    std::vector<StepCfg> cA = base(tuneA, prm.lanes_A, avgA, SmA, prm.col_splits_A, E.ellLenA > 0, stgA, E.w16okA);
+    std::vector<StepCfg> cAt = base(tuneAt, prm.lanes_At, avgAt, SmAt, prm.col_splits_At, E.ellLenAt > 0, stgAt,
+                                    E.w16okAt);
+    auto apply = [](const StepCfg& c, int& S, int& G, int& Gs, bool& stream, bool& rec, bool& ell, bool& stage,
+                    bool& w16) {
         S = c.S;
         stream = c.S == 1 && c.stream;
         rec = c.rec;
         ell = c.ell;
+        stage = c.stage;
+        w16 = c.w16;
         if (c.S > 1) Gs = c.G;
         else G = std::max(c.G, 2);
     };
-    apply(cA[0], E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA);
-    apply(cAt[0], E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt);
+    apply(cA[0], E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA, E.stageA, E.w16A);
+    apply(cAt[0], E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt, E.stageAt, E.w16At);
     E.K_tuned = 0;
     if (E.nnz < 200000 || (cA.size() == 1 && cAt.size() == 1))
         return VJ_OK;  // launch-bound sizes: the first candidate is as good as timing
@@ -993,8 +1117,8 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
         if (d.prec == E.prec && d.lanes_A == prm.lanes_A && d.lanes_At == prm.lanes_At &&
             d.col_splits_A == prm.col_splits_A && d.col_splits_At == prm.col_splits_At &&
             d.check_frequency == prm.check_frequency) {
-            apply(d.a, E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA);
-            apply(d.at, E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt);
+            apply(d.a, E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA, E.stageA, E.w16A);
+            apply(d.at, E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt, E.stageAt, E.w16At);
             E.GA = d.GA;
             E.GAt = d.GAt;
             E.K_tuned = d.K_tuned;
@@ -1010,26 +1134,27 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     {   // one untimed pass first: the first candidate is otherwise timed on colder clocks and
         // caches than the rest, which biases the comparison against it
         double up = 0, ud = 0;
-        apply(cA[0], E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA);
-        apply(cAt[0], E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt);
+        apply(cA[0], E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA, E.stageA, E.w16A);
+        apply(cAt[0], E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt, E.stageAt, E.w16At);
         VJ_TRY((time_kernels<V, T>(ctx, E, av, tv, kTuneReps, &up, &ud)));
     }
     for (size_t r = 0; r < rounds; ++r) {
         const StepCfg& a = cA[std::min(r, cA.size() - 1)];
         const StepCfg& b = cAt[std::min(r, cAt.size() - 1)];
-        apply(a, E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA);
-        apply(b, E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt);
+        apply(a, E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA, E.stageA, E.w16A);
+        apply(b, E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt, E.stageAt, E.w16At);
         double up = 0, ud = 0;
         VJ_TRY((time_kernels<V, T>(ctx, E, av, tv, kTuneReps, &up, &ud)));
         if (prm.verbose >= 2)
-            std::fprintf(stderr, "[autotune] A: S=%d G=%d stream=%d rec=%d ell=%d %.1f us | A^T: S=%d G=%d stream=%d rec=%d ell=%d %.1f us\n",
-                         a.S, a.G, a.stream ? 1 : 0, a.rec ? 1 : 0, a.ell ? 1 : 0, ud, b.S, b.G, b.stream ? 1 : 0,
-                         b.rec ? 1 : 0, b.ell ? 1 : 0, up);
+            std::fprintf(stderr, "[autotune] A: S=%d G=%d stream=%d rec=%d ell=%d stage=%d w16=%d %.1f us | A^T: S=%d "
+                         "G=%d stream=%d rec=%d ell=%d stage=%d w16=%d %.1f us\n",
+                         a.S, a.G, a.stream ? 1 : 0, a.rec ? 1 : 0, a.ell ? 1 : 0, a.stage ? 1 : 0, a.w16 ? 1 : 0, ud,
+                         b.S, b.G, b.stream ? 1 : 0, b.rec ? 1 : 0, b.ell ? 1 : 0, b.stage ? 1 : 0, b.w16 ? 1 : 0, up);
         if (r == 0 || (r < cA.size() && ud < (1.0 - kTuneMargin) * bestA)) { bestA = ud; wA = a; }
         if (r == 0 || (r < cAt.size() && up < (1.0 - kTuneMargin) * bestAt)) { bestAt = up; wAt = b; }
     }
-    apply(wA, E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA);
-    apply(wAt, E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt);
+    apply(wA, E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA, E.stageA, E.w16A);
+    apply(wAt, E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt, E.stageAt, E.w16At);
     // The FP64 check passes run lane-group kernels with E.GA / E.GAt lanes per row. A split
     // winner leaves them at the last candidate timed: give them the width that suits the rows.
     if (E.SA > 1 || E.ellA) E.GA = std::max(choose_lanes(avgA), 2);
@@ -1206,9 +1331,12 @@ int run_typed(vj_ctx* ctx, FastEngine& E, vj_result* out, clk::time_point t_star
         r.time_to_target = conv_time[q];
         r.host_device_transfers = transfers;
         r.refinements = E.prec == VJ_PREC_FP64 ? 0 : z.restarts;
-        // 0: tile-streamed kernel, -3 / -4: sliced ELL (separate arrays / FP32 records)
-        r.lanes_A = E.ellA ? (E.recA ? -4 : -3) : (E.SA > 1 ? E.GsA : (E.streamA ? 0 : E.GA));
-        r.lanes_At = E.ellAt ? (E.recAt ? -4 : -3) : (E.SAt > 1 ? E.GsAt : (E.streamAt ? 0 : E.GAt));
+        // 0: tile-streamed kernel, -3 / -4: sliced ELL, -5 / -6: LDS-staged (separate arrays /
+        // FP32 records), -7: LDS-staged with 16-bit window offsets
+        r.lanes_A = E.stageA ? (E.w16A ? -7 : (E.recA ? -6 : -5))
+                             : (E.ellA ? (E.recA ? -4 : -3) : (E.SA > 1 ? E.GsA : (E.streamA ? 0 : E.GA)));
+        r.lanes_At = E.stageAt ? (E.w16At ? -7 : (E.recAt ? -6 : -5))
+                               : (E.ellAt ? (E.recAt ? -4 : -3) : (E.SAt > 1 ? E.GsAt : (E.streamAt ? 0 : E.GAt)));
         r.col_splits_A = E.SA;
         r.col_splits_At = E.SAt;
         r.check_interval = E.K;
@@ -1424,11 +1552,13 @@ int fast_benchmark(vj_ctx* ctx, int reps, vj_kernel_stat* out, int cap, int* wri
     if (E.SAt > 1 && E.GsAt == 0) std::snprintf(names[0], 48, "split_stream_partials+primal_finish<%s%s>", pt, rt);
     else if (E.SAt > 1) std::snprintf(names[0], 48, "split_partials+primal_finish<%s,G=%d>", pt, E.GsAt);
     else if (E.ellAt) std::snprintf(names[0], 48, "ell_primal_step<%s%s>", pt, rt);
+    else if (E.stageAt) std::snprintf(names[0], 48, "staged_primal_step<%s%s>", pt, E.w16At ? ",w16" : rt);
     else if (E.streamAt) std::snprintf(names[0], 48, "stream_primal_step<%s%s>", pt, rt);
     else std::snprintf(names[0], 48, "primal_step<%s,G=%d>", pt, E.GAt);
     if (E.SA > 1 && E.GsA == 0) std::snprintf(names[1], 48, "split_stream_partials+dual_finish<%s%s>", pt, ra);
     else if (E.SA > 1) std::snprintf(names[1], 48, "split_partials+dual_finish<%s,G=%d>", pt, E.GsA);
     else if (E.ellA) std::snprintf(names[1], 48, "ell_dual_step<%s%s>", pt, ra);
+    else if (E.stageA) std::snprintf(names[1], 48, "staged_dual_step<%s%s>", pt, E.w16A ? ",w16" : ra);
     else if (E.streamA) std::snprintf(names[1], 48, "stream_dual_step<%s%s>", pt, ra);
     else std::snprintf(names[1], 48, "dual_step<%s,G=%d>", pt, E.GA);
     const double us[2] = {up, ud};

@@ -407,6 +407,24 @@ struct RecEntries {
     }
 };
 
+// Entries of the LDS-staged kernels with 16-bit column offsets from their tile's window start
+// (windows up to 65,536 columns): 2 bytes of index per entry instead of 4. The kernels
+// subtract the window start from their chunk bases instead (window_relative below).
+template <class V>
+struct Win16Entries {
+    const uint16_t* __restrict__ off;
+    const V* __restrict__ val;
+    template <class T>
+    __device__ __forceinline__ void load(int64_t k, int& c, T& a) const {
+        c = static_cast<int>(__builtin_nontemporal_load(off + k));
+        a = static_cast<T>(load_val(val, k));
+    }
+};
+template <class Ent>
+struct window_relative { static constexpr bool value = false; };
+template <class V>
+struct window_relative<Win16Entries<V>> { static constexpr bool value = true; };
+
 // Loads in flight per lane in one pass of the streamed SpMV loops: every lane issues all of
 // its entry loads of a pass, then all of its gathers. Tiles hold a few entries per lane
 // (about 3 per lane and column block on the 1M x 2M LP), so issuing them together, rather
@@ -579,6 +597,215 @@ __global__ __launch_bounds__(kBlock) void stream_dual_step(
     if (flags) {
         const int mask = ((flags & 1) ? 1 : 0) | ((flags & 2) ? 6 : 0);
         write_tile_partials<3, MULTI>(c, my, f, l, tile, part, mask);
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// LDS-staged fused steps, for batches of small LPs. A batched tile-streamed step pays one L2
+// line per nonzero for its gathers, and that line traffic, not HBM, sets its speed. Here every
+// tile holds rows of a single instance (at most kStageThreads rows and stage_cap<T>() entries,
+// built on the device) and carries the window [lo, hi) of columns its entries touch. The
+// workgroup copies the window of the gathered vector into LDS with coalesced loads (chunk by
+// chunk when it is wider than the buffer) and each entry then reads its vector element from
+// LDS. A lane keeps its stage_depth<T>() entries in registers across the chunks; every entry
+// is multiplied in exactly one chunk. The products then go to the same buffer by entry
+// position and one lane per row adds its segment in index order: the row sums are those of
+// the tile-streamed kernels, bit for bit. 64 KB of LDS per workgroup leaves room for two
+// workgroups (32 waves) per CU, which the kernels keep by asking for 8 waves per SIMD (at most
+// 64 VGPRs: the first build, with predicated entry loads, spilled to scratch and ran 9x slower).
+constexpr int kStageThreads = 1024;
+constexpr int kStageBytes = 65536;
+template <class T>
+constexpr int stage_cap() { return kStageBytes / static_cast<int>(sizeof(T)); }
+template <class T>
+constexpr int stage_depth() { return stage_cap<T>() / kStageThreads; }
+
+// lds[q] = vec[base + q] for q < len, with direct global-to-LDS loads (global_load_lds_dword):
+// wave w copies 4-byte words q0 .. q0+63 (q0 = 64 w, 64 w + 1024, ...), lane l word q0 + l, into
+// LDS at the wave-uniform base lds + q0. The loads occupy no VGPRs, so every word of a lane is
+// in flight at once (a copy through registers, four loads at a time, waited for each group);
+// the caller's barrier waits for them (vmcnt). Lanes past the end re-read the last word and
+// write it to LDS words past len, which no entry reads (the buffer holds whole waves' words).
+template <class T>
+__device__ __forceinline__ void stage_window(const T* __restrict__ vec, int base, int len, T* __restrict__ lds) {
+    constexpr int W = static_cast<int>(sizeof(T) / 4);
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(vec) + static_cast<int64_t>(base) * W;
+    uint32_t* dst = reinterpret_cast<uint32_t*>(lds);
+    const int words = len * W;
+    const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
+    for (int q0 = static_cast<int>(threadIdx.x) - lane; q0 < words; q0 += kStageThreads) {
+        const int q = q0 + lane < words ? q0 + lane : words - 1;
+        __builtin_amdgcn_global_load_lds(src + q, dst + q0, 4, 0, 0);
+    }
+}
+
+// Products of the tile's entries [s0, s1) (at most stage_cap<T>()) with vec, whose elements
+// the entries read in the window [wlo, whi); on return lds[p - s0] = a_p * vec[col_p]. Entry
+// columns are absolute, or relative to wlo for the 16-bit offset format.
+template <class Ent, class T>
+__device__ __forceinline__ void staged_products(int s0, int s1, int wlo, int whi, const Ent& ent,
+                                                const T* __restrict__ vec, T* __restrict__ lds) {
+    constexpr int U = stage_depth<T>();
+    constexpr int CAP = stage_cap<T>();
+    if (s1 <= s0) return;   // no entries (uniform over the workgroup)
+    int c[U];
+    T a[U];
+    // every lane loads all U slots from an in-bounds position (the tile's last entry for the
+    // slots past its end) and marks the extra ones with column -1, whose offset from any chunk
+    // base is out of range: unconditional loads keep the slots in plain registers
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int p = s0 + static_cast<int>(threadIdx.x) + u * kStageThreads;
+        int cc;
+        T aa;
+        ent.load(p < s1 ? p : s1 - 1, cc, aa);
+        c[u] = p < s1 ? cc : -1;
+        a[u] = aa;
+    }
+    for (int base = wlo; base < whi; base += CAP) {
+        const int len = whi - base < CAP ? whi - base : CAP;
+        if (base != wlo) __syncthreads();   // the previous chunk has been read
+        stage_window<T>(vec, base, len, lds);
+        __syncthreads();
+        const int cb = window_relative<Ent>::value ? base - wlo : base;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const unsigned d = static_cast<unsigned>(c[u] - cb);
+            if (d < static_cast<unsigned>(len)) a[u] = a[u] * lds[d];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int p = s0 + static_cast<int>(threadIdx.x) + u * kStageThreads;
+        if (p < s1) lds[p - s0] = a[u];
+    }
+    __syncthreads();
+}
+
+// Whole-workgroup sum of one row longer than the buffer (gathers from memory; fixed order).
+template <class Ent, class T>
+__device__ __forceinline__ T staged_long_row(int s0, int s1, const Ent& ent, const T* __restrict__ vec,
+                                             T* __restrict__ lds) {
+    T acc = 0;
+    for (int p = s0 + static_cast<int>(threadIdx.x); p < s1; p += kStageThreads) {
+        int c;
+        T a;
+        ent.load(p, c, a);
+        acc += a * vec[c];
+    }
+    lds[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kStageThreads / 2; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) lds[threadIdx.x] += lds[threadIdx.x + w];
+        __syncthreads();
+    }
+    const T r = lds[0];
+    __syncthreads();
+    return r;
+}
+
+// The tile's residual partials (one instance per tile) to part[slot * 3 + q].
+__device__ __forceinline__ void stage_partials(const double (&v)[3], int slot, double* __restrict__ part, int mask,
+                                               double (*red)[3]) {
+    double t[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) t[q] = group_sum<kWave>(v[q]);
+    const int wave = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) red[wave][q] = t[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            double s = red[0][q];
+            for (int w = 1; w < kStageThreads / kWave; ++w) s += red[w][q];
+            if ((mask >> q) & 1) part[static_cast<int64_t>(slot) * 3 + q] = s;
+        }
+    }
+}
+
+template <class Ent, class T, bool MULTI>
+__global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8))) void staged_primal_step(
+    const int* __restrict__ tile_row, const int* __restrict__ tile_inst, const int* __restrict__ win,
+    const int* __restrict__ tp, Ent ent, const T* __restrict__ y, const T* __restrict__ dhat,
+    const T* __restrict__ lcs, const T* __restrict__ ucs, T* __restrict__ x, T* __restrict__ xbar,
+    const double* __restrict__ xhat, double* __restrict__ xfull, const InstState* __restrict__ st,
+    double* __restrict__ part, int it, int flags, double rho) {
+    __shared__ T lds[stage_cap<T>()];
+    __shared__ double red[kStageThreads / kWave][3];
+    const int tile = blockIdx.x;
+    const int my = MULTI ? tile_inst[tile] : 0;   // every row of the tile belongs to this instance
+    if (st[my].active == 0) return;
+    const int r0 = tile_row[tile], r1 = tile_row[tile + 1];
+    const int s0 = tp[r0], s1 = tp[r1];
+    double c[3] = {0.0, 0.0, 0.0};
+    if (s1 - s0 > stage_cap<T>()) {   // one long row
+        const T acc = staged_long_row<Ent, T>(s0, s1, ent, y, lds);
+        if (threadIdx.x == 0) primal_epilogue<T>(acc, r0, st[my], it, flags, rho, dhat, lcs, ucs, x, xbar, xhat, xfull, c);
+    } else {
+        const int j = r0 + static_cast<int>(threadIdx.x);
+        const bool mine = j < r1;
+        PrimalOps<T> o{};
+        int q0 = 0, q1 = 0;
+        if (mine) {   // the row's operands and bounds are loaded before the staging
+            o = primal_ops<T>(j, x, xbar, dhat, lcs, ucs, st[my], it, rho);
+            q0 = tp[j] - s0;
+            q1 = tp[j + 1] - s0;
+        }
+        staged_products<Ent, T>(s0, s1, win[2 * tile], win[2 * tile + 1], ent, y, lds);
+        if (mine) {
+            T acc = 0;
+            for (int q = q0; q < q1; ++q) acc += lds[q];
+            primal_update<T>(acc, o, j, st[my], it, flags, rho, x, xbar, xhat, xfull, c);
+        }
+    }
+    if (flags) {
+        const int mask = ((flags & 1) ? 1 : 0) | ((flags & 2) ? 6 : 0);
+        stage_partials(c, tile + my, part, mask, red);
+    }
+}
+
+template <class Ent, class T, bool MULTI>
+__global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8))) void staged_dual_step(
+    const int* __restrict__ tile_row, const int* __restrict__ tile_inst, const int* __restrict__ win,
+    const int* __restrict__ rp, Ent ent, const T* __restrict__ xbar, const T* __restrict__ yhat_t,
+    const T* __restrict__ lrs, const T* __restrict__ urs, T* __restrict__ y, const double* __restrict__ yhat,
+    double* __restrict__ yfull, const InstState* __restrict__ st, double* __restrict__ part, int it, int flags,
+    double rho) {
+    __shared__ T lds[stage_cap<T>()];
+    __shared__ double red[kStageThreads / kWave][3];
+    const int tile = blockIdx.x;
+    const int my = MULTI ? tile_inst[tile] : 0;
+    if (st[my].active == 0) return;
+    const int r0 = tile_row[tile], r1 = tile_row[tile + 1];
+    const int s0 = rp[r0], s1 = rp[r1];
+    double c[3] = {0.0, 0.0, 0.0};
+    if (s1 - s0 > stage_cap<T>()) {
+        const T w = staged_long_row<Ent, T>(s0, s1, ent, xbar, lds);
+        if (threadIdx.x == 0) dual_epilogue<T>(w, r0, st[my], it, flags, rho, yhat_t, lrs, urs, y, yhat, yfull, c);
+    } else {
+        const int i = r0 + static_cast<int>(threadIdx.x);
+        const bool mine = i < r1;
+        DualOps<T> o{};
+        int q0 = 0, q1 = 0;
+        if (mine) {
+            o = dual_ops<T>(i, y, yhat_t, lrs, urs);
+            q0 = rp[i] - s0;
+            q1 = rp[i + 1] - s0;
+        }
+        staged_products<Ent, T>(s0, s1, win[2 * tile], win[2 * tile + 1], ent, xbar, lds);
+        if (mine) {
+            T w = 0;
+            for (int q = q0; q < q1; ++q) w += lds[q];
+            dual_update<T>(w, o, i, st[my], it, flags, rho, y, yhat, yfull, c);
+        }
+    }
+    if (flags) {
+        const int mask = ((flags & 1) ? 1 : 0) | ((flags & 2) ? 6 : 0);
+        stage_partials(c, tile + my, part, mask, red);
     }
 }
 

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <vector>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
@@ -58,9 +59,10 @@ __global__ void gather_transposed(int64_t nnz, const int* __restrict__ perm, con
 
 // next[r]: the end of a tile starting at row r: the largest e in [r+1, min(rows, r+max_rows)]
 // with ptr_k[e] - ptr_k[r] <= cap in every one of the S pointer arrays (ptr_k = ptrs +
-// k*stride), or r+1 when row r alone exceeds cap in some array (a tile of its own).
+// k*stride), or r+1 when row r alone exceeds cap in some array (a tile of its own). With
+// inst_of set, a tile also ends where the instance of row r ends (off[inst_of[r] + 1]).
 __global__ void tile_next(int rows, int S, const int* __restrict__ ptrs, int64_t stride, int cap, int max_rows,
-                          int* __restrict__ next) {
+                          const int* __restrict__ inst_of, const int* __restrict__ off, int* __restrict__ next) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= rows) return;
     auto fits = [&](int e) {
@@ -71,6 +73,10 @@ __global__ void tile_next(int rows, int S, const int* __restrict__ ptrs, int64_t
         return true;
     };
     int lo = r + 1, hi = r + max_rows < rows ? r + max_rows : rows;   // answer in [lo, hi]
+    if (inst_of) {
+        const int end = off[inst_of[r] + 1];
+        hi = end < hi ? end : hi;
+    }
     if (!fits(lo)) {
         next[r] = r + 1;
         return;
@@ -110,6 +116,61 @@ __global__ void instance_tiles(int count, int ntiles, const int* __restrict__ ti
     };
     first[q] = tile_of(off[q]);
     last[q] = tile_of(off[q + 1] - 1);
+}
+
+// win[2t], win[2t+1]: the least column index of tile t's entries and one past the greatest
+// (0, 0 for a tile without entries). One workgroup per tile.
+__global__ __launch_bounds__(kBlock) void tile_windows(int ntiles, const int* __restrict__ tile_row,
+                                                       const int* __restrict__ ptr, const int* __restrict__ idx,
+                                                       int* __restrict__ win) {
+    __shared__ int lo_s[kBlock / kWave], hi_s[kBlock / kWave];
+    const int t = blockIdx.x;
+    if (t >= ntiles) return;
+    const int s0 = ptr[tile_row[t]], s1 = ptr[tile_row[t + 1]];
+    int lo = 2147483647, hi = -1;
+    for (int p = s0 + static_cast<int>(threadIdx.x); p < s1; p += kBlock) {
+        const int c = idx[p];
+        lo = c < lo ? c : lo;
+        hi = c > hi ? c : hi;
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const int a = __shfl_xor(lo, o, kWave), b = __shfl_xor(hi, o, kWave);
+        lo = a < lo ? a : lo;
+        hi = b > hi ? b : hi;
+    }
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        lo_s[w] = lo;
+        hi_s[w] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBlock / kWave; ++k) {
+            lo = lo_s[k] < lo ? lo_s[k] : lo;
+            hi = hi_s[k] > hi ? hi_s[k] : hi;
+        }
+        win[2 * t] = hi < 0 ? 0 : lo;
+        win[2 * t + 1] = hi < 0 ? 0 : hi + 1;
+    }
+}
+
+// off16[p] = idx[p] - win[2t] for the entries p of tile t (one workgroup per tile; every
+// window is at most 65,536 columns wide).
+__global__ __launch_bounds__(kBlock) void window_offsets(int ntiles, const int* __restrict__ tile_row,
+                                                         const int* __restrict__ ptr, const int* __restrict__ idx,
+                                                         const int* __restrict__ win, uint16_t* __restrict__ off16) {
+    const int t = blockIdx.x;
+    if (t >= ntiles) return;
+    const int s0 = ptr[tile_row[t]], s1 = ptr[tile_row[t + 1]];
+    const int lo = win[2 * t];
+    for (int p = s0 + static_cast<int>(threadIdx.x); p < s1; p += kBlock) off16[p] = static_cast<uint16_t>(idx[p] - lo);
+}
+
+// inst[t] = the instance of tile t's first row (0 without an instance map).
+__global__ void tile_instances(int ntiles, const int* __restrict__ tile_row, const int* __restrict__ inst_of,
+                               int* __restrict__ inst) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ntiles) inst[t] = inst_of ? inst_of[tile_row[t]] : 0;
 }
 
 __global__ void tile_bases(int ntiles, int S, const int* __restrict__ tile_row, const int* __restrict__ ptrs,
@@ -311,13 +372,13 @@ int device_transpose(hipStream_t s, int in_rows, int out_rows, int64_t nnz, cons
 }
 
 int device_row_tiles(hipStream_t s, int rows, int S, const int* ptrs, int64_t stride, int cap, int max_rows,
-                     DevBuf& tile_row, int* ntiles) {
+                     DevBuf& tile_row, int* ntiles, const int* inst_of, const int* off) {
     DevBuf next, tmp, cnt;
     VJ_TRY(next.alloc(static_cast<size_t>(rows) * 4));
     VJ_TRY(tmp.alloc((static_cast<size_t>(rows) + 1) * 4));
     VJ_TRY(cnt.alloc(4));
     hipLaunchKernelGGL(tile_next, dim3(grid_for(rows, kBlock)), dim3(kBlock), 0, s, rows, S, ptrs, stride, cap, max_rows,
-                       next.as<int>());
+                       inst_of, off, next.as<int>());
     hipLaunchKernelGGL(tile_chain, dim3(1), dim3(1), 0, s, rows, next.as<int>(), tmp.as<int>(), cnt.as<int>());
     VJ_LAUNCHED();
     int t = 0;
@@ -327,6 +388,48 @@ int device_row_tiles(hipStream_t s, int rows, int S, const int* ptrs, int64_t st
     VJ_HIP(hipMemcpyAsync(tile_row.p, tmp.p, (static_cast<size_t>(t) + 1) * 4, hipMemcpyDeviceToDevice, s));
     VJ_HIP(hipStreamSynchronize(s));
     *ntiles = t;
+    return VJ_OK;
+}
+
+int device_tile_windows(hipStream_t s, int ntiles, const int* tile_row, const int* ptr, const int* idx, DevBuf& win,
+                        int64_t* total, int* widest) {
+    VJ_TRY(win.alloc(std::max<size_t>(static_cast<size_t>(ntiles), 1) * 2 * 4));
+    if (ntiles > 0) {
+        hipLaunchKernelGGL(tile_windows, dim3(ntiles), dim3(kBlock), 0, s, ntiles, tile_row, ptr, idx, win.as<int>());
+        VJ_LAUNCHED();
+    }
+    std::vector<int> h(static_cast<size_t>(ntiles) * 2);
+    if (ntiles > 0) VJ_HIP(hipMemcpyAsync(h.data(), win.p, h.size() * 4, hipMemcpyDeviceToHost, s));
+    VJ_HIP(hipStreamSynchronize(s));
+    int64_t sum = 0;
+    int w = 0;
+    for (int t = 0; t < ntiles; ++t) {
+        sum += h[2 * t + 1] - h[2 * t];
+        w = std::max(w, h[2 * t + 1] - h[2 * t]);
+    }
+    *total = sum;
+    *widest = w;
+    return VJ_OK;
+}
+
+int device_window_offsets(hipStream_t s, int ntiles, const int* tile_row, const int* ptr, const int* idx,
+                          const int* win, int64_t nnz, DevBuf& off16) {
+    VJ_TRY(off16.alloc(std::max<size_t>(static_cast<size_t>(nnz), 1) * 2));
+    if (ntiles > 0) {
+        hipLaunchKernelGGL(window_offsets, dim3(ntiles), dim3(kBlock), 0, s, ntiles, tile_row, ptr, idx, win,
+                           off16.as<uint16_t>());
+        VJ_LAUNCHED();
+    }
+    return VJ_OK;
+}
+
+int device_tile_instances(hipStream_t s, int ntiles, const int* tile_row, const int* inst_of, DevBuf& inst) {
+    VJ_TRY(inst.alloc(std::max<size_t>(static_cast<size_t>(ntiles), 1) * 4));
+    if (ntiles > 0) {
+        hipLaunchKernelGGL(tile_instances, dim3(grid_for(ntiles, kBlock)), dim3(kBlock), 0, s, ntiles, tile_row, inst_of,
+                           inst.as<int>());
+        VJ_LAUNCHED();
+    }
     return VJ_OK;
 }
 

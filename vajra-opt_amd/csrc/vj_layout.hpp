@@ -13,9 +13,24 @@ int device_transpose(hipStream_t s, int in_rows, int out_rows, int64_t nnz, cons
 
 // Row tiles: consecutive rows, at most max_rows rows and at most cap entries in each of the S
 // pointer arrays ptrs + k*stride (k < S); a row over cap in some array is a tile of its own.
+// With inst_of / off (row -> instance, instance row offsets) no tile spans two instances.
 // tile_row [ntiles+1] holds the first row of each tile and rows at the end.
 int device_row_tiles(hipStream_t s, int rows, int S, const int* ptrs, int64_t stride, int cap, int max_rows,
-                     DevBuf& tile_row, int* ntiles);
+                     DevBuf& tile_row, int* ntiles, const int* inst_of = nullptr, const int* off = nullptr);
+
+// Column windows of row tiles of a CSR (ptr, idx): win [2*ntiles], tile t's entries read
+// columns in [win[2t], win[2t+1]) (0, 0 for a tile without entries); *total = the sum of the
+// window widths, *widest = the widest window.
+int device_tile_windows(hipStream_t s, int ntiles, const int* tile_row, const int* ptr, const int* idx, DevBuf& win,
+                        int64_t* total, int* widest);
+
+// 16-bit window offsets of the entries: off16[p] = idx[p] - win[2t] for every entry p of tile
+// t (the caller checks that no window is wider than 65,536 columns).
+int device_window_offsets(hipStream_t s, int ntiles, const int* tile_row, const int* ptr, const int* idx,
+                          const int* win, int64_t nnz, DevBuf& off16);
+
+// inst[t] = the instance of the first row of tile t (inst_of[tile_row[t]]; 0 when inst_of is null).
+int device_tile_instances(hipStream_t s, int ntiles, const int* tile_row, const int* inst_of, DevBuf& inst);
 
 // Per instance (rows off[q]..off[q+1]-1): the tiles holding its first and last row.
 int device_instance_tiles(hipStream_t s, int count, int ntiles, const int* tile_row, const int* off, DevBuf& first,
