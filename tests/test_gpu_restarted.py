@@ -308,6 +308,35 @@ def test_staged_equals_tile_streamed(engine, prec, stream, staged):
         assert np.array_equal(x0, x1) and np.array_equal(y0, y1)
 
 
+@pytest.mark.parametrize("prec,lanes", [("fp32", -7), ("fp64", -5)])
+def test_staged_batch_graph_and_stream_identical(engine, prec, lanes):
+    """A batch whose instances stop at very different iteration counts, with the LDS-staged
+    kernels: as instances finish, the host drops their tiles from the staged launches (and
+    captures the block graph again with the smaller grids). Graph replays and direct launches
+    prune at the same blocks, so both return the same bits, and every instance is checked by
+    the oracle at its own tolerance."""
+    lps = [vp.gen_random_lp(4200 + i, m, int(2.5 * m)) for i, m in enumerate([300, 2500, 600, 4000, 1200, 5000])]
+    lps += [lp_cases.structured_feasible_lp(*s) for s in STRUCTURED[:4]]
+    engine.upload_batch(lps)
+    out = []
+    for graph in (1, 0):
+        print(f"[case] staged batch {prec} lanes={lanes} use_graph={graph}", flush=True)
+        res = engine.solve_batch(vp.default_params(vp.MODE_RESTARTED, precision=prec, tolerance=1e-7,
+                                                   max_iterations=200000, lanes_A=lanes, lanes_At=lanes,
+                                                   use_graph=graph))
+        xs, ys = engine.download_batch()
+        for lp, r, x, y in zip(lps, res, xs, ys):
+            assert r.status == 1 and (r.lanes_A, r.lanes_At) == (lanes, lanes), r.as_dict()
+            k = O.rel_kkt(lp, x, y)
+            assert k[0] <= 1e-7 and abs(k[0] - r.rel_kkt) <= 1e-10 + 1e-6 * k[0], (k, r.rel_kkt)
+        out.append(([r.iterations for r in res], xs, ys))
+    (it0, xs0, ys0), (it1, xs1, ys1) = out
+    print(f"[case] iterations per instance {it0}", flush=True)
+    assert it0 == it1 and max(it0) >= 4 * min(it0)
+    for x0, y0, x1, y1 in zip(xs0, ys0, xs1, ys1):
+        assert np.array_equal(x0, x1) and np.array_equal(y0, y1)
+
+
 def test_infeasible_lps_never_reported_optimal(engine):
     """LPs that SciPy-HiGHS finds infeasible: the restarted engine (which, like the
     reference's PDHG, has no infeasibility certificate) stops at its iteration limit and

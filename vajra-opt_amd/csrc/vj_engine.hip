@@ -228,8 +228,16 @@ struct FastEngine {
     // first / last tile, and the sum of the windows' widths (gwin*); stageA / stageAt select them
     // with every window at most 65,536 columns wide, 16-bit window offsets of the entries
     // (off16*, built when w16ok*) replace the 4-byte column indices (w16A / w16At select them)
-    // instG*: each staged tile's instance
+    // instG*: each staged tile's instance (hInstG*: on the host). listG*: the tiles a staged
+    // launch runs, nlist* of them (its grid): every tile at the start of a solve, then the tiles
+    // of the instances still active, rebuilt by the host (with the stream drained) when half of
+    // the listed tiles belong to finished instances; pruned: some list leaves tiles out.
     DevBuf tileGA, tileGAt, winGA, winGAt, tfGA, tlGA, tfGAt, tlGAt, off16A, off16At, instGA, instGAt;
+    DevBuf listGA, listGAt;
+    std::vector<int> hInstGA, hInstGAt;
+    int rplA = 1, rplAt = 1;   // rows per lane of the staged tiles (1 or 2)
+    int nlistA = 0, nlistAt = 0;
+    bool pruned = false;
     int ntGA = 0, ntGAt = 0, gcapA = 0, gcapAt = 0;
     int64_t gwinA = 0, gwinAt = 0;
     bool stageA = false, stageAt = false, w16okA = false, w16okAt = false, w16A = false, w16At = false;
@@ -312,20 +320,24 @@ int launch_primal(vj_ctx* ctx, FastEngine& E, const V* tv, T* x, T* xbar, const 
         return VJ_OK;
     }
     if (E.stageAt) {
-        VJ_TRY(require({E.tileGAt.p, E.winGAt.p, E.instGAt.p}, "staged primal_step"));
+        VJ_TRY(require({E.tileGAt.p, E.winGAt.p, E.instGAt.p, E.listGAt.p}, "staged primal_step"));
+        if (E.nlistAt == 0) return VJ_OK;   // every instance has finished
         auto go = [&](auto ent) {
             using Ent = decltype(ent);
-            const dim3 grid(E.ntGAt);
-            if (E.multi)
-                hipLaunchKernelGGL((staged_primal_step<Ent, T, true>), grid, dim3(kStageThreads), 0, ctx->stream,
+            auto go2 = [&](auto multi, auto rpl) {
+                hipLaunchKernelGGL((staged_primal_step<Ent, T, decltype(multi)::value, decltype(rpl)::value>),
+                                   dim3(E.nlistAt), dim3(kStageThreads), 0, ctx->stream, E.listGAt.as<int>(),
                                    E.tileGAt.as<int>(), E.instGAt.as<int>(), E.winGAt.as<int>(), P.tp.as<int>(), ent, y,
                                    E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), x, xbar, E.xhat.as<double>(),
                                    E.xfull.as<double>(), E.st.as<InstState>(), E.part_x.as<double>(), it, flags, E.rho);
-            else
-                hipLaunchKernelGGL((staged_primal_step<Ent, T, false>), grid, dim3(kStageThreads), 0, ctx->stream,
-                                   E.tileGAt.as<int>(), E.instGAt.as<int>(), E.winGAt.as<int>(), P.tp.as<int>(), ent, y,
-                                   E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), x, xbar, E.xhat.as<double>(),
-                                   E.xfull.as<double>(), E.st.as<InstState>(), E.part_x.as<double>(), it, flags, E.rho);
+            };
+            using One = std::integral_constant<int, 1>;
+            using Two = std::integral_constant<int, 2>;
+            if (E.multi) {
+                if (E.rplAt == 2) go2(std::true_type{}, Two{}); else go2(std::true_type{}, One{});
+            } else {
+                if (E.rplAt == 2) go2(std::false_type{}, Two{}); else go2(std::false_type{}, One{});
+            }
         };
         if (E.w16At) {
             VJ_TRY(require({E.off16At.p}, "staged primal_step (16-bit offsets)"));
@@ -430,20 +442,24 @@ int launch_dual(vj_ctx* ctx, FastEngine& E, const V* av, const T* xbar, T* y, in
         return VJ_OK;
     }
     if (E.stageA) {
-        VJ_TRY(require({E.tileGA.p, E.winGA.p, E.instGA.p}, "staged dual_step"));
+        VJ_TRY(require({E.tileGA.p, E.winGA.p, E.instGA.p, E.listGA.p}, "staged dual_step"));
+        if (E.nlistA == 0) return VJ_OK;
         auto go = [&](auto ent) {
             using Ent = decltype(ent);
-            const dim3 grid(E.ntGA);
-            if (E.multi)
-                hipLaunchKernelGGL((staged_dual_step<Ent, T, true>), grid, dim3(kStageThreads), 0, ctx->stream,
+            auto go2 = [&](auto multi, auto rpl) {
+                hipLaunchKernelGGL((staged_dual_step<Ent, T, decltype(multi)::value, decltype(rpl)::value>),
+                                   dim3(E.nlistA), dim3(kStageThreads), 0, ctx->stream, E.listGA.as<int>(),
                                    E.tileGA.as<int>(), E.instGA.as<int>(), E.winGA.as<int>(), P.rp.as<int>(), ent, xbar,
                                    yht, lrs, urs, y, E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(),
                                    E.part_y.as<double>(), it, flags, E.rho);
-            else
-                hipLaunchKernelGGL((staged_dual_step<Ent, T, false>), grid, dim3(kStageThreads), 0, ctx->stream,
-                                   E.tileGA.as<int>(), E.instGA.as<int>(), E.winGA.as<int>(), P.rp.as<int>(), ent, xbar,
-                                   yht, lrs, urs, y, E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(),
-                                   E.part_y.as<double>(), it, flags, E.rho);
+            };
+            using One = std::integral_constant<int, 1>;
+            using Two = std::integral_constant<int, 2>;
+            if (E.multi) {
+                if (E.rplA == 2) go2(std::true_type{}, Two{}); else go2(std::true_type{}, One{});
+            } else {
+                if (E.rplA == 2) go2(std::false_type{}, Two{}); else go2(std::false_type{}, One{});
+            }
         };
         if (E.w16A) {
             VJ_TRY(require({E.off16A.p}, "staged dual_step (16-bit offsets)"));
@@ -578,6 +594,10 @@ int choose_splits(int64_t vec_len) {
     return S;
 }
 
+// Rows per lane of a staged tile: 2 where 2,048 rows of the average length fill at most the
+// tile's entry capacity (short rows: the columns of a random instance, 6 entries), else 1.
+int stage_rows_per_lane(double avg_row, int cap) { return 2.0 * kStageThreads * avg_row <= cap ? 2 : 1; }
+
 // Row tiles of the LDS-staged kernels, built on the device once per upload and entry size:
 // for batches (the kernels are autotuner candidates there), and for a single LP only when
 // the caller forces them (lanes -5 / -6): a single LP's tiles read windows about as wide as
@@ -593,7 +613,9 @@ int setup_stage(vj_ctx* ctx, FastEngine& E) {
     const int* coff = E.multi ? P.d_col_off.as<int>() : nullptr;
     if ((E.multi || prm.lanes_A <= -5) && E.gcapA != cap) {
         int widest = 0;
-        VJ_TRY(device_row_tiles(s, E.m, 1, P.rp.as<int>(), 0, cap, kStageThreads, E.tileGA, &E.ntGA, rinst, roff));
+        E.rplA = stage_rows_per_lane(static_cast<double>(E.nnz) / std::max(1, E.m), cap);
+        VJ_TRY(device_row_tiles(s, E.m, 1, P.rp.as<int>(), 0, cap, E.rplA * kStageThreads, E.tileGA, &E.ntGA, rinst,
+                                roff));
         VJ_TRY(device_tile_windows(s, E.ntGA, E.tileGA.as<int>(), P.rp.as<int>(), P.ci.as<int>(), E.winGA, &E.gwinA,
                                    &widest));
         VJ_TRY(device_instance_tiles(s, E.count, E.ntGA, E.tileGA.as<int>(), P.d_row_off.as<int>(), E.tfGA, E.tlGA));
@@ -602,11 +624,18 @@ int setup_stage(vj_ctx* ctx, FastEngine& E) {
             VJ_TRY(device_window_offsets(s, E.ntGA, E.tileGA.as<int>(), P.rp.as<int>(), P.ci.as<int>(),
                                          E.winGA.as<int>(), E.nnz, E.off16A));
         VJ_TRY(device_tile_instances(s, E.ntGA, E.tileGA.as<int>(), rinst, E.instGA));
+        VJ_TRY(E.listGA.alloc(std::max<size_t>(static_cast<size_t>(E.ntGA), 1) * 4));
+        E.hInstGA.resize(E.ntGA);
+        if (E.ntGA > 0)
+            VJ_HIP(hipMemcpyAsync(E.hInstGA.data(), E.instGA.p, static_cast<size_t>(E.ntGA) * 4, hipMemcpyDeviceToHost, s));
+        VJ_HIP(hipStreamSynchronize(s));
         E.gcapA = cap;
     }
     if ((E.multi || prm.lanes_At <= -5) && E.gcapAt != cap) {
         int widest = 0;
-        VJ_TRY(device_row_tiles(s, E.n, 1, P.tp.as<int>(), 0, cap, kStageThreads, E.tileGAt, &E.ntGAt, cinst, coff));
+        E.rplAt = stage_rows_per_lane(static_cast<double>(E.nnz) / std::max(1, E.n), cap);
+        VJ_TRY(device_row_tiles(s, E.n, 1, P.tp.as<int>(), 0, cap, E.rplAt * kStageThreads, E.tileGAt, &E.ntGAt, cinst,
+                                coff));
         VJ_TRY(device_tile_windows(s, E.ntGAt, E.tileGAt.as<int>(), P.tp.as<int>(), P.ti.as<int>(), E.winGAt,
                                    &E.gwinAt, &widest));
         VJ_TRY(device_instance_tiles(s, E.count, E.ntGAt, E.tileGAt.as<int>(), P.d_col_off.as<int>(), E.tfGAt,
@@ -616,7 +645,38 @@ int setup_stage(vj_ctx* ctx, FastEngine& E) {
             VJ_TRY(device_window_offsets(s, E.ntGAt, E.tileGAt.as<int>(), P.tp.as<int>(), P.ti.as<int>(),
                                          E.winGAt.as<int>(), E.nnz, E.off16At));
         VJ_TRY(device_tile_instances(s, E.ntGAt, E.tileGAt.as<int>(), cinst, E.instGAt));
+        VJ_TRY(E.listGAt.alloc(std::max<size_t>(static_cast<size_t>(E.ntGAt), 1) * 4));
+        E.hInstGAt.resize(E.ntGAt);
+        if (E.ntGAt > 0)
+            VJ_HIP(hipMemcpyAsync(E.hInstGAt.data(), E.instGAt.p, static_cast<size_t>(E.ntGAt) * 4,
+                                  hipMemcpyDeviceToHost, s));
+        VJ_HIP(hipStreamSynchronize(s));
         E.gcapAt = cap;
+    }
+    return VJ_OK;
+}
+
+// The staged launches' tile lists: every tile (hs null), or the tiles of the instances active
+// in hs. The caller has drained the stream (no kernel reads the lists); an executable graph
+// captured with the old grids is released.
+int set_stage_lists(vj_ctx* ctx, FastEngine& E, const InstState* hs) {
+    hipStream_t s = ctx->stream;
+    auto build = [&](const std::vector<int>& inst, DevBuf& list, int& n) -> int {
+        std::vector<int> h;
+        h.reserve(inst.size());
+        for (int t = 0; t < static_cast<int>(inst.size()); ++t)
+            if (!hs || hs[inst[t]].active) h.push_back(t);
+        n = static_cast<int>(h.size());
+        if (n > 0) VJ_HIP(hipMemcpyAsync(list.p, h.data(), static_cast<size_t>(n) * 4, hipMemcpyHostToDevice, s));
+        VJ_HIP(hipStreamSynchronize(s));   // h is released on return
+        return VJ_OK;
+    };
+    if (E.gcapA) VJ_TRY(build(E.hInstGA, E.listGA, E.nlistA));
+    if (E.gcapAt) VJ_TRY(build(E.hInstGAt, E.listGAt, E.nlistAt));
+    E.pruned = hs != nullptr && ((E.gcapA && E.nlistA < E.ntGA) || (E.gcapAt && E.nlistAt < E.ntGAt));
+    if (E.gexec) {
+        VJ_HIP(hipGraphExecDestroy(E.gexec));
+        E.gexec = nullptr;
     }
     return VJ_OK;
 }
@@ -1187,6 +1247,46 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     return VJ_OK;
 }
 
+// Capture one check block as a hipGraph and instantiate it (E.gexec).
+template <class V, class T>
+int capture_block(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
+    hipStream_t s = ctx->stream;
+    hipGraph_t g = nullptr;
+    VJ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_block<V, T>(ctx, E, av, tv);
+    hipError_t ec = hipStreamEndCapture(s, &g);
+    if (rc != VJ_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    VJ_HIP(ec);
+    const hipError_t ei = hipGraphInstantiate(&E.gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);   // the executable graph keeps what it needs
+    VJ_HIP(ei);
+    return VJ_OK;
+}
+
+// Staged batch launches: when at most half of the tiles a launch lists belong to instances
+// still active (by the state polled one block behind), drain the stream and list only the
+// tiles of the instances active now; the block graph is captured again with the new grids.
+template <class V, class T>
+int prune_stage_lists(vj_ctx* ctx, FastEngine& E, const InstState* polled, const InstState* latest, bool graph,
+                      const V* av, const V* tv) {
+    if (E.count < 2 || !(E.stageA || E.stageAt)) return VJ_OK;
+    auto live = [&](const std::vector<int>& inst) {
+        int n = 0;
+        for (int q : inst) n += polled[q].active != 0;
+        return n;
+    };
+    const bool a = E.stageA && live(E.hInstGA) * 2 <= E.nlistA;
+    const bool at = E.stageAt && live(E.hInstGAt) * 2 <= E.nlistAt;
+    if (!a && !at) return VJ_OK;
+    VJ_HIP(hipStreamSynchronize(ctx->stream));   // `latest` now holds the state after the last block
+    VJ_TRY(set_stage_lists(ctx, E, latest));
+    if (graph) VJ_TRY((capture_block<V, T>(ctx, E, av, tv)));
+    return VJ_OK;
+}
+
 template <class V, class T>
 int run_typed(vj_ctx* ctx, FastEngine& E, vj_result* out, clk::time_point t_start) {
     hipStream_t s = ctx->stream;
@@ -1205,6 +1305,7 @@ int run_typed(vj_ctx* ctx, FastEngine& E, vj_result* out, clk::time_point t_star
     VJ_TRY(setup_ell(ctx, E));
     VJ_HIP(hipStreamSynchronize(s));
     const double layout_time = secs(t_layout, clk::now());
+    if (E.gcapA || E.gcapAt) VJ_TRY(set_stage_lists(ctx, E, nullptr));   // every staged tile
     VJ_TRY(launch_restart<T>(ctx, E, 1));  // initial centring
     VJ_TRY((autotune<V, T>(ctx, E, av, tv)));
     // after autotuning the scratch runs left the real iterates untouched; recentre anyway
@@ -1225,20 +1326,7 @@ int run_typed(vj_ctx* ctx, FastEngine& E, vj_result* out, clk::time_point t_star
         E.h_state_cap = cnt;
     }
     bool use_graph = prm.use_graph != 0;
-    if (use_graph) {
-        hipGraph_t g = nullptr;
-        VJ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_block<V, T>(ctx, E, av, tv);
-        hipError_t ec = hipStreamEndCapture(s, &g);
-        if (rc != VJ_OK) {
-            if (g) (void)hipGraphDestroy(g);
-            return rc;
-        }
-        VJ_HIP(ec);
-        const hipError_t ei = hipGraphInstantiate(&E.gexec, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);   // the executable graph keeps what it needs
-        VJ_HIP(ei);
-    }
+    if (use_graph) VJ_TRY((capture_block<V, T>(ctx, E, av, tv)));
     ScopedEvent sev[2], sev0;
     VJ_HIP(hipEventCreate(&sev[0].e));
     VJ_HIP(hipEventCreate(&sev[1].e));
@@ -1269,6 +1357,7 @@ int run_typed(vj_ctx* ctx, FastEngine& E, vj_result* out, clk::time_point t_star
             }
             if (active == 0) { all_done = true; break; }
             if (secs(t_start, clk::now()) > prm.time_limit) { timed_out = true; break; }
+            VJ_TRY((prune_stage_lists<V, T>(ctx, E, hs, E.h_state[b & 1], use_graph, av, tv)));
             if (prm.verbose >= 1 && secs(t_report, clk::now()) >= 10.0) {
                 t_report = clk::now();
                 int64_t it = 0;
@@ -1462,6 +1551,9 @@ int fast_benchmark_blocks(vj_ctx* ctx, int warmup, int steps, double* ms_per_blo
     std::vector<InstState> saved(E.count), live;
     VJ_HIP(hipMemcpyAsync(saved.data(), E.st.p, sb, hipMemcpyDeviceToHost, s));
     VJ_HIP(hipStreamSynchronize(s));
+    // every instance iterates below: staged launches list every tile again (the graph of the
+    // solve, captured with pruned grids, is released and the blocks are launched directly)
+    if (E.pruned) VJ_TRY(set_stage_lists(ctx, E, nullptr));
     // every instance iterates through all blocks: none may converge or reach its limit
     live = saved;
     for (auto& z : live) {
@@ -1519,6 +1611,10 @@ int fast_benchmark(vj_ctx* ctx, int reps, vj_kernel_stat* out, int cap, int* wri
     // them with all instances marked active, then put the final state back
     std::vector<InstState> saved(E.count), active(E.count);
     VJ_HIP(hipMemcpy(saved.data(), E.st.p, E.count * sizeof(InstState), hipMemcpyDeviceToHost));
+    if (E.pruned) {   // staged launches list every tile again
+        VJ_HIP(hipStreamSynchronize(ctx->stream));
+        VJ_TRY(set_stage_lists(ctx, E, nullptr));
+    }
     active = saved;
     for (auto& z : active) z.active = 1;
     VJ_HIP(hipMemcpy(E.st.p, active.data(), E.count * sizeof(InstState), hipMemcpyHostToDevice));

@@ -613,6 +613,11 @@ __global__ __launch_bounds__(kBlock) void stream_dual_step(
 // the tile-streamed kernels, bit for bit. 64 KB of LDS per workgroup leaves room for two
 // workgroups (32 waves) per CU, which the kernels keep by asking for 8 waves per SIMD (at most
 // 64 VGPRs: the first build, with predicated entry loads, spilled to scratch and ran 9x slower).
+// Workgroup b runs tile list[b]: the host launches the tiles of the instances still active
+// (the list shrinks as instances finish; each workgroup still checks its instance's state),
+// because with 64 KB of LDS per workgroup only 512 are resident and workgroups of finished
+// instances, each exiting after two loads, cost about 64 us per iteration in the tail of a
+// 512-LP batch.
 constexpr int kStageThreads = 1024;
 constexpr int kStageBytes = 65536;
 template <class T>
@@ -727,16 +732,19 @@ __device__ __forceinline__ void stage_partials(const double (&v)[3], int slot, d
     }
 }
 
-template <class Ent, class T, bool MULTI>
+// RPL rows per lane: tiles of up to RPL * kStageThreads rows. Matrices with short rows (the
+// columns of a random batch instance have 6 entries) take RPL = 2, so that a tile still holds
+// enough entries to pay for copying its window.
+template <class Ent, class T, bool MULTI, int RPL>
 __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8))) void staged_primal_step(
-    const int* __restrict__ tile_row, const int* __restrict__ tile_inst, const int* __restrict__ win,
-    const int* __restrict__ tp, Ent ent, const T* __restrict__ y, const T* __restrict__ dhat,
+    const int* __restrict__ list, const int* __restrict__ tile_row, const int* __restrict__ tile_inst,
+    const int* __restrict__ win, const int* __restrict__ tp, Ent ent, const T* __restrict__ y, const T* __restrict__ dhat,
     const T* __restrict__ lcs, const T* __restrict__ ucs, T* __restrict__ x, T* __restrict__ xbar,
     const double* __restrict__ xhat, double* __restrict__ xfull, const InstState* __restrict__ st,
     double* __restrict__ part, int it, int flags, double rho) {
     __shared__ T lds[stage_cap<T>()];
     __shared__ double red[kStageThreads / kWave][3];
-    const int tile = blockIdx.x;
+    const int tile = list[blockIdx.x];
     const int my = MULTI ? tile_inst[tile] : 0;   // every row of the tile belongs to this instance
     if (st[my].active == 0) return;
     const int r0 = tile_row[tile], r1 = tile_row[tile + 1];
@@ -746,20 +754,27 @@ __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8
         const T acc = staged_long_row<Ent, T>(s0, s1, ent, y, lds);
         if (threadIdx.x == 0) primal_epilogue<T>(acc, r0, st[my], it, flags, rho, dhat, lcs, ucs, x, xbar, xhat, xfull, c);
     } else {
-        const int j = r0 + static_cast<int>(threadIdx.x);
-        const bool mine = j < r1;
-        PrimalOps<T> o{};
-        int q0 = 0, q1 = 0;
-        if (mine) {   // the row's operands and bounds are loaded before the staging
-            o = primal_ops<T>(j, x, xbar, dhat, lcs, ucs, st[my], it, rho);
-            q0 = tp[j] - s0;
-            q1 = tp[j + 1] - s0;
+        // the rows' operands and bounds are loaded before the staging (a lane past the tile's end
+        // loads its last row's and skips the update)
+        PrimalOps<T> o[RPL];
+        int q0[RPL], q1[RPL];
+#pragma unroll
+        for (int u = 0; u < RPL; ++u) {
+            const int j = r0 + static_cast<int>(threadIdx.x) + u * kStageThreads;
+            const int jc = j < r1 ? j : r1 - 1;
+            o[u] = primal_ops<T>(jc, x, xbar, dhat, lcs, ucs, st[my], it, rho);
+            q0[u] = tp[jc] - s0;
+            q1[u] = tp[jc + 1] - s0;
         }
         staged_products<Ent, T>(s0, s1, win[2 * tile], win[2 * tile + 1], ent, y, lds);
-        if (mine) {
-            T acc = 0;
-            for (int q = q0; q < q1; ++q) acc += lds[q];
-            primal_update<T>(acc, o, j, st[my], it, flags, rho, x, xbar, xhat, xfull, c);
+#pragma unroll
+        for (int u = 0; u < RPL; ++u) {
+            const int j = r0 + static_cast<int>(threadIdx.x) + u * kStageThreads;
+            if (j < r1) {
+                T acc = 0;
+                for (int q = q0[u]; q < q1[u]; ++q) acc += lds[q];
+                primal_update<T>(acc, o[u], j, st[my], it, flags, rho, x, xbar, xhat, xfull, c);
+            }
         }
     }
     if (flags) {
@@ -768,16 +783,16 @@ __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8
     }
 }
 
-template <class Ent, class T, bool MULTI>
+template <class Ent, class T, bool MULTI, int RPL>
 __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8))) void staged_dual_step(
-    const int* __restrict__ tile_row, const int* __restrict__ tile_inst, const int* __restrict__ win,
-    const int* __restrict__ rp, Ent ent, const T* __restrict__ xbar, const T* __restrict__ yhat_t,
+    const int* __restrict__ list, const int* __restrict__ tile_row, const int* __restrict__ tile_inst,
+    const int* __restrict__ win, const int* __restrict__ rp, Ent ent, const T* __restrict__ xbar, const T* __restrict__ yhat_t,
     const T* __restrict__ lrs, const T* __restrict__ urs, T* __restrict__ y, const double* __restrict__ yhat,
     double* __restrict__ yfull, const InstState* __restrict__ st, double* __restrict__ part, int it, int flags,
     double rho) {
     __shared__ T lds[stage_cap<T>()];
     __shared__ double red[kStageThreads / kWave][3];
-    const int tile = blockIdx.x;
+    const int tile = list[blockIdx.x];
     const int my = MULTI ? tile_inst[tile] : 0;
     if (st[my].active == 0) return;
     const int r0 = tile_row[tile], r1 = tile_row[tile + 1];
@@ -787,20 +802,25 @@ __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8
         const T w = staged_long_row<Ent, T>(s0, s1, ent, xbar, lds);
         if (threadIdx.x == 0) dual_epilogue<T>(w, r0, st[my], it, flags, rho, yhat_t, lrs, urs, y, yhat, yfull, c);
     } else {
-        const int i = r0 + static_cast<int>(threadIdx.x);
-        const bool mine = i < r1;
-        DualOps<T> o{};
-        int q0 = 0, q1 = 0;
-        if (mine) {
-            o = dual_ops<T>(i, y, yhat_t, lrs, urs);
-            q0 = rp[i] - s0;
-            q1 = rp[i + 1] - s0;
+        DualOps<T> o[RPL];
+        int q0[RPL], q1[RPL];
+#pragma unroll
+        for (int u = 0; u < RPL; ++u) {
+            const int i = r0 + static_cast<int>(threadIdx.x) + u * kStageThreads;
+            const int ic = i < r1 ? i : r1 - 1;
+            o[u] = dual_ops<T>(ic, y, yhat_t, lrs, urs);
+            q0[u] = rp[ic] - s0;
+            q1[u] = rp[ic + 1] - s0;
         }
         staged_products<Ent, T>(s0, s1, win[2 * tile], win[2 * tile + 1], ent, xbar, lds);
-        if (mine) {
-            T w = 0;
-            for (int q = q0; q < q1; ++q) w += lds[q];
-            dual_update<T>(w, o, i, st[my], it, flags, rho, y, yhat, yfull, c);
+#pragma unroll
+        for (int u = 0; u < RPL; ++u) {
+            const int i = r0 + static_cast<int>(threadIdx.x) + u * kStageThreads;
+            if (i < r1) {
+                T w = 0;
+                for (int q = q0[u]; q < q1[u]; ++q) w += lds[q];
+                dual_update<T>(w, o[u], i, st[my], it, flags, rho, y, yhat, yfull, c);
+            }
         }
     }
     if (flags) {
