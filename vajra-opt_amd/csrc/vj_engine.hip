@@ -222,6 +222,12 @@ struct FastEngine {
     bool streamA = true, streamAt = true, tiles_built = false;
     int ntA = 0, ntAt = 0;
     DevBuf tileA, tileAt, tfA, tlA, tfAt, tlAt;
+    // batches: the tile-streamed launches' lists of tiles holding a row of an active instance
+    // (listTA / listTAt, nlistT* of them), kept like the staged lists; hFirstT* / hLastT*: the
+    // first and last instance of each tile, on the host
+    DevBuf listTA, listTAt;
+    std::vector<int> hFirstTA, hLastTA, hFirstTAt, hLastTAt;
+    int nlistTA = 0, nlistTAt = 0;
     // LDS-staged kernels (batches): row tiles that never span two instances, at most
     // kStageThreads rows and stage_cap entries (gcap*: the cap they were built for, which
     // differs between 8-byte and 4-byte iterates), each tile's column window, per instance the
@@ -352,14 +358,15 @@ int launch_primal(vj_ctx* ctx, FastEngine& E, const V* tv, T* x, T* xbar, const 
         VJ_TRY(require({E.tileAt.p}, "stream primal_step"));
         VJ_TRY(with_entries<V>(E.recAt, E.recAt_buf, P.ti.as<int>(), tv, "stream records", [&](auto ent) {
             using Ent = decltype(ent);
-            if (E.multi)
-                hipLaunchKernelGGL((stream_primal_step<Ent, T, true>), dim3(E.ntAt), dim3(kBlock), 0, ctx->stream, E.n,
-                                   E.tileAt.as<int>(), P.tp.as<int>(), ent, y, E.dhat.as<T>(), E.lcs.as<T>(),
+            if (E.multi) {
+                if (E.nlistTAt > 0)
+                hipLaunchKernelGGL((stream_primal_step<Ent, T, true>), dim3(E.nlistTAt), dim3(kBlock), 0, ctx->stream, E.n,
+                                   E.listTAt.as<int>(), E.tileAt.as<int>(), P.tp.as<int>(), ent, y, E.dhat.as<T>(), E.lcs.as<T>(),
                                    E.ucs.as<T>(), x, xbar, E.xhat.as<double>(), E.xfull.as<double>(),
                                    E.st.as<InstState>(), P.col_inst.as<int>(), E.part_x.as<double>(), it, flags, E.rho);
-            else
+            } else
                 hipLaunchKernelGGL((stream_primal_step<Ent, T, false>), dim3(E.ntAt), dim3(kBlock), 0, ctx->stream, E.n,
-                                   E.tileAt.as<int>(), P.tp.as<int>(), ent, y, E.dhat.as<T>(), E.lcs.as<T>(),
+                                   nullptr, E.tileAt.as<int>(), P.tp.as<int>(), ent, y, E.dhat.as<T>(), E.lcs.as<T>(),
                                    E.ucs.as<T>(), x, xbar, E.xhat.as<double>(), E.xfull.as<double>(),
                                    E.st.as<InstState>(), P.col_inst.as<int>(), E.part_x.as<double>(), it, flags, E.rho);
         }));
@@ -474,14 +481,15 @@ int launch_dual(vj_ctx* ctx, FastEngine& E, const V* av, const T* xbar, T* y, in
         VJ_TRY(require({E.tileA.p}, "stream dual_step"));
         VJ_TRY(with_entries<V>(E.recA, E.recA_buf, P.ci.as<int>(), av, "stream records", [&](auto ent) {
             using Ent = decltype(ent);
-            if (E.multi)
-                hipLaunchKernelGGL((stream_dual_step<Ent, T, true>), dim3(E.ntA), dim3(kBlock), 0, ctx->stream, E.m,
-                                   E.tileA.as<int>(), P.rp.as<int>(), ent, xbar, yht, lrs, urs, y,
+            if (E.multi) {
+                if (E.nlistTA > 0)
+                hipLaunchKernelGGL((stream_dual_step<Ent, T, true>), dim3(E.nlistTA), dim3(kBlock), 0, ctx->stream, E.m,
+                                   E.listTA.as<int>(), E.tileA.as<int>(), P.rp.as<int>(), ent, xbar, yht, lrs, urs, y,
                                    E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(),
                                    P.row_inst.as<int>(), E.part_y.as<double>(), it, flags, E.rho);
-            else
+            } else
                 hipLaunchKernelGGL((stream_dual_step<Ent, T, false>), dim3(E.ntA), dim3(kBlock), 0, ctx->stream, E.m,
-                                   E.tileA.as<int>(), P.rp.as<int>(), ent, xbar, yht, lrs, urs, y,
+                                   nullptr, E.tileA.as<int>(), P.rp.as<int>(), ent, xbar, yht, lrs, urs, y,
                                    E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(),
                                    P.row_inst.as<int>(), E.part_y.as<double>(), it, flags, E.rho);
         }));
@@ -594,6 +602,25 @@ int choose_splits(int64_t vec_len) {
     return S;
 }
 
+// first[t] / last[t]: the first and last instance holding a row of tile t (rows off[q] ..
+// off[q+1]-1 belong to instance q), from the device tile array.
+int tile_instance_ranges(hipStream_t s, int ntiles, const DevBuf& tile_row, const std::vector<int64_t>& off,
+                         std::vector<int>& first, std::vector<int>& last) {
+    std::vector<int> tr(static_cast<size_t>(ntiles) + 1);
+    VJ_HIP(hipMemcpyAsync(tr.data(), tile_row.p, tr.size() * 4, hipMemcpyDeviceToHost, s));
+    VJ_HIP(hipStreamSynchronize(s));
+    auto inst_of = [&](int64_t row) {   // the last q with off[q] <= row
+        return static_cast<int>(std::upper_bound(off.begin(), off.end(), row) - off.begin()) - 1;
+    };
+    first.resize(ntiles);
+    last.resize(ntiles);
+    for (int t = 0; t < ntiles; ++t) {
+        first[t] = inst_of(tr[t]);
+        last[t] = inst_of(tr[t + 1] - 1);
+    }
+    return VJ_OK;
+}
+
 // Rows per lane of a staged tile: 2 where 2,048 rows of the average length fill at most the
 // tile's entry capacity (short rows: the columns of a random instance, 6 entries), else 1.
 int stage_rows_per_lane(double avg_row, int cap) { return 2.0 * kStageThreads * avg_row <= cap ? 2 : 1; }
@@ -671,9 +698,29 @@ int set_stage_lists(vj_ctx* ctx, FastEngine& E, const InstState* hs) {
         VJ_HIP(hipStreamSynchronize(s));   // h is released on return
         return VJ_OK;
     };
+    // tile-streamed tiles of a batch may hold rows of several instances: listed while one is active
+    auto build_ranges = [&](const std::vector<int>& first, const std::vector<int>& last, DevBuf& list,
+                            int& n) -> int {
+        std::vector<int> h;
+        h.reserve(first.size());
+        for (int t = 0; t < static_cast<int>(first.size()); ++t) {
+            bool live = !hs;
+            for (int q = first[t]; !live && q <= last[t]; ++q) live = hs[q].active != 0;
+            if (live) h.push_back(t);
+        }
+        n = static_cast<int>(h.size());
+        if (n > 0) VJ_HIP(hipMemcpyAsync(list.p, h.data(), static_cast<size_t>(n) * 4, hipMemcpyHostToDevice, s));
+        VJ_HIP(hipStreamSynchronize(s));
+        return VJ_OK;
+    };
     if (E.gcapA) VJ_TRY(build(E.hInstGA, E.listGA, E.nlistA));
     if (E.gcapAt) VJ_TRY(build(E.hInstGAt, E.listGAt, E.nlistAt));
-    E.pruned = hs != nullptr && ((E.gcapA && E.nlistA < E.ntGA) || (E.gcapAt && E.nlistAt < E.ntGAt));
+    if (E.multi && E.tiles_built) {
+        VJ_TRY(build_ranges(E.hFirstTA, E.hLastTA, E.listTA, E.nlistTA));
+        VJ_TRY(build_ranges(E.hFirstTAt, E.hLastTAt, E.listTAt, E.nlistTAt));
+    }
+    E.pruned = hs != nullptr && ((E.gcapA && E.nlistA < E.ntGA) || (E.gcapAt && E.nlistAt < E.ntGAt) ||
+                                 (E.multi && E.tiles_built && (E.nlistTA < E.ntA || E.nlistTAt < E.ntAt)));
     if (E.gexec) {
         VJ_HIP(hipGraphExecDestroy(E.gexec));
         E.gexec = nullptr;
@@ -706,6 +753,12 @@ int setup(vj_ctx* ctx, FastEngine& E) {
         VJ_TRY(device_instance_tiles(s, cnt, E.ntA, E.tileA.as<int>(), P.d_row_off.as<int>(), E.tfA, E.tlA));
         VJ_TRY(device_row_tiles(s, n, 1, P.tp.as<int>(), 0, kStreamCap, kBlock, E.tileAt, &E.ntAt));
         VJ_TRY(device_instance_tiles(s, cnt, E.ntAt, E.tileAt.as<int>(), P.d_col_off.as<int>(), E.tfAt, E.tlAt));
+        if (E.multi) {
+            VJ_TRY(tile_instance_ranges(s, E.ntA, E.tileA, P.row_off, E.hFirstTA, E.hLastTA));
+            VJ_TRY(tile_instance_ranges(s, E.ntAt, E.tileAt, P.col_off, E.hFirstTAt, E.hLastTAt));
+            VJ_TRY(E.listTA.alloc(std::max<size_t>(static_cast<size_t>(E.ntA), 1) * 4));
+            VJ_TRY(E.listTAt.alloc(std::max<size_t>(static_cast<size_t>(E.ntAt), 1) * 4));
+        }
         E.tiles_built = true;
     }
     VJ_TRY(setup_stage(ctx, E));
@@ -1266,20 +1319,34 @@ int capture_block(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     return VJ_OK;
 }
 
-// Staged batch launches: when at most half of the tiles a launch lists belong to instances
-// still active (by the state polled one block behind), drain the stream and list only the
-// tiles of the instances active now; the block graph is captured again with the new grids.
+// Batch launches (LDS-staged and tile-streamed steps): when at most half of the tiles a launch
+// lists hold rows of instances still active (by the state polled one block behind), drain the
+// stream and list only the tiles of the instances active now; the block graph is captured
+// again with the new grids.
 template <class V, class T>
 int prune_stage_lists(vj_ctx* ctx, FastEngine& E, const InstState* polled, const InstState* latest, bool graph,
                       const V* av, const V* tv) {
-    if (E.count < 2 || !(E.stageA || E.stageAt)) return VJ_OK;
+    if (E.count < 2) return VJ_OK;
     auto live = [&](const std::vector<int>& inst) {
         int n = 0;
         for (int q : inst) n += polled[q].active != 0;
         return n;
     };
-    const bool a = E.stageA && live(E.hInstGA) * 2 <= E.nlistA;
-    const bool at = E.stageAt && live(E.hInstGAt) * 2 <= E.nlistAt;
+    auto live_ranges = [&](const std::vector<int>& first, const std::vector<int>& last) {
+        int n = 0;
+        for (size_t t = 0; t < first.size(); ++t) {
+            bool l = false;
+            for (int q = first[t]; !l && q <= last[t]; ++q) l = polled[q].active != 0;
+            n += l;
+        }
+        return n;
+    };
+    const bool sa = E.SA == 1 && E.streamA && !E.ellA && !E.stageA;   // tile-streamed steps of the batch
+    const bool sat = E.SAt == 1 && E.streamAt && !E.ellAt && !E.stageAt;
+    const bool a = (E.stageA && live(E.hInstGA) * 2 <= E.nlistA) ||
+                   (sa && live_ranges(E.hFirstTA, E.hLastTA) * 2 <= E.nlistTA);
+    const bool at = (E.stageAt && live(E.hInstGAt) * 2 <= E.nlistAt) ||
+                    (sat && live_ranges(E.hFirstTAt, E.hLastTAt) * 2 <= E.nlistTAt);
     if (!a && !at) return VJ_OK;
     VJ_HIP(hipStreamSynchronize(ctx->stream));   // `latest` now holds the state after the last block
     VJ_TRY(set_stage_lists(ctx, E, latest));
@@ -1305,7 +1372,7 @@ int run_typed(vj_ctx* ctx, FastEngine& E, vj_result* out, clk::time_point t_star
     VJ_TRY(setup_ell(ctx, E));
     VJ_HIP(hipStreamSynchronize(s));
     const double layout_time = secs(t_layout, clk::now());
-    if (E.gcapA || E.gcapAt) VJ_TRY(set_stage_lists(ctx, E, nullptr));   // every staged tile
+    if (E.gcapA || E.gcapAt || E.multi) VJ_TRY(set_stage_lists(ctx, E, nullptr));   // every tile
     VJ_TRY(launch_restart<T>(ctx, E, 1));  // initial centring
     VJ_TRY((autotune<V, T>(ctx, E, av, tv)));
     // after autotuning the scratch runs left the real iterates untouched; recentre anyway

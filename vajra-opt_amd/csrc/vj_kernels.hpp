@@ -504,12 +504,13 @@ __device__ __forceinline__ T long_row_sum(int s0, int s1, const Ent& ent, const 
 
 template <class Ent, class T, bool MULTI>
 __global__ __launch_bounds__(kBlock) void stream_primal_step(
-    int n, const int* __restrict__ tile_row, const int* __restrict__ tp, Ent ent, const T* __restrict__ y, const T* __restrict__ dhat, const T* __restrict__ lcs,
+    int n, const int* __restrict__ list, const int* __restrict__ tile_row, const int* __restrict__ tp, Ent ent,
+    const T* __restrict__ y, const T* __restrict__ dhat, const T* __restrict__ lcs,
     const T* __restrict__ ucs, T* __restrict__ x, T* __restrict__ xbar, const double* __restrict__ xhat,
     double* __restrict__ xfull, const InstState* __restrict__ st, const int* __restrict__ col_inst,
     double* __restrict__ part, int it, int flags, double rho) {
     __shared__ T lds[kStreamCap];
-    const int tile = blockIdx.x;
+    const int tile = list ? list[blockIdx.x] : static_cast<int>(blockIdx.x);   // batches: the live tiles
     const int r0 = tile_row[tile], r1 = tile_row[tile + 1];
     int f = 0, l = 0;
     if (MULTI) {
@@ -553,12 +554,13 @@ __global__ __launch_bounds__(kBlock) void stream_primal_step(
 
 template <class Ent, class T, bool MULTI>
 __global__ __launch_bounds__(kBlock) void stream_dual_step(
-    int m, const int* __restrict__ tile_row, const int* __restrict__ rp, Ent ent, const T* __restrict__ xbar, const T* __restrict__ yhat_t, const T* __restrict__ lrs,
+    int m, const int* __restrict__ list, const int* __restrict__ tile_row, const int* __restrict__ rp, Ent ent,
+    const T* __restrict__ xbar, const T* __restrict__ yhat_t, const T* __restrict__ lrs,
     const T* __restrict__ urs, T* __restrict__ y, const double* __restrict__ yhat, double* __restrict__ yfull,
     const InstState* __restrict__ st, const int* __restrict__ row_inst, double* __restrict__ part, int it, int flags,
     double rho) {
     __shared__ T lds[kStreamCap];
-    const int tile = blockIdx.x;
+    const int tile = list ? list[blockIdx.x] : static_cast<int>(blockIdx.x);
     const int r0 = tile_row[tile], r1 = tile_row[tile + 1];
     int f = 0, l = 0;
     if (MULTI) {
