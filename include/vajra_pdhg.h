@@ -132,12 +132,19 @@ typedef struct vj_params {
     double restart_sufficient;     /* default 0.2 */
     double restart_necessary;      /* default 0.8 */
     double restart_artificial;     /* default 0.5 (PDLP: 0.36) */
-    double primal_weight_smoothing;/* theta in log w <- theta log(|dy|/|dx|) + (1-theta) log w; default 0.5 */
+    double primal_weight_smoothing;/* proportional gain theta of the primal-weight update at a restart:
+                                      with e = log(w |dx| / |dy|) (|dx|, |dy|: the distances the epoch
+                                      moved), log w <- log w - theta e - ki (sum of e over the solve's
+                                      restarts); with ki = 0 this is PDLP's smoothing
+                                      log w <- theta log(|dy|/|dx|) + (1-theta) log w; default 0.5 */
     int power_iterations;          /* RESTARTED: power-iteration steps for ||A||_2; default 64 */
     double restart_drift;          /* RESTARTED: also restart (an FP64 refinement step) when a check's
                                       FP64 rel-KKT exceeds restart_drift times the least rel-KKT of
                                       the epoch; 0 = default (2 with FP32 iterates, off with FP64
                                       and BF16), < 0 = off, > 1 = this factor for every precision */
+    double primal_weight_integral; /* RESTARTED: integral gain ki of the primal-weight update (see
+                                      primal_weight_smoothing), in [0, 1); default 0 (PDLP's
+                                      smoothing) */
 } vj_params;
 /* Fill defaults for a mode (reference defaults for VJ_MODE_REFERENCE). */
 void vj_params_default(vj_params* p, int mode);

@@ -245,6 +245,10 @@ void test_device_and_fallback() {
     o.use_gpu = true;
     o.enable_presolve = false;
     o.iteration_limit = 5000;
+    // the device solves to this relative KKT error; at the default 1e-6 whether the returned
+    // point also passes the reference's 1e-7 status guard depends on where the iteration
+    // stops (the guard itself is covered by status_guard)
+    o.set("tolerance", 1e-9);
     if (g_gpu) {
         const indus::Solution s = indus::solve(M, o);
         CHECK(s.status == indus::SolveStatus::kOptimal);
@@ -253,7 +257,7 @@ void test_device_and_fallback() {
         std::cout << "  kernel mode: " << t.kernel_mode << "\n";
         CHECK(t.kernel_mode.find("restarted_halpern") == 0);
         CHECK(t.iterations == s.iterations);
-        CHECK(t.rel_kkt <= 1e-6);
+        CHECK(t.rel_kkt <= 1e-9);
     } else {
         // no device: a device request fails loudly ...
         bool threw = false;

@@ -4,7 +4,7 @@ to study the algorithm on the CPU: its setup (Ruiz passes and one Pock-Chambolle
 cumulative factors bounded to [1/max_scale, max_scale], bound/objective rescaling whose
 norms leave out rows and columns at that bound, 64-step power iteration), the correction
 coordinates around an FP64 centre, the reflected Halpern step, and the restart rule and
-primal-weight smoothing of the control kernel, checked every K iterations on the original
+primal-weight controller of the control kernel, checked every K iterations on the original
 LP's rel-KKT. It reproduces the device's FP64 iteration counts on the LPs it was compared on;
 it is not the product and nothing in the product calls it."""
 import numpy as np
@@ -57,8 +57,8 @@ def setup(lp, max_scale=1e6, exclude_bounded=True, ruiz_passes=10, pock_chamboll
                 bs=bs, omega=omega, norm=np.sqrt(lam))
 
 
-def solve(lp, S=None, K=64, tol=1e-6, max_iters=200000, rho=1.0, artificial=0.5, smoothing=0.5, sufficient=0.2,
-          necessary=0.8, step=0.95, **setup_kw):
+def solve(lp, S=None, K=64, tol=1e-6, max_iters=200000, rho=1.0, artificial=0.5, smoothing=0.5, integral=0.0,
+          sufficient=0.2, necessary=0.8, step=0.95, **setup_kw):
     """Returns status (1 optimal, 6 iteration limit), iterations, restarts and the original-space
     point (x, canonical y) of the last check."""
     S = S or setup(lp, **setup_kw)
@@ -74,6 +74,7 @@ def solve(lp, S=None, K=64, tol=1e-6, max_iters=200000, rho=1.0, artificial=0.5,
     dh, lcs, ucs, lrs, urs = recentre(xh, yh)
     x, y = np.zeros(n), np.zeros(m)
     k = iters = restarts = 0
+    pw_sum = 0.0
     r0 = r_prev = np.inf
     restarted_prev = True
     while True:
@@ -113,7 +114,12 @@ def solve(lp, S=None, K=64, tol=1e-6, max_iters=200000, rho=1.0, artificial=0.5,
             restarted_prev, restarts, k = True, restarts + 1, 0
             dx, dy = np.sqrt(xq2), np.sqrt(yq2)
             if dx > 1e-10 and dy > 1e-10:
-                omega = np.exp(smoothing * np.log(dy / dx) + (1 - smoothing) * np.log(omega))
+                if integral == 0.0:
+                    omega = np.exp(smoothing * np.log(dy / dx) + (1 - smoothing) * np.log(omega))
+                else:   # proportional-integral control of e = log(omega |dx| / |dy|)
+                    e = np.log(omega) - np.log(dy / dx)
+                    pw_sum += e
+                    omega = np.exp(np.log(omega) - (smoothing * e + integral * pw_sum))
             xh, yh = xf, yf
             dh, lcs, ucs, lrs, urs = recentre(xh, yh)
             x, y = np.zeros(n), np.zeros(m)

@@ -40,7 +40,8 @@ def test_params_defaults_follow_reference():
     assert q.mode == vp.MODE_RESTARTED and q.check_frequency == 0 and q.use_graph == 1 and q.reflection == 1.0
     # restart rule and primal-weight smoothing of the restarted engine
     assert (q.restart_sufficient, q.restart_necessary, q.restart_artificial) == (0.2, 0.8, 0.5)
-    assert q.primal_weight_smoothing == 0.5 and q.power_iterations == 64 and q.step_scale == 0.95
+    assert q.primal_weight_smoothing == 0.5 and q.primal_weight_integral == 0.0
+    assert q.power_iterations == 64 and q.step_scale == 0.95
     assert q.restart_drift == 0.0   # 0: the engine's default (2 with FP32 iterates, off otherwise)
 
 

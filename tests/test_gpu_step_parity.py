@@ -31,8 +31,8 @@ VARIANTS = {
 }
 
 
-def model_point(lp, blocks, rho=1.0):
-    r = M.solve(lp, K=K, tol=0.0, max_iters=blocks * K, rho=rho)
+def model_point(lp, blocks, rho=1.0, **kw):
+    r = M.solve(lp, K=K, tol=0.0, max_iters=blocks * K, rho=rho, **kw)
     assert r["iters"] == blocks * K
     return r["x"], r["y"], r["restarts"]
 
@@ -82,6 +82,24 @@ def test_batch_blocks_equal_model(engine, lanes, blocks):
         assert r.iterations == blocks * K, r.as_dict()
         mx, my, restarts = model_point(lp, blocks)
         assert r.restart_count == restarts
+        close(x, mx, (name, "x"))
+        close(y, my, (name, "y"))
+
+
+@pytest.mark.parametrize("variant", ["tile_streamed", "lds_staged"])
+def test_primal_weight_integral_equals_model(engine, variant):
+    """The proportional-integral primal-weight controller (primal_weight_integral > 0): 6 blocks
+    cross several restarts, each of which moves the weight by theta e + ki sum(e); the device
+    equals the model's run with the same gains."""
+    for name, lp in lps():
+        print(f"[case] step parity integral 0.1 {variant} {name}", flush=True)
+        engine.upload(lp)
+        r = engine.solve(vp.default_params(vp.MODE_RESTARTED, precision="fp64", tolerance=0.0, max_iterations=6 * K,
+                                           check_frequency=K, primal_weight_smoothing=0.3,
+                                           primal_weight_integral=0.1, **VARIANTS[variant]))
+        x, _, y = engine.download()
+        mx, my, restarts = model_point(lp, 6, smoothing=0.3, integral=0.1)
+        assert r.restart_count == restarts and restarts >= 2, (r.restart_count, restarts)
         close(x, mx, (name, "x"))
         close(y, my, (name, "y"))
 

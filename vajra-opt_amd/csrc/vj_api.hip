@@ -201,6 +201,7 @@ void vj_params_default(vj_params* p, int mode) {
     p->restart_necessary = 0.8;
     p->restart_artificial = 0.5;
     p->primal_weight_smoothing = 0.5;
+    p->primal_weight_integral = 0.0;
     p->power_iterations = 64;
     p->restart_drift = 0.0;
 }
@@ -261,7 +262,8 @@ int vj_params_set(vj_ctx* ctx, const vj_params* p) {
     if (!(p->restart_sufficient > 0.0 && p->restart_sufficient < 1.0) ||
         !(p->restart_necessary >= p->restart_sufficient && p->restart_necessary < 1.0) ||
         !(p->restart_artificial > 0.0 && p->restart_artificial <= 1.0) ||
-        !(p->primal_weight_smoothing >= 0.0 && p->primal_weight_smoothing <= 1.0) || p->power_iterations < 1 ||
+        !(p->primal_weight_smoothing >= 0.0 && p->primal_weight_smoothing <= 1.0) ||
+        !(p->primal_weight_integral >= 0.0 && p->primal_weight_integral < 1.0) || p->power_iterations < 1 ||
         p->power_iterations > 10000 || std::isnan(p->restart_drift) ||
         (p->restart_drift > 0.0 && p->restart_drift <= 1.0)) {
         set_error("restart/primal-weight/power-iteration parameters out of range");

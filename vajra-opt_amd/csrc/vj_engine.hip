@@ -1104,6 +1104,7 @@ int enqueue_block(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     const double drift = ctx->params.restart_drift;
     a.kkt_drift = drift > 1.0 ? drift : (drift == 0.0 && E.prec == VJ_PREC_FP32 ? kRefineDrift : 0.0);
     a.pw_theta = ctx->params.primal_weight_smoothing;
+    a.pw_ki = ctx->params.primal_weight_integral;
     a.hist = E.hist_cap > 0 ? E.hist.as<double>() : nullptr;
     a.hist_cap = E.hist_cap;
     hipLaunchKernelGGL(control_kernel, dim3(E.count), dim3(kBlock), 0, ctx->stream, E.st.as<InstState>(), a);

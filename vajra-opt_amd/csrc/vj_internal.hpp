@@ -85,6 +85,7 @@ struct alignas(16) InstState {
     double tol;
     double kkt_min;                    // least rel-KKT at the checks of the current epoch (0: none yet)
     double time_to_target;             // filled by host
+    double pw_sum;                     // sum of the primal-weight errors at the restarts so far
     int64_t k;                         // iterations since the last restart (epoch index)
     int64_t iters;                     // total iterations
     int64_t max_iters;

@@ -1302,8 +1302,10 @@ __global__ __launch_bounds__(kBlock) void kkt_finish(
 // FP64 rel-KKT of a check exceeds kkt_drift times the least rel-KKT of the epoch: the
 // working-precision iteration then converges towards the fixed point of its rounded data
 // while the true error grows, and the restart's FP64 re-centring is the refinement step
-// that removes that error. At a restart the primal weight is smoothed as in PDLP:
-// log omega <- theta log(|Dy|/|Dx|) + (1 - theta) log omega (theta = 0.5 by default).
+// that removes that error. At a restart the primal weight follows a proportional-integral
+// controller on the error e = log(omega |Dx| / |Dy|) (|Dx|, |Dy|: the distances the epoch moved):
+// log omega <- log omega - (theta e + ki sum(e)); with ki = 0 (the default, theta = 0.5) it is
+// PDLP's smoothing log omega <- theta log(|Dy|/|Dx|) + (1 - theta) log omega.
 struct ControlArgs {
     const double* px; int Rx;      // primal-step partials (cols), fixed tile height Rx
     const double* py; int Ry;      // dual-step partials (rows), fixed tile height Ry
@@ -1317,6 +1319,7 @@ struct ControlArgs {
     int K;
     double restart_sufficient, restart_necessary, restart_artificial, pw_theta;
     double kkt_drift;              // 0: off (FP64 and BF16 iterates)
+    double pw_ki;                  // integral gain of the primal-weight controller
     double* hist;                  // per instance: [checks] rel_kkt history (may be null)
     int hist_cap;
 };
@@ -1375,7 +1378,13 @@ __global__ __launch_bounds__(kBlock) void control_kernel(InstState* __restrict__
             s.k = 0;
             const double dx = sqrt(xq2), dy = sqrt(yq2);
             if (dx > 1e-10 && dy > 1e-10) {
-                s.omega = exp(a.pw_theta * log(dy / dx) + (1.0 - a.pw_theta) * log(s.omega));
+                if (a.pw_ki == 0.0) {
+                    s.omega = exp(a.pw_theta * log(dy / dx) + (1.0 - a.pw_theta) * log(s.omega));
+                } else {
+                    const double e = log(s.omega) - log(dy / dx);
+                    s.pw_sum += e;
+                    s.omega = exp(log(s.omega) - (a.pw_theta * e + a.pw_ki * s.pw_sum));
+                }
                 s.tau = s.eta / s.omega;
                 s.sigma = s.eta * s.omega;
             }
