@@ -74,7 +74,7 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--batch-count", type=int, default=CFG3["count"])
     ap.add_argument("--stair-count", type=int, default=CFG4["count"])
-    ap.add_argument("--seq-random", type=int, default=64,
+    ap.add_argument("--seq-random", type=int, default=128,
                     help="random instances solved one by one for the sequential rate (every k-th of the batch)")
     ap.add_argument("--seq-staircase", type=int, default=128,
                     help="staircase instances solved one by one (all 128 by default)")
