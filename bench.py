@@ -79,7 +79,10 @@ def main():
     ap.add_argument("--seq-staircase", type=int, default=128,
                     help="staircase instances solved one by one (all 128 by default)")
     ap.add_argument("--cpu-iters", type=int, default=300)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp64"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp64", "fp16"])
+    ap.add_argument("--batch-extra", default="fp16", choices=["fp16", "bf16", "fp32", "fp64", "none"],
+                    help="a second precision for the batch legs (FP16 matrix values with FP32 iterates and FP64 "
+                         "refinement by default), reported beside --precision")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch each check block's kernels directly instead of replaying a hipGraph (profiled "
                          "runs: graph replays under rocprofv3 kernel tracing have ended in a host fault)")
@@ -261,22 +264,28 @@ def bench_batch(vp, kind, args, budget_left):
     log(f"batch {kind}: {cnt} LPs, sum nnz={nnz}, generated in {time.perf_counter() - t0:.1f}s")
     eng = vp.Engine(0)
     eng.upload_batch(lps)
-    p = vp.default_params(vp.MODE_RESTARTED, precision=args.precision, tolerance=args.tol,
-                          max_iterations=args.max_iters, time_limit=args.time_limit, verbose=VERBOSE,
-                          use_graph=USE_GRAPH)
-    t0 = time.perf_counter()
-    res = eng.solve_batch(p)
-    t_batch = time.perf_counter() - t0
-    ok = sum(1 for r in res if r.status == 1)
-    worst = max(r.rel_kkt for r in res)
-    log(f"batch {kind}: {ok}/{cnt} optimal in {t_batch:.2f}s (max iters {max(r.iterations for r in res)}), "
-        f"worst rel_kkt {worst:.2e}, check interval {res[0].check_interval}, "
-        f"lanes {res[0].lanes_A},{res[0].lanes_At}")
+
+    def run(prec):
+        p = vp.default_params(vp.MODE_RESTARTED, precision=prec, tolerance=args.tol,
+                              max_iterations=args.max_iters, time_limit=args.time_limit, verbose=VERBOSE,
+                              use_graph=USE_GRAPH)
+        t0 = time.perf_counter()
+        res = eng.solve_batch(p)
+        t_batch = time.perf_counter() - t0
+        ok = sum(1 for r in res if r.status == 1)
+        worst = max(r.rel_kkt for r in res)
+        log(f"batch {kind} {prec}: {ok}/{cnt} optimal in {t_batch:.2f}s (max iters {max(r.iterations for r in res)}), "
+            f"worst rel_kkt {worst:.2e}, check interval {res[0].check_interval}, "
+            f"lanes {res[0].lanes_A},{res[0].lanes_At}")
+        return dict(optimal=ok, worst_rel_kkt=worst, check_interval=res[0].check_interval,
+                    lanes=[res[0].lanes_A, res[0].lanes_At], max_iterations=max(r.iterations for r in res),
+                    batch_s=t_batch, batch_lps_per_s=cnt / t_batch, precision=prec)
+
+    out = dict(count=cnt, sum_nnz=nnz, **run(args.precision))
+    extra_prec = args.batch_extra if args.batch_extra not in ("none", args.precision) else None
+    if extra_prec and budget_left(f"batch {kind} {extra_prec}"):
+        out[extra_prec] = run(extra_prec)
     eng.close()
-    out = dict(count=cnt, sum_nnz=nnz, optimal=ok, worst_rel_kkt=worst, check_interval=res[0].check_interval,
-               lanes=[res[0].lanes_A, res[0].lanes_At],
-               max_iterations=max(r.iterations for r in res), batch_s=t_batch, batch_lps_per_s=cnt / t_batch,
-               precision=args.precision)
     if seq_n <= 0 or not budget_left(f"sequential {kind}"):
         return out
     # sequential plain FP64 single solves, same tolerance: every k-th instance of the batch
@@ -294,11 +303,15 @@ def bench_batch(vp, kind, args, budget_left):
         seq_it += r.iterations
     e.close()
     seq_lps = len(sample) / seq_t
-    log(f"batch {kind}: {cnt / t_batch:.1f} LPs/s batched vs {seq_lps:.1f} LPs/s sequential fp64 "
-        f"({seq_ok}/{len(sample)} optimal, every {step}th instance)")
+    log(f"batch {kind}: {out['batch_lps_per_s']:.1f} LPs/s batched ({out['precision']}) vs {seq_lps:.1f} LPs/s "
+        f"sequential fp64 ({seq_ok}/{len(sample)} optimal, every {step}th instance)")
     out.update(sequential_sample=len(sample), sequential_every=step, sequential_optimal=seq_ok,
                sequential_s=seq_t, sequential_lps_per_s=seq_lps, sequential_mean_iterations=seq_it / len(sample),
-               speedup=(cnt / t_batch) / seq_lps)
+               speedup=out["batch_lps_per_s"] / seq_lps)
+    if extra_prec and extra_prec in out:
+        out[extra_prec]["speedup"] = out[extra_prec]["batch_lps_per_s"] / seq_lps
+        log(f"batch {kind}: {out[extra_prec]['batch_lps_per_s']:.1f} LPs/s batched ({extra_prec}) -> "
+            f"{out[extra_prec]['speedup']:.1f}x sequential")
     return out
 
 

@@ -8,7 +8,7 @@
 //
 // PDHG keys read on this path: "tolerance", "gap_tolerance", "check_frequency", "verbose"
 // (reference, pdhg.cpp:62-67) and the engine's own: "pdhg_mode" ("restarted" | "reference"),
-// "pdhg_precision" ("fp64" | "fp32" | "bf16"), "pdhg_graph" (bool), "allow_cpu_fallback"
+// "pdhg_precision" ("fp64" | "fp32" | "bf16" | "fp16"), "pdhg_graph" (bool), "allow_cpu_fallback"
 // (bool: with no device, run the CPU PDHG instead of throwing).
 #pragma once
 #include <cstdint>

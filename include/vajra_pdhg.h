@@ -60,7 +60,9 @@ enum {
 enum {
     VJ_PREC_FP64 = 0,       /* FP64 everywhere */
     VJ_PREC_FP32 = 1,       /* FP32 iterates, matrix and projections; FP64 refinement */
-    VJ_PREC_BF16 = 2        /* BF16 matrix values, FP32 accumulate/iterates; FP64 refinement */
+    VJ_PREC_BF16 = 2,       /* BF16 matrix values, FP32 accumulate/iterates; FP64 refinement */
+    VJ_PREC_FP16 = 3        /* FP16 matrix values (the scaled matrix must fit |a| <= 65504),
+                               FP32 accumulate/iterates; FP64 refinement */
 };
 /* SolveStatus numbering of the reference (include/indus/types.hpp:34-47). */
 enum {
@@ -140,8 +142,9 @@ typedef struct vj_params {
     int power_iterations;          /* RESTARTED: power-iteration steps for ||A||_2; default 64 */
     double restart_drift;          /* RESTARTED: also restart (an FP64 refinement step) when a check's
                                       FP64 rel-KKT exceeds restart_drift times the least rel-KKT of
-                                      the epoch; 0 = default (2 with FP32 iterates, off with FP64
-                                      and BF16), < 0 = off, > 1 = this factor for every precision */
+                                      the epoch; 0 = default (2 with FP32 values, off with FP64,
+                                      BF16 and FP16 values), < 0 = off, > 1 = this factor for every
+                                      precision */
     double primal_weight_integral; /* RESTARTED: integral gain ki of the primal-weight update (see
                                       primal_weight_smoothing), in [0, 1); default 0 (PDLP's
                                       smoothing) */

@@ -5,8 +5,9 @@ checked by the oracle: the rel-KKT recomputed on the host from the downloaded (x
 * 1,000,000 x 2,000,000 random LP (12M nonzeros, 10% of the columns scaled by 10^U[-3,3]),
   FP64 and FP32 iterates with FP64 refinement, autotuned kernels (column split, sliced ELL,
   tuned check interval, drift restarts); the two objectives agree to 1e-6 relative.
-* 512 uneven random LPs (m ~ U{500..20000}, n = 2.5 m) as one batch, FP32 and FP64.
-* 128 block-staircase LPs (T ~ U{12..96} periods of 300 x 800) as one batch, FP32 and FP64.
+* 512 uneven random LPs (m ~ U{500..20000}, n = 2.5 m) as one batch, FP32, FP64 and FP16 values.
+* 128 block-staircase LPs (T ~ U{12..96} periods of 300 x 800) as one batch, FP32, FP64 and FP16
+  values (FP16 matrix values with FP32 iterates and FP64 refinement).
 
 "Converged" is the build-defined rel-KKT of the restarted engine; the reference's own
 stopping rule (src/solvers/pdhg/pdhg.cpp:249-365) has no relative KKT error."""
@@ -92,7 +93,7 @@ def random_batch():
     eng.close()
 
 
-@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("prec", ["fp32", "fp64", "fp16"])
 def test_random_batch_512(random_batch, prec):
     lps, eng = random_batch
     print(f"[case] 512 uneven random LPs {prec} (sum nnz {sum(lp.nnz for lp in lps)})", flush=True)
@@ -109,7 +110,7 @@ def staircase_batch():
     eng.close()
 
 
-@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("prec", ["fp32", "fp64", "fp16"])
 def test_staircase_batch_128(staircase_batch, prec):
     lps, eng = staircase_batch
     print(f"[case] 128 staircase LPs {prec} (sum nnz {sum(lp.nnz for lp in lps)})", flush=True)

@@ -3,6 +3,8 @@
 The bar (build-defined rel-KKT, FP64 on the original data, evaluated both by the device
 audit and by the oracle on the downloaded point): rel-KKT <= 1e-6, and the objective within
 1e-6 (relative) of the published / dual-simplex optimum."""
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -46,7 +48,7 @@ def test_kat_fp64(engine, name, lanes):
     assert abs(got - obj) <= 1e-6 * (1 + abs(obj))
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16", "fp16"])
 def test_cfg0_to_1e6(engine, prec):
     lp = lp_cases.cfg0_lp()
     print(f"[case] restarted cfg0 {prec}", flush=True)
@@ -142,7 +144,7 @@ def test_batch_matches_single(engine):
         assert abs(k[0] - r.rel_kkt) <= 1e-9 + 1e-6 * k[0]
 
 
-@pytest.mark.parametrize("prec,sa,sat", [("fp64", 8, 4), ("fp32", 4, 2), ("bf16", 2, 8)])
+@pytest.mark.parametrize("prec,sa,sat", [("fp64", 8, 4), ("fp32", 4, 2), ("bf16", 2, 8), ("fp16", 4, 2)])
 def test_column_split_spmv(engine, prec, sa, sat):
     """The XCD column-split SpMV (block partial sums + ordered finish) solves to the same
     tolerance as the fused kernels, deterministically (two runs bit-identical)."""
@@ -214,7 +216,8 @@ def test_tile_stream_variants(engine, prec, lanes, sa, sat):
     assert abs(o1 - o0) <= 2e-5 * (1 + abs(o0)), (o1, o0)
 
 
-@pytest.mark.parametrize("prec,lanes", [("fp32", -4), ("fp32", -3), ("fp64", -3), ("bf16", -4), ("bf16", -3)])
+@pytest.mark.parametrize("prec,lanes", [("fp32", -4), ("fp32", -3), ("fp64", -3), ("bf16", -4), ("bf16", -3),
+                                        ("fp16", -3)])
 def test_sliced_ell_variants(engine, prec, lanes):
     """The sliced-ELL step kernels (slices of 64 rows, one thread per row; -4 reads FP32
     (index, value) records) forced on the 2,000 x 5,000 LP, whose rows of A vary in length so
@@ -283,14 +286,16 @@ def test_structured_batch_against_highs(engine, prec, lanes):
 
 
 @pytest.mark.parametrize("prec,stream,staged", [("fp64", -1, -5), ("fp32", -2, -6), ("bf16", -1, -5),
-                                                ("fp64", -1, -7), ("fp32", -2, -7), ("bf16", -1, -7)])
+                                                ("fp64", -1, -7), ("fp32", -2, -7), ("bf16", -1, -7),
+                                                ("fp16", -1, -5), ("fp16", -1, -7)])
 def test_staged_equals_tile_streamed(engine, prec, stream, staged):
     """The LDS-staged steps form the same products and add each row in index order, as the
     tile-streamed steps do, so one check block of a batch (before the first restart decision,
     whose residual partials the two kernels sum in different tile orders) returns the same bits.
     The batch mixes random and structured LPs, with windows wider than one LDS chunk (the
     7,000 x 17,500 instance copies its x_bar window in two chunks in FP32, three in FP64).
-    Lanes -7 reads 16-bit column offsets from each tile's window start."""
+    Lanes -7 reads 16-bit column offsets from each tile's window start (with BF16 and FP16
+    values, packed with the value into one 4-byte record)."""
     lps = [vp.gen_random_lp(4100 + i, m, int(2.5 * m)) for i, m in enumerate([500, 7000, 1300])]
     lps += [lp_cases.structured_feasible_lp(*s) for s in STRUCTURED[:3]]
     engine.upload_batch(lps)
@@ -335,6 +340,32 @@ def test_staged_batch_graph_and_stream_identical(engine, prec, lanes):
     assert it0 == it1 and max(it0) >= 4 * min(it0)
     for x0, y0, x1, y1 in zip(xs0, ys0, xs1, ys1):
         assert np.array_equal(x0, x1) and np.array_equal(y0, y1)
+
+
+def test_fp16_range_checked(engine):
+    """FP16 matrix values: column 0 of the 2,000 x 5,000 LP scaled by 1e5 (cost scaled with it,
+    bounds divided: the same LP in other units) holds entries beyond the FP16 range (65504)
+    once scaling is off, and the solve is refused with an error rather than run with infinite
+    entries. The context stays usable: the unscaled LP then solves in FP16 to rel-KKT 1e-6
+    (oracle-checked)."""
+    lp = lp_cases.cfg0_lp()
+    s = 1e5
+    vals = lp.vals.copy()
+    vals[lp.col_ptr[0]:lp.col_ptr[1]] *= s
+    c, lc, uc = lp.c.copy(), lp.col_lower.copy(), lp.col_upper.copy()
+    c[0] *= s
+    lc[0] /= s
+    uc[0] /= s
+    lp2 = dataclasses.replace(lp, vals=vals, c=c, col_lower=lc, col_upper=uc, _csr=None)
+    assert np.max(np.abs(vals)) > 65504
+    print("[case] fp16 range check", flush=True)
+    engine.upload(lp2)
+    with pytest.raises(vp.VjError) as e:
+        engine.solve(vp.default_params(vp.MODE_RESTARTED, precision="fp16", tolerance=1e-6, enable_scaling=0))
+    assert "FP16" in str(e.value)
+    r, x, rd, y = solve(engine, lp, precision="fp16")
+    assert r.status == 1, r.as_dict()
+    check_point(lp, r, x, y, 1e-6)
 
 
 def test_infeasible_lps_never_reported_optimal(engine):

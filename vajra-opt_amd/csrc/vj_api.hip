@@ -249,7 +249,7 @@ int vj_params_set(vj_ctx* ctx, const vj_params* p) {
         set_error("unknown mode");
         return VJ_ERR_INVALID_ARGUMENT;
     }
-    if (p->precision < VJ_PREC_FP64 || p->precision > VJ_PREC_BF16) {
+    if (p->precision < VJ_PREC_FP64 || p->precision > VJ_PREC_FP16) {
         set_error("unknown precision");
         return VJ_ERR_INVALID_ARGUMENT;
     }

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace vj {
 
@@ -12,8 +13,17 @@ constexpr int kBlock = 256;          // threads per workgroup (4 wavefronts of 6
 constexpr int kWave = 64;
 constexpr int kEllSlice = 64;        // rows per slice of the sliced-ELL layout (one wavefront)
 
-// BF16 matrix values are stored as raw 16-bit patterns (upper half of an FP32).
+// BF16 matrix values are stored as raw 16-bit patterns (upper half of an FP32); FP16 values
+// as IEEE binary16 patterns.
 struct bf16_t { uint16_t bits; };
+struct f16_t { uint16_t bits; };
+
+__device__ __forceinline__ float f16_bits_to_float(uint16_t b) { return static_cast<float>(__builtin_bit_cast(_Float16, b)); }
+// Round to nearest even (values beyond the FP16 range become infinite; the engine checks the
+// range before it converts).
+__device__ __forceinline__ uint16_t f16_bits_from(double v) {
+    return __builtin_bit_cast(uint16_t, static_cast<_Float16>(static_cast<float>(v)));
+}
 
 // Matrix streams are read once per iteration: non-temporal loads keep them from evicting
 // the gathered iterate vectors from L2 and the Infinity Cache.
@@ -22,6 +32,9 @@ __device__ __forceinline__ float load_val(const float* p, int64_t k) { return __
 __device__ __forceinline__ float load_val(const bf16_t* p, int64_t k) {
     const uint16_t b = __builtin_nontemporal_load(&p[k].bits);
     return __uint_as_float(static_cast<uint32_t>(b) << 16);
+}
+__device__ __forceinline__ float load_val(const f16_t* p, int64_t k) {
+    return f16_bits_to_float(__builtin_nontemporal_load(&p[k].bits));
 }
 __device__ __forceinline__ int load_idx(const int* p, int64_t k) { return __builtin_nontemporal_load(p + k); }
 

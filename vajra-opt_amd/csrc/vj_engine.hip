@@ -67,6 +67,26 @@ __global__ void convert_f32(int64_t len, const double* __restrict__ a, float* __
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < len) b[i] = static_cast<float>(a[i]);
 }
+__global__ void convert_f16(int64_t len, const double* __restrict__ a, f16_t* __restrict__ b) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < len) b[i].bits = f16_bits_from(a[i]);
+}
+// part[b] = the largest |a| over the entries workgroup b visits (grid-stride, fixed order).
+__global__ __launch_bounds__(kBlock) void max_abs_partial(int64_t len, const double* __restrict__ a,
+                                                          double* __restrict__ part) {
+    __shared__ double lds[kBlock];
+    double m = 0.0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < len;
+         i += static_cast<int64_t>(gridDim.x) * kBlock)
+        m = fmax(m, fabs(a[i]));
+    lds[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) lds[threadIdx.x] = fmax(lds[threadIdx.x], lds[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = lds[0];
+}
 __global__ void convert_bf16(int64_t len, const double* __restrict__ a, bf16_t* __restrict__ b) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= len) return;
@@ -240,6 +260,7 @@ struct FastEngine {
     // the listed tiles belong to finished instances; pruned: some list leaves tiles out.
     DevBuf tileGA, tileGAt, winGA, winGAt, tfGA, tlGA, tfGAt, tlGAt, off16A, off16At, instGA, instGAt;
     DevBuf listGA, listGAt;
+    DevBuf w16recA, w16recAt;   // BF16/FP16: 4-byte (16-bit window offset, value) records of the staged tiles
     std::vector<int> hInstGA, hInstGAt;
     int rplA = 1, rplAt = 1;   // rows per lane of the staged tiles (1 or 2)
     int nlistA = 0, nlistAt = 0;
@@ -347,7 +368,12 @@ int launch_primal(vj_ctx* ctx, FastEngine& E, const V* tv, T* x, T* xbar, const 
         };
         if (E.w16At) {
             VJ_TRY(require({E.off16At.p}, "staged primal_step (16-bit offsets)"));
-            go(Win16Entries<V>{E.off16At.as<uint16_t>(), tv});
+            if constexpr (std::is_same<V, bf16_t>::value || std::is_same<V, f16_t>::value) {
+                VJ_TRY(require({E.w16recAt.p}, "staged primal_step (16-bit value records)"));
+                go(Win16Records<V>{E.w16recAt.as<uint32_t>()});
+            } else {
+                go(Win16Entries<V>{E.off16At.as<uint16_t>(), tv});
+            }
         } else {
             VJ_TRY(with_entries<V>(E.recAt, E.recAt_buf, P.ti.as<int>(), tv, "staged records", go));
         }
@@ -470,7 +496,12 @@ int launch_dual(vj_ctx* ctx, FastEngine& E, const V* av, const T* xbar, T* y, in
         };
         if (E.w16A) {
             VJ_TRY(require({E.off16A.p}, "staged dual_step (16-bit offsets)"));
-            go(Win16Entries<V>{E.off16A.as<uint16_t>(), av});
+            if constexpr (std::is_same<V, bf16_t>::value || std::is_same<V, f16_t>::value) {
+                VJ_TRY(require({E.w16recA.p}, "staged dual_step (16-bit value records)"));
+                go(Win16Records<V>{E.w16recA.as<uint32_t>()});
+            } else {
+                go(Win16Entries<V>{E.off16A.as<uint16_t>(), av});
+            }
         } else {
             VJ_TRY(with_entries<V>(E.recA, E.recA_buf, P.ci.as<int>(), av, "staged records", go));
         }
@@ -882,12 +913,48 @@ int setup(vj_ctx* ctx, FastEngine& E) {
         hipLaunchKernelGGL(pack_records, dim3(grid_for(nnz, kBlock)), dim3(kBlock), 0, s, nnz, ti, E.tv_lo.as<float>(),
                            E.recAt_buf.as<vj_i32x2>());
         VJ_LAUNCHED();
-    } else if (E.prec == VJ_PREC_BF16) {
+    } else if (E.prec == VJ_PREC_BF16 || E.prec == VJ_PREC_FP16) {
         VJ_TRY(E.av_lo.alloc(nnz * 2));
         VJ_TRY(E.tv_lo.alloc(nnz * 2));
-        hipLaunchKernelGGL(convert_bf16, dim3(grid_for(nnz, kBlock)), dim3(kBlock), 0, s, nnz, av, E.av_lo.as<bf16_t>());
-        hipLaunchKernelGGL(convert_bf16, dim3(grid_for(nnz, kBlock)), dim3(kBlock), 0, s, nnz, tv, E.tv_lo.as<bf16_t>());
+        const bool f16 = E.prec == VJ_PREC_FP16;
+        if (f16) {   // the scaled values must fit FP16 (|a| <= 65504); A and A^T hold the same values
+            const int parts = 1024;
+            DevBuf part;
+            VJ_TRY(part.alloc(parts * 8));
+            hipLaunchKernelGGL(max_abs_partial, dim3(parts), dim3(kBlock), 0, s, nnz, av, part.as<double>());
+            VJ_LAUNCHED();
+            std::vector<double> h(parts);
+            VJ_HIP(hipMemcpyAsync(h.data(), part.p, parts * 8, hipMemcpyDeviceToHost, s));
+            VJ_HIP(hipStreamSynchronize(s));
+            const double mx = *std::max_element(h.begin(), h.end());
+            if (!(mx <= 65504.0)) {
+                set_error("FP16 matrix values: the scaled matrix holds |a| = " + std::to_string(mx) +
+                          ", beyond the FP16 range (65504); use fp32 or enable scaling");
+                return VJ_ERR_INVALID_ARGUMENT;
+            }
+            hipLaunchKernelGGL(convert_f16, dim3(grid_for(nnz, kBlock)), dim3(kBlock), 0, s, nnz, av, E.av_lo.as<f16_t>());
+            hipLaunchKernelGGL(convert_f16, dim3(grid_for(nnz, kBlock)), dim3(kBlock), 0, s, nnz, tv, E.tv_lo.as<f16_t>());
+        } else {
+            hipLaunchKernelGGL(convert_bf16, dim3(grid_for(nnz, kBlock)), dim3(kBlock), 0, s, nnz, av,
+                               E.av_lo.as<bf16_t>());
+            hipLaunchKernelGGL(convert_bf16, dim3(grid_for(nnz, kBlock)), dim3(kBlock), 0, s, nnz, tv,
+                               E.tv_lo.as<bf16_t>());
+        }
         VJ_LAUNCHED();
+        // the staged kernels' 4-byte records, where 16-bit window offsets exist
+        auto pack = [&](const DevBuf& off, const DevBuf& val, DevBuf& rec) -> int {
+            VJ_TRY(rec.alloc(nnz * 4));
+            if (f16)
+                hipLaunchKernelGGL(pack_w16<f16_t>, dim3(grid_for(nnz, kBlock)), dim3(kBlock), 0, s, nnz,
+                                   off.as<uint16_t>(), val.as<f16_t>(), rec.as<uint32_t>());
+            else
+                hipLaunchKernelGGL(pack_w16<bf16_t>, dim3(grid_for(nnz, kBlock)), dim3(kBlock), 0, s, nnz,
+                                   off.as<uint16_t>(), val.as<bf16_t>(), rec.as<uint32_t>());
+            VJ_LAUNCHED();
+            return VJ_OK;
+        };
+        if (E.w16okA) VJ_TRY(pack(E.off16A, E.av_lo, E.w16recA));
+        if (E.w16okAt) VJ_TRY(pack(E.off16At, E.tv_lo, E.w16recAt));
     }
 
     // state
@@ -962,6 +1029,8 @@ int setup_one_split(vj_ctx* ctx, FastEngine& E, int S, int rows, int cols, const
         hipLaunchKernelGGL(pack_records, dim3(g), dim3(kBlock), 0, s, nnz, idx.as<int>(), val.as<float>(),
                            rec.as<vj_i32x2>());
     }
+    else if (E.prec == VJ_PREC_FP16)
+        hipLaunchKernelGGL(gather_split_vals<f16_t>, dim3(g), dim3(kBlock), 0, s, nnz, perm.as<int>(), vals_dev, val.as<f16_t>());
     else
         hipLaunchKernelGGL(gather_split_vals<bf16_t>, dim3(g), dim3(kBlock), 0, s, nnz, perm.as<int>(), vals_dev, val.as<bf16_t>());
     VJ_LAUNCHED();
@@ -1019,6 +1088,8 @@ int setup_one_ell(vj_ctx* ctx, FastEngine& E, int rows, const int* rp, const int
         VJ_TRY(rec.alloc(static_cast<size_t>(len) * 8));
         hipLaunchKernelGGL(pack_records, dim3(g), dim3(kBlock), 0, s, len, idx.as<int>(), val.as<float>(),
                            rec.as<vj_i32x2>());
+    } else if (E.prec == VJ_PREC_FP16) {
+        hipLaunchKernelGGL(gather_split_vals<f16_t>, dim3(g), dim3(kBlock), 0, s, len, perm.as<int>(), vals_dev, val.as<f16_t>());
     } else {
         hipLaunchKernelGGL(gather_split_vals<bf16_t>, dim3(g), dim3(kBlock), 0, s, len, perm.as<int>(), vals_dev, val.as<bf16_t>());
     }
@@ -1576,6 +1647,7 @@ int fast_solve(vj_ctx* ctx, vj_result* out) {
         case VJ_PREC_FP64: rc = run_typed<double, double>(ctx, E, out, t0); break;
         case VJ_PREC_FP32: rc = run_typed<float, float>(ctx, E, out, t0); break;
         case VJ_PREC_BF16: rc = run_typed<bf16_t, float>(ctx, E, out, t0); break;
+        case VJ_PREC_FP16: rc = run_typed<f16_t, float>(ctx, E, out, t0); break;
         default: set_error("unknown precision"); return VJ_ERR_INVALID_ARGUMENT;
     }
     if (rc != VJ_OK) return rc;
@@ -1602,6 +1674,7 @@ int enqueue_blocks_any(vj_ctx* ctx, FastEngine& E, int count) {
     switch (E.prec) {
         case VJ_PREC_FP64: return enqueue_blocks<double, double>(ctx, E, count);
         case VJ_PREC_FP32: return enqueue_blocks<float, float>(ctx, E, count);
+        case VJ_PREC_FP16: return enqueue_blocks<f16_t, float>(ctx, E, count);
         default: return enqueue_blocks<bf16_t, float>(ctx, E, count);
     }
 }
@@ -1694,7 +1767,10 @@ int fast_benchmark(vj_ctx* ctx, int reps, vj_kernel_stat* out, int cap, int* wri
             rc = time_kernels<float, float>(ctx, E, E.av_lo.as<float>(), E.tv_lo.as<float>(), reps, &up, &ud);
             break;
         default:
-            rc = time_kernels<bf16_t, float>(ctx, E, E.av_lo.as<bf16_t>(), E.tv_lo.as<bf16_t>(), reps, &up, &ud);
+            if (E.prec == VJ_PREC_FP16)
+                rc = time_kernels<f16_t, float>(ctx, E, E.av_lo.as<f16_t>(), E.tv_lo.as<f16_t>(), reps, &up, &ud);
+            else
+                rc = time_kernels<bf16_t, float>(ctx, E, E.av_lo.as<bf16_t>(), E.tv_lo.as<bf16_t>(), reps, &up, &ud);
             break;
     }
     VJ_HIP(hipMemcpy(E.st.p, saved.data(), E.count * sizeof(InstState), hipMemcpyHostToDevice));
@@ -1709,7 +1785,8 @@ int fast_benchmark(vj_ctx* ctx, int reps, vj_kernel_stat* out, int cap, int* wri
     // dual_step rows of A: reads xbar (gather, n), y, yhat, lrs, urs (4m); writes y (m)
     const double b_dual = nnz * (sv + 4) + (m + 1) * 4 + n * sx + 5 * m * sx;
     // names carry the kernel variant the autotuner chose, as the profiler lists it
-    const char* pt = E.prec == VJ_PREC_FP64 ? "double" : (E.prec == VJ_PREC_FP32 ? "float" : "bf16");
+    const char* pt = E.prec == VJ_PREC_FP64 ? "double"
+                   : (E.prec == VJ_PREC_FP32 ? "float" : (E.prec == VJ_PREC_FP16 ? "f16" : "bf16"));
     char names[2][48];
     const char* ra = E.recA ? ",rec" : "";
     const char* rt = E.recAt ? ",rec" : "";

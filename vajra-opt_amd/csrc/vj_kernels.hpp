@@ -420,10 +420,36 @@ struct Win16Entries {
         a = static_cast<T>(load_val(val, k));
     }
 };
+// 16-bit values (BF16 or FP16): the 16-bit window offset (low half) and the value's bits (high
+// half) in one 4-byte record, one load per entry. For BF16 the high half is the upper half of
+// the value's FP32 bit pattern.
+template <class V>
+struct Win16Records {
+    const uint32_t* __restrict__ rec;
+    template <class T>
+    __device__ __forceinline__ void load(int64_t k, int& c, T& a) const {
+        const uint32_t r = __builtin_nontemporal_load(rec + k);
+        c = static_cast<int>(r & 0xffffu);
+        if constexpr (std::is_same<V, bf16_t>::value)
+            a = static_cast<T>(__uint_as_float(r & 0xffff0000u));
+        else
+            a = static_cast<T>(f16_bits_to_float(static_cast<uint16_t>(r >> 16)));
+    }
+};
 template <class Ent>
 struct window_relative { static constexpr bool value = false; };
 template <class V>
 struct window_relative<Win16Entries<V>> { static constexpr bool value = true; };
+template <class V>
+struct window_relative<Win16Records<V>> { static constexpr bool value = true; };
+
+// rec[k] = off16[k] | value bits << 16 (the staged kernels' 4-byte records of 16-bit values).
+template <class V>
+__global__ void pack_w16(int64_t nnz, const uint16_t* __restrict__ off16, const V* __restrict__ val,
+                         uint32_t* __restrict__ rec) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < nnz) rec[k] = static_cast<uint32_t>(off16[k]) | (static_cast<uint32_t>(val[k].bits) << 16);
+}
 
 // Loads in flight per lane in one pass of the streamed SpMV loops: every lane issues all of
 // its entry loads of a pass, then all of its gathers. Tiles hold a few entries per lane
@@ -1060,6 +1086,8 @@ __global__ void gather_split_vals(int64_t nnz, const int* __restrict__ perm, con
         out[q] = v;
     } else if constexpr (sizeof(V) == 4) {
         out[q] = static_cast<float>(v);
+    } else if constexpr (std::is_same<V, f16_t>::value) {
+        out[q].bits = f16_bits_from(v);
     } else {
         uint32_t u = __float_as_uint(static_cast<float>(v));
         if ((u & 0x7f800000u) != 0x7f800000u) u += 0x7fffu + ((u >> 16) & 1u);

@@ -39,7 +39,8 @@ vj_params params_from(const Options& o) {
     vj_params p;
     vj_params_default(&p, mode == "reference" ? VJ_MODE_REFERENCE : VJ_MODE_RESTARTED);
     const std::string prec = o.get_string("pdhg_precision", "fp64");
-    p.precision = prec == "fp32" ? VJ_PREC_FP32 : (prec == "bf16" ? VJ_PREC_BF16 : VJ_PREC_FP64);
+    p.precision = prec == "fp32" ? VJ_PREC_FP32
+                : (prec == "bf16" ? VJ_PREC_BF16 : (prec == "fp16" ? VJ_PREC_FP16 : VJ_PREC_FP64));
     p.max_iterations = o.iteration_limit;
     p.time_limit = o.time_limit;
     p.tolerance = o.get_double("tolerance", 1e-6);
@@ -54,7 +55,8 @@ vj_params params_from(const Options& o) {
 std::string kernel_mode(const vj_params& p, const vj_result& r) {
     if (p.mode == VJ_MODE_REFERENCE) return "reference_map_exact_order";
     std::string s = "restarted_halpern_";
-    s += p.precision == VJ_PREC_FP64 ? "fp64" : (p.precision == VJ_PREC_FP32 ? "fp32_refined" : "bf16_refined");
+    s += p.precision == VJ_PREC_FP64 ? "fp64"
+         : (p.precision == VJ_PREC_FP32 ? "fp32_refined" : (p.precision == VJ_PREC_FP16 ? "fp16_refined" : "bf16_refined"));
     s += r.col_splits_A > 1 || r.col_splits_At > 1 ? "_xcd_split" : (r.lanes_A == 0 ? "_tile_stream" : (r.lanes_A < 0 ? "_sliced_ell" : "_lane_groups"));
     s += p.use_graph ? "_graph" : "";
     return s;

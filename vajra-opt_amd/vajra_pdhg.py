@@ -25,10 +25,10 @@ _lib = ctypes.CDLL(LIB_PATH)
 # ---- constants (include/vajra_pdhg.h) -------------------------------------------------
 OK, ERR_INVALID_ARGUMENT, ERR_NO_DEVICE, ERR_ALLOC, ERR_LAUNCH, ERR_NOT_READY, ERR_UNSUPPORTED = range(7)
 MODE_REFERENCE, MODE_RESTARTED = 0, 1
-PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
+PREC_FP64, PREC_FP32, PREC_BF16, PREC_FP16 = 0, 1, 2, 3
 STATUS = {0: "NOT_SOLVED", 1: "OPTIMAL", 2: "INFEASIBLE", 3: "UNBOUNDED", 5: "FEASIBLE",
           6: "ITERATION_LIMIT", 7: "TIME_LIMIT", 9: "NUMERICAL_ERROR"}
-PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16": PREC_BF16}
+PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16": PREC_BF16, "fp16": PREC_FP16}
 
 _c_int, _c_i64, _c_dbl, _vp = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
 
