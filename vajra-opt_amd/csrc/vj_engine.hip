@@ -652,9 +652,10 @@ int tile_instance_ranges(hipStream_t s, int ntiles, const DevBuf& tile_row, cons
     return VJ_OK;
 }
 
-// Rows per lane of a staged tile: 2 where 2,048 rows of the average length fill at most the
-// tile's entry capacity (short rows: the columns of a random instance, 6 entries), else 1.
-int stage_rows_per_lane(double avg_row, int cap) { return 2.0 * kStageThreads * avg_row <= cap ? 2 : 1; }
+// Rows per lane of a staged tile: 2 where 1,024 rows of the average length fill at most three
+// quarters of the tile's entry capacity (short rows: the columns of a random instance, 6
+// entries, or of a staircase instance, about 8), else 1. Tiles still stop at the capacity.
+int stage_rows_per_lane(double avg_row, int cap) { return kStageThreads * avg_row <= 0.75 * cap ? 2 : 1; }
 
 // Row tiles of the LDS-staged kernels, built on the device once per upload and entry size:
 // for batches (the kernels are autotuner candidates there), and for a single LP only when
