@@ -260,6 +260,7 @@ struct FastEngine {
     // the listed tiles belong to finished instances; pruned: some list leaves tiles out.
     DevBuf tileGA, tileGAt, winGA, winGAt, tfGA, tlGA, tfGAt, tlGAt, off16A, off16At, instGA, instGAt;
     DevBuf listGA, listGAt;
+    DevBuf descGA, descGAt;   // per staged tile: rows, entries, window, instance (8 ints)
     DevBuf w16recA, w16recAt;   // BF16/FP16: 4-byte (16-bit window offset, value) records of the staged tiles
     std::vector<int> hInstGA, hInstGAt;
     int rplA = 1, rplAt = 1;   // rows per lane of the staged tiles (1 or 2)
@@ -347,14 +348,14 @@ int launch_primal(vj_ctx* ctx, FastEngine& E, const V* tv, T* x, T* xbar, const 
         return VJ_OK;
     }
     if (E.stageAt) {
-        VJ_TRY(require({E.tileGAt.p, E.winGAt.p, E.instGAt.p, E.listGAt.p}, "staged primal_step"));
+        VJ_TRY(require({E.descGAt.p, E.listGAt.p}, "staged primal_step"));
         if (E.nlistAt == 0) return VJ_OK;   // every instance has finished
         auto go = [&](auto ent) {
             using Ent = decltype(ent);
             auto go2 = [&](auto multi, auto rpl) {
                 hipLaunchKernelGGL((staged_primal_step<Ent, T, decltype(multi)::value, decltype(rpl)::value>),
                                    dim3(E.nlistAt), dim3(kStageThreads), 0, ctx->stream, E.listGAt.as<int>(),
-                                   E.tileGAt.as<int>(), E.instGAt.as<int>(), E.winGAt.as<int>(), P.tp.as<int>(), ent, y,
+                                   E.descGAt.as<int4>(), P.tp.as<int>(), ent, y,
                                    E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), x, xbar, E.xhat.as<double>(),
                                    E.xfull.as<double>(), E.st.as<InstState>(), E.part_x.as<double>(), it, flags, E.rho);
             };
@@ -475,14 +476,14 @@ int launch_dual(vj_ctx* ctx, FastEngine& E, const V* av, const T* xbar, T* y, in
         return VJ_OK;
     }
     if (E.stageA) {
-        VJ_TRY(require({E.tileGA.p, E.winGA.p, E.instGA.p, E.listGA.p}, "staged dual_step"));
+        VJ_TRY(require({E.descGA.p, E.listGA.p}, "staged dual_step"));
         if (E.nlistA == 0) return VJ_OK;
         auto go = [&](auto ent) {
             using Ent = decltype(ent);
             auto go2 = [&](auto multi, auto rpl) {
                 hipLaunchKernelGGL((staged_dual_step<Ent, T, decltype(multi)::value, decltype(rpl)::value>),
                                    dim3(E.nlistA), dim3(kStageThreads), 0, ctx->stream, E.listGA.as<int>(),
-                                   E.tileGA.as<int>(), E.instGA.as<int>(), E.winGA.as<int>(), P.rp.as<int>(), ent, xbar,
+                                   E.descGA.as<int4>(), P.rp.as<int>(), ent, xbar,
                                    yht, lrs, urs, y, E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(),
                                    E.part_y.as<double>(), it, flags, E.rho);
             };
@@ -683,6 +684,8 @@ int setup_stage(vj_ctx* ctx, FastEngine& E) {
             VJ_TRY(device_window_offsets(s, E.ntGA, E.tileGA.as<int>(), P.rp.as<int>(), P.ci.as<int>(),
                                          E.winGA.as<int>(), E.nnz, E.off16A));
         VJ_TRY(device_tile_instances(s, E.ntGA, E.tileGA.as<int>(), rinst, E.instGA));
+        VJ_TRY(device_stage_descs(s, E.ntGA, E.tileGA.as<int>(), P.rp.as<int>(), E.winGA.as<int>(), E.instGA.as<int>(),
+                                  E.descGA));
         VJ_TRY(E.listGA.alloc(std::max<size_t>(static_cast<size_t>(E.ntGA), 1) * 4));
         E.hInstGA.resize(E.ntGA);
         if (E.ntGA > 0)
@@ -704,6 +707,8 @@ int setup_stage(vj_ctx* ctx, FastEngine& E) {
             VJ_TRY(device_window_offsets(s, E.ntGAt, E.tileGAt.as<int>(), P.tp.as<int>(), P.ti.as<int>(),
                                          E.winGAt.as<int>(), E.nnz, E.off16At));
         VJ_TRY(device_tile_instances(s, E.ntGAt, E.tileGAt.as<int>(), cinst, E.instGAt));
+        VJ_TRY(device_stage_descs(s, E.ntGAt, E.tileGAt.as<int>(), P.tp.as<int>(), E.winGAt.as<int>(),
+                                  E.instGAt.as<int>(), E.descGAt));
         VJ_TRY(E.listGAt.alloc(std::max<size_t>(static_cast<size_t>(E.ntGAt), 1) * 4));
         E.hInstGAt.resize(E.ntGAt);
         if (E.ntGAt > 0)

@@ -765,18 +765,17 @@ __device__ __forceinline__ void stage_partials(const double (&v)[3], int slot, d
 // enough entries to pay for copying its window.
 template <class Ent, class T, bool MULTI, int RPL>
 __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8))) void staged_primal_step(
-    const int* __restrict__ list, const int* __restrict__ tile_row, const int* __restrict__ tile_inst,
-    const int* __restrict__ win, const int* __restrict__ tp, Ent ent, const T* __restrict__ y, const T* __restrict__ dhat,
+    const int* __restrict__ list, const int4* __restrict__ desc, const int* __restrict__ tp, Ent ent, const T* __restrict__ y, const T* __restrict__ dhat,
     const T* __restrict__ lcs, const T* __restrict__ ucs, T* __restrict__ x, T* __restrict__ xbar,
     const double* __restrict__ xhat, double* __restrict__ xfull, const InstState* __restrict__ st,
     double* __restrict__ part, int it, int flags, double rho) {
     __shared__ T lds[stage_cap<T>()];
     __shared__ double red[kStageThreads / kWave][3];
     const int tile = list[blockIdx.x];
-    const int my = MULTI ? tile_inst[tile] : 0;   // every row of the tile belongs to this instance
+    const int4 d0 = desc[2 * tile], d1 = desc[2 * tile + 1];   // rows, entries | window, instance
+    const int my = MULTI ? d1.z : 0;   // every row of the tile belongs to this instance
     if (st[my].active == 0) return;
-    const int r0 = tile_row[tile], r1 = tile_row[tile + 1];
-    const int s0 = tp[r0], s1 = tp[r1];
+    const int r0 = d0.x, r1 = d0.y, s0 = d0.z, s1 = d0.w;
     double c[3] = {0.0, 0.0, 0.0};
     if (s1 - s0 > stage_cap<T>()) {   // one long row
         const T acc = staged_long_row<Ent, T>(s0, s1, ent, y, lds);
@@ -794,7 +793,7 @@ __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8
             q0[u] = tp[jc] - s0;
             q1[u] = tp[jc + 1] - s0;
         }
-        staged_products<Ent, T>(s0, s1, win[2 * tile], win[2 * tile + 1], ent, y, lds);
+        staged_products<Ent, T>(s0, s1, d1.x, d1.y, ent, y, lds);
 #pragma unroll
         for (int u = 0; u < RPL; ++u) {
             const int j = r0 + static_cast<int>(threadIdx.x) + u * kStageThreads;
@@ -813,18 +812,17 @@ __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8
 
 template <class Ent, class T, bool MULTI, int RPL>
 __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8))) void staged_dual_step(
-    const int* __restrict__ list, const int* __restrict__ tile_row, const int* __restrict__ tile_inst,
-    const int* __restrict__ win, const int* __restrict__ rp, Ent ent, const T* __restrict__ xbar, const T* __restrict__ yhat_t,
+    const int* __restrict__ list, const int4* __restrict__ desc, const int* __restrict__ rp, Ent ent, const T* __restrict__ xbar, const T* __restrict__ yhat_t,
     const T* __restrict__ lrs, const T* __restrict__ urs, T* __restrict__ y, const double* __restrict__ yhat,
     double* __restrict__ yfull, const InstState* __restrict__ st, double* __restrict__ part, int it, int flags,
     double rho) {
     __shared__ T lds[stage_cap<T>()];
     __shared__ double red[kStageThreads / kWave][3];
     const int tile = list[blockIdx.x];
-    const int my = MULTI ? tile_inst[tile] : 0;
+    const int4 d0 = desc[2 * tile], d1 = desc[2 * tile + 1];
+    const int my = MULTI ? d1.z : 0;
     if (st[my].active == 0) return;
-    const int r0 = tile_row[tile], r1 = tile_row[tile + 1];
-    const int s0 = rp[r0], s1 = rp[r1];
+    const int r0 = d0.x, r1 = d0.y, s0 = d0.z, s1 = d0.w;
     double c[3] = {0.0, 0.0, 0.0};
     if (s1 - s0 > stage_cap<T>()) {
         const T w = staged_long_row<Ent, T>(s0, s1, ent, xbar, lds);
@@ -840,7 +838,7 @@ __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8
             q0[u] = rp[ic] - s0;
             q1[u] = rp[ic + 1] - s0;
         }
-        staged_products<Ent, T>(s0, s1, win[2 * tile], win[2 * tile + 1], ent, xbar, lds);
+        staged_products<Ent, T>(s0, s1, d1.x, d1.y, ent, xbar, lds);
 #pragma unroll
         for (int u = 0; u < RPL; ++u) {
             const int i = r0 + static_cast<int>(threadIdx.x) + u * kStageThreads;

@@ -166,6 +166,24 @@ __global__ __launch_bounds__(kBlock) void window_offsets(int ntiles, const int* 
     for (int p = s0 + static_cast<int>(threadIdx.x); p < s1; p += kBlock) off16[p] = static_cast<uint16_t>(idx[p] - lo);
 }
 
+// desc[8t .. 8t+7] = {first row, end row, first entry, end entry, window start, window end,
+// instance, 0} of tile t: everything a staged workgroup needs before its loads, in one place.
+__global__ void stage_descs(int ntiles, const int* __restrict__ tile_row, const int* __restrict__ ptr,
+                            const int* __restrict__ win, const int* __restrict__ inst, int* __restrict__ desc) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    const int r0 = tile_row[t], r1 = tile_row[t + 1];
+    int* d = desc + 8 * static_cast<int64_t>(t);
+    d[0] = r0;
+    d[1] = r1;
+    d[2] = ptr[r0];
+    d[3] = ptr[r1];
+    d[4] = win[2 * t];
+    d[5] = win[2 * t + 1];
+    d[6] = inst[t];
+    d[7] = 0;
+}
+
 // inst[t] = the instance of tile t's first row (0 without an instance map).
 __global__ void tile_instances(int ntiles, const int* __restrict__ tile_row, const int* __restrict__ inst_of,
                                int* __restrict__ inst) {
@@ -428,6 +446,17 @@ int device_tile_instances(hipStream_t s, int ntiles, const int* tile_row, const 
     if (ntiles > 0) {
         hipLaunchKernelGGL(tile_instances, dim3(grid_for(ntiles, kBlock)), dim3(kBlock), 0, s, ntiles, tile_row, inst_of,
                            inst.as<int>());
+        VJ_LAUNCHED();
+    }
+    return VJ_OK;
+}
+
+int device_stage_descs(hipStream_t s, int ntiles, const int* tile_row, const int* ptr, const int* win,
+                       const int* inst, DevBuf& desc) {
+    VJ_TRY(desc.alloc(std::max<size_t>(static_cast<size_t>(ntiles), 1) * 32));
+    if (ntiles > 0) {
+        hipLaunchKernelGGL(stage_descs, dim3(grid_for(ntiles, kBlock)), dim3(kBlock), 0, s, ntiles, tile_row, ptr, win,
+                           inst, desc.as<int>());
         VJ_LAUNCHED();
     }
     return VJ_OK;

@@ -32,6 +32,11 @@ int device_window_offsets(hipStream_t s, int ntiles, const int* tile_row, const 
 // inst[t] = the instance of the first row of tile t (inst_of[tile_row[t]]; 0 when inst_of is null).
 int device_tile_instances(hipStream_t s, int ntiles, const int* tile_row, const int* inst_of, DevBuf& inst);
 
+// Descriptors of staged tiles, 8 ints each: first row, end row, first entry (ptr[first row]),
+// end entry, window start, window end (win), instance (inst), 0.
+int device_stage_descs(hipStream_t s, int ntiles, const int* tile_row, const int* ptr, const int* win,
+                       const int* inst, DevBuf& desc);
+
 // Per instance (rows off[q]..off[q+1]-1): the tiles holding its first and last row.
 int device_instance_tiles(hipStream_t s, int count, int ntiles, const int* tile_row, const int* off, DevBuf& first,
                           DevBuf& last);
