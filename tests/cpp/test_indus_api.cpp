@@ -539,7 +539,7 @@ void test_gpu_trivial_sizes() {
 void test_mixed_precision_option() {
     const indus::Model M = indus::io::read_mps(g_dir + "/crude_blend.mps");
     indus::Options o = pdhg_options(1e-8, 200000);
-    for (const char* prec : {"fp32", "bf16"}) {
+    for (const char* prec : {"fp32", "bf16", "fp16"}) {
         o.set("pdhg_precision", prec);
         const indus::Solution s = indus::solve(M, o);
         CHECK(s.status == indus::SolveStatus::kOptimal);
