@@ -720,41 +720,36 @@ int setup_stage(vj_ctx* ctx, FastEngine& E) {
     return VJ_OK;
 }
 
-// The staged launches' tile lists: every tile (hs null), or the tiles of the instances active
-// in hs. The caller has drained the stream (no kernel reads the lists); an executable graph
-// captured with the old grids is released.
-int set_stage_lists(vj_ctx* ctx, FastEngine& E, const InstState* hs) {
+// The batch launches' tile lists (staged and, for batches, tile-streamed steps): every tile (hs
+// null), or the tiles holding a row of an instance active in hs. The caller has drained the
+// stream (no kernel reads the lists); an executable graph captured with the old grids is
+// released.
+int set_launch_lists(vj_ctx* ctx, FastEngine& E, const InstState* hs) {
     hipStream_t s = ctx->stream;
-    auto build = [&](const std::vector<int>& inst, DevBuf& list, int& n) -> int {
+    auto active = [&](int q) { return !hs || hs[q].active != 0; };
+    // list the tiles t < count for which live(t) holds, in ascending order
+    auto upload = [&](int count, auto live, DevBuf& list, int& n) -> int {
         std::vector<int> h;
-        h.reserve(inst.size());
-        for (int t = 0; t < static_cast<int>(inst.size()); ++t)
-            if (!hs || hs[inst[t]].active) h.push_back(t);
+        h.reserve(count);
+        for (int t = 0; t < count; ++t)
+            if (live(t)) h.push_back(t);
         n = static_cast<int>(h.size());
         if (n > 0) VJ_HIP(hipMemcpyAsync(list.p, h.data(), static_cast<size_t>(n) * 4, hipMemcpyHostToDevice, s));
         VJ_HIP(hipStreamSynchronize(s));   // h is released on return
         return VJ_OK;
     };
-    // tile-streamed tiles of a batch may hold rows of several instances: listed while one is active
-    auto build_ranges = [&](const std::vector<int>& first, const std::vector<int>& last, DevBuf& list,
-                            int& n) -> int {
-        std::vector<int> h;
-        h.reserve(first.size());
-        for (int t = 0; t < static_cast<int>(first.size()); ++t) {
-            bool live = !hs;
-            for (int q = first[t]; !live && q <= last[t]; ++q) live = hs[q].active != 0;
-            if (live) h.push_back(t);
-        }
-        n = static_cast<int>(h.size());
-        if (n > 0) VJ_HIP(hipMemcpyAsync(list.p, h.data(), static_cast<size_t>(n) * 4, hipMemcpyHostToDevice, s));
-        VJ_HIP(hipStreamSynchronize(s));
-        return VJ_OK;
+    // a tile-streamed tile of a batch may hold rows of several instances: listed while one is active
+    auto any_active = [&](int first, int last) {
+        for (int q = first; q <= last; ++q)
+            if (active(q)) return true;
+        return false;
     };
-    if (E.gcapA) VJ_TRY(build(E.hInstGA, E.listGA, E.nlistA));
-    if (E.gcapAt) VJ_TRY(build(E.hInstGAt, E.listGAt, E.nlistAt));
+    if (E.gcapA) VJ_TRY(upload(E.ntGA, [&](int t) { return active(E.hInstGA[t]); }, E.listGA, E.nlistA));
+    if (E.gcapAt) VJ_TRY(upload(E.ntGAt, [&](int t) { return active(E.hInstGAt[t]); }, E.listGAt, E.nlistAt));
     if (E.multi && E.tiles_built) {
-        VJ_TRY(build_ranges(E.hFirstTA, E.hLastTA, E.listTA, E.nlistTA));
-        VJ_TRY(build_ranges(E.hFirstTAt, E.hLastTAt, E.listTAt, E.nlistTAt));
+        VJ_TRY(upload(E.ntA, [&](int t) { return any_active(E.hFirstTA[t], E.hLastTA[t]); }, E.listTA, E.nlistTA));
+        VJ_TRY(upload(E.ntAt, [&](int t) { return any_active(E.hFirstTAt[t], E.hLastTAt[t]); }, E.listTAt,
+                      E.nlistTAt));
     }
     E.pruned = hs != nullptr && ((E.gcapA && E.nlistA < E.ntGA) || (E.gcapAt && E.nlistAt < E.ntGAt) ||
                                  (E.multi && E.tiles_built && (E.nlistTA < E.ntA || E.nlistTAt < E.ntAt)));
@@ -1402,7 +1397,7 @@ int capture_block(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
 // stream and list only the tiles of the instances active now; the block graph is captured
 // again with the new grids.
 template <class V, class T>
-int prune_stage_lists(vj_ctx* ctx, FastEngine& E, const InstState* polled, const InstState* latest, bool graph,
+int prune_launch_lists(vj_ctx* ctx, FastEngine& E, const InstState* polled, const InstState* latest, bool graph,
                       const V* av, const V* tv) {
     if (E.count < 2) return VJ_OK;
     auto live = [&](const std::vector<int>& inst) {
@@ -1427,7 +1422,7 @@ int prune_stage_lists(vj_ctx* ctx, FastEngine& E, const InstState* polled, const
                     (sat && live_ranges(E.hFirstTAt, E.hLastTAt) * 2 <= E.nlistTAt);
     if (!a && !at) return VJ_OK;
     VJ_HIP(hipStreamSynchronize(ctx->stream));   // `latest` now holds the state after the last block
-    VJ_TRY(set_stage_lists(ctx, E, latest));
+    VJ_TRY(set_launch_lists(ctx, E, latest));
     if (graph) VJ_TRY((capture_block<V, T>(ctx, E, av, tv)));
     return VJ_OK;
 }
@@ -1450,7 +1445,7 @@ int run_typed(vj_ctx* ctx, FastEngine& E, vj_result* out, clk::time_point t_star
     VJ_TRY(setup_ell(ctx, E));
     VJ_HIP(hipStreamSynchronize(s));
     const double layout_time = secs(t_layout, clk::now());
-    if (E.gcapA || E.gcapAt || E.multi) VJ_TRY(set_stage_lists(ctx, E, nullptr));   // every tile
+    if (E.gcapA || E.gcapAt || E.multi) VJ_TRY(set_launch_lists(ctx, E, nullptr));   // every tile
     VJ_TRY(launch_restart<T>(ctx, E, 1));  // initial centring
     VJ_TRY((autotune<V, T>(ctx, E, av, tv)));
     // after autotuning the scratch runs left the real iterates untouched; recentre anyway
@@ -1502,7 +1497,7 @@ int run_typed(vj_ctx* ctx, FastEngine& E, vj_result* out, clk::time_point t_star
             }
             if (active == 0) { all_done = true; break; }
             if (secs(t_start, clk::now()) > prm.time_limit) { timed_out = true; break; }
-            VJ_TRY((prune_stage_lists<V, T>(ctx, E, hs, E.h_state[b & 1], use_graph, av, tv)));
+            VJ_TRY((prune_launch_lists<V, T>(ctx, E, hs, E.h_state[b & 1], use_graph, av, tv)));
             if (prm.verbose >= 1 && secs(t_report, clk::now()) >= 10.0) {
                 t_report = clk::now();
                 int64_t it = 0;
@@ -1700,7 +1695,7 @@ int fast_benchmark_blocks(vj_ctx* ctx, int warmup, int steps, double* ms_per_blo
     VJ_HIP(hipStreamSynchronize(s));
     // every instance iterates below: staged launches list every tile again (the graph of the
     // solve, captured with pruned grids, is released and the blocks are launched directly)
-    if (E.pruned) VJ_TRY(set_stage_lists(ctx, E, nullptr));
+    if (E.pruned) VJ_TRY(set_launch_lists(ctx, E, nullptr));
     // every instance iterates through all blocks: none may converge or reach its limit
     live = saved;
     for (auto& z : live) {
@@ -1760,7 +1755,7 @@ int fast_benchmark(vj_ctx* ctx, int reps, vj_kernel_stat* out, int cap, int* wri
     VJ_HIP(hipMemcpy(saved.data(), E.st.p, E.count * sizeof(InstState), hipMemcpyDeviceToHost));
     if (E.pruned) {   // staged launches list every tile again
         VJ_HIP(hipStreamSynchronize(ctx->stream));
-        VJ_TRY(set_stage_lists(ctx, E, nullptr));
+        VJ_TRY(set_launch_lists(ctx, E, nullptr));
     }
     active = saved;
     for (auto& z : active) z.active = 1;
