@@ -80,9 +80,10 @@ def main():
                     help="staircase instances solved one by one (all 128 by default)")
     ap.add_argument("--cpu-iters", type=int, default=300)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp64", "fp16"])
-    ap.add_argument("--batch-extra", default="fp16", choices=["fp16", "bf16", "fp32", "fp64", "none"],
-                    help="a second precision for the batch legs (FP16 matrix values with FP32 iterates and FP64 "
-                         "refinement by default), reported beside --precision")
+    ap.add_argument("--batch-extra", default="fp16,fp64",
+                    help="further precisions of the batch legs, comma-separated (fp16: FP16 matrix values with FP32 "
+                         "iterates and FP64 refinement; fp64: the plain FP64 engine; 'none' for neither), each "
+                         "reported beside --precision")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch each check block's kernels directly instead of replaying a hipGraph (profiled "
                          "runs: graph replays under rocprofv3 kernel tracing have ended in a host fault)")
@@ -282,9 +283,12 @@ def bench_batch(vp, kind, args, budget_left):
                     batch_s=t_batch, batch_lps_per_s=cnt / t_batch, precision=prec)
 
     out = dict(count=cnt, sum_nnz=nnz, **run(args.precision))
-    extra_prec = args.batch_extra if args.batch_extra not in ("none", args.precision) else None
-    if extra_prec and budget_left(f"batch {kind} {extra_prec}"):
-        out[extra_prec] = run(extra_prec)
+    extras = [p for p in args.batch_extra.split(",") if p and p not in ("none", args.precision)]
+    for prec in extras:
+        if prec not in ("fp16", "bf16", "fp32", "fp64"):
+            raise SystemExit(f"--batch-extra: unknown precision {prec}")
+        if budget_left(f"batch {kind} {prec}"):
+            out[prec] = run(prec)
     eng.close()
     if seq_n <= 0 or not budget_left(f"sequential {kind}"):
         return out
@@ -308,10 +312,11 @@ def bench_batch(vp, kind, args, budget_left):
     out.update(sequential_sample=len(sample), sequential_every=step, sequential_optimal=seq_ok,
                sequential_s=seq_t, sequential_lps_per_s=seq_lps, sequential_mean_iterations=seq_it / len(sample),
                speedup=out["batch_lps_per_s"] / seq_lps)
-    if extra_prec and extra_prec in out:
-        out[extra_prec]["speedup"] = out[extra_prec]["batch_lps_per_s"] / seq_lps
-        log(f"batch {kind}: {out[extra_prec]['batch_lps_per_s']:.1f} LPs/s batched ({extra_prec}) -> "
-            f"{out[extra_prec]['speedup']:.1f}x sequential")
+    for prec in extras:
+        if prec in out:
+            out[prec]["speedup"] = out[prec]["batch_lps_per_s"] / seq_lps
+            log(f"batch {kind}: {out[prec]['batch_lps_per_s']:.1f} LPs/s batched ({prec}) -> "
+                f"{out[prec]['speedup']:.1f}x sequential")
     return out
 
 
