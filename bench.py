@@ -42,12 +42,13 @@ def log(*a):
 
 USE_GRAPH = 1
 VERBOSE = 1
+REFINE = 0
 
 
 def solve_timed(vp, eng, prec, tol, max_iters, time_limit, force=None):
     # verbose=1: the engine prints a progress line to stderr every 10 s of a long solve; 2 also
     # prints the autotuner's timings
-    kw = dict(use_graph=USE_GRAPH)
+    kw = dict(use_graph=USE_GRAPH, refine_every_check=REFINE)
     if force:
         la, lat, sa, sat = (int(v) for v in force.split(","))
         kw.update(lanes_A=la, lanes_At=lat, col_splits_A=sa, col_splits_At=sat)
@@ -90,6 +91,9 @@ def main():
     ap.add_argument("--force-kernels", default=None, metavar="LANES_A,LANES_AT,SPLITS_A,SPLITS_AT",
                     help="fix the headline solve's kernel variants (vj_params lanes_A, lanes_At, col_splits_A, "
                          "col_splits_At) instead of autotuning them, e.g. to profile the variants a timed run chose")
+    ap.add_argument("--refine-every-check", type=int, default=0, choices=[-1, 0, 1],
+                    help="vj_params refine_every_check of the headline and batch solves (1: the FP64 centre "
+                         "moves at every check, -1: at restarts only, 0: the engine's default per precision)")
     ap.add_argument("--verbose", type=int, default=1, choices=[1, 2],
                     help="2: the engine also prints the autotuner's candidate timings")
     args = ap.parse_args()
@@ -99,9 +103,10 @@ def main():
         sys.exit(2)
     if args.steps < 1 or args.warmup < 0:
         raise SystemExit("--steps must be >= 1 and --warmup >= 0")
-    global USE_GRAPH, VERBOSE
+    global USE_GRAPH, VERBOSE, REFINE
     USE_GRAPH = 0 if args.no_graph else 1
     VERBOSE = args.verbose
+    REFINE = args.refine_every_check
 
     import vajra_pdhg as vp
 
@@ -269,7 +274,7 @@ def bench_batch(vp, kind, args, budget_left):
     def run(prec):
         p = vp.default_params(vp.MODE_RESTARTED, precision=prec, tolerance=args.tol,
                               max_iterations=args.max_iters, time_limit=args.time_limit, verbose=VERBOSE,
-                              use_graph=USE_GRAPH)
+                              use_graph=USE_GRAPH, refine_every_check=REFINE)
         t0 = time.perf_counter()
         res = eng.solve_batch(p)
         t_batch = time.perf_counter() - t0

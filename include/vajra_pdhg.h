@@ -148,6 +148,11 @@ typedef struct vj_params {
     double primal_weight_integral; /* RESTARTED: integral gain ki of the primal-weight update (see
                                       primal_weight_smoothing), in [0, 1); default 0 (PDLP's
                                       smoothing) */
+    int refine_every_check;        /* RESTARTED: where the FP64 centre moves (the refinement step of
+                                      FP32/BF16/FP16 iterates): 1 = at every check, keeping the
+                                      Halpern epoch (its anchor becomes an explicit vector); -1 = at
+                                      restarts only; 0 = default (every check with BF16 and FP16
+                                      values, restarts only with FP32 and FP64) */
 } vj_params;
 /* Fill defaults for a mode (reference defaults for VJ_MODE_REFERENCE). */
 void vj_params_default(vj_params* p, int mode);

@@ -5,7 +5,8 @@ cumulative factors bounded to [1/max_scale, max_scale], bound/objective rescalin
 norms leave out rows and columns at that bound, 64-step power iteration), the correction
 coordinates around an FP64 centre, the reflected Halpern step, and the restart rule and
 primal-weight controller of the control kernel, checked every K iterations on the original
-LP's rel-KKT. It reproduces the device's FP64 iteration counts on the LPs it was compared on;
+LP's rel-KKT, and (refine_every_check) the centre moved at every check with an explicit
+Halpern anchor. It reproduces the device's FP64 iteration counts on the LPs it was compared on;
 it is not the product and nothing in the product calls it."""
 import numpy as np
 import scipy.sparse as sp
@@ -58,9 +59,11 @@ def setup(lp, max_scale=1e6, exclude_bounded=True, ruiz_passes=10, pock_chamboll
 
 
 def solve(lp, S=None, K=64, tol=1e-6, max_iters=200000, rho=1.0, artificial=0.5, smoothing=0.5, integral=0.0,
-          sufficient=0.2, necessary=0.8, step=0.95, **setup_kw):
+          sufficient=0.2, necessary=0.8, step=0.95, refine_every_check=False, **setup_kw):
     """Returns status (1 optimal, 6 iteration limit), iterations, restarts and the original-space
-    point (x, canonical y) of the last check."""
+    point (x, canonical y) of the last check. refine_every_check: a check that does not restart
+    moves the centre to the check point and shifts the iterate and the Halpern anchor (ax, ay,
+    zero at a restart) with it; every update adds (1 - lam) times the anchor."""
     S = S or setup(lp, **setup_kw)
     A, AT, c, lc, uc, lr, ur = S["A"], S["AT"], S["c"], S["lc"], S["uc"], S["lr"], S["ur"]
     m, n = A.shape
@@ -73,6 +76,7 @@ def solve(lp, S=None, K=64, tol=1e-6, max_iters=200000, rho=1.0, artificial=0.5,
 
     dh, lcs, ucs, lrs, urs = recentre(xh, yh)
     x, y = np.zeros(n), np.zeros(m)
+    ax, ay = np.zeros(n), np.zeros(m)
     k = iters = restarts = 0
     pw_sum = 0.0
     r0 = r_prev = np.inf
@@ -80,14 +84,14 @@ def solve(lp, S=None, K=64, tol=1e-6, max_iters=200000, rho=1.0, artificial=0.5,
     while True:
         tau, sig = eta / omega, eta * omega
         for it in range(K):
-            lam = (k + it + 1) / (k + it + 2)
+            lam, pull = (k + it + 1) / (k + it + 2), 1.0 / (k + it + 2)
             xt = np.clip(x - tau * (dh - AT @ y), lcs, ucs)
             if it == 0:
                 xq0 = np.dot(xt - x, xt - x)
             if it == K - 1:
-                xq1, xq2, xT = np.dot(xt - x, xt - x), np.dot(xt, xt), xt
+                xq1, xq2, xT = np.dot(xt - x, xt - x), np.dot(xt - ax, xt - ax), xt
             xbar = 2 * xt - x
-            x = lam * ((1 + rho) * xt - rho * x)
+            x = lam * ((1 + rho) * xt - rho * x) + pull * ax
             w = A @ xbar
             full = yh + y
             yt = np.where(sig * (w - lrs) <= full, y + sig * (lrs - w),
@@ -95,8 +99,8 @@ def solve(lp, S=None, K=64, tol=1e-6, max_iters=200000, rho=1.0, artificial=0.5,
             if it == 0:
                 yq0 = np.dot(yt - y, yt - y)
             if it == K - 1:
-                yq1, yq2, yT = np.dot(yt - y, yt - y), np.dot(yt, yt), yt
-            y = lam * ((1 + rho) * yt - rho * y)
+                yq1, yq2, yT = np.dot(yt - y, yt - y), np.dot(yt - ay, yt - ay), yt
+            y = lam * ((1 + rho) * yt - rho * y) + pull * ay
         iters += K
         k += K
         xf, yf = xh + xT, yh + yT
@@ -123,3 +127,8 @@ def solve(lp, S=None, K=64, tol=1e-6, max_iters=200000, rho=1.0, artificial=0.5,
             xh, yh = xf, yf
             dh, lcs, ucs, lrs, urs = recentre(xh, yh)
             x, y = np.zeros(n), np.zeros(m)
+            ax, ay = np.zeros(n), np.zeros(m)
+        elif refine_every_check:
+            xh, yh = xf, yf
+            dh, lcs, ucs, lrs, urs = recentre(xh, yh)
+            x, y, ax, ay = x - xT, y - yT, ax - xT, ay - yT

@@ -41,6 +41,7 @@ def test_params_defaults_follow_reference():
     # restart rule and primal-weight smoothing of the restarted engine
     assert (q.restart_sufficient, q.restart_necessary, q.restart_artificial) == (0.2, 0.8, 0.5)
     assert q.primal_weight_smoothing == 0.5 and q.primal_weight_integral == 0.0
+    assert q.refine_every_check == 0   # per precision: every check with 16-bit values, restarts otherwise
     assert q.power_iterations == 64 and q.step_scale == 0.95
     assert q.restart_drift == 0.0   # 0: the engine's default (2 with FP32 iterates, off otherwise)
 

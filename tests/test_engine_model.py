@@ -45,3 +45,28 @@ def test_structured_and_rescaled_lps_converge(dec):
         assert r["status"] == 1, (s, dec, r["kkt"][:4])
         got = float(np.dot(lp.c, r["x"])) + lp.offset
         assert abs(got - opt) <= 1e-5 * (1 + abs(opt)), (s, dec, got, opt)
+
+
+@pytest.mark.parametrize("rho", [1.0, 0.5])
+def test_refine_every_check_changes_nothing_in_exact_arithmetic(rho):
+    """Moving the centre at a check and shifting the iterate and the Halpern anchor with it is a
+    change of coordinates: in exact arithmetic the iteration is the same. In FP64 the runs with
+    and without it must then agree to rounding over blocks that cross restarts and re-centrings
+    (a wrong sign or a missing anchor term in the shift gives O(1) differences), and the restart
+    decisions must be the same."""
+    for lp in (lp_cases.cfg0_lp(), lp_cases.structured_feasible_lp(5, 60, 150)):
+        a = M.solve(lp, K=64, tol=0.0, max_iters=12 * 64, rho=rho)
+        b = M.solve(lp, K=64, tol=0.0, max_iters=12 * 64, rho=rho, refine_every_check=True)
+        assert a["restarts"] == b["restarts"] and 1 <= a["restarts"] < 12, (a["restarts"], b["restarts"])
+        for v in ("x", "y"):
+            scale = max(1.0, float(np.max(np.abs(a[v]))))
+            assert float(np.max(np.abs(a[v] - b[v]))) <= 1e-9 * scale, v
+
+
+def test_refine_every_check_reaches_the_optimum():
+    lp = lp_cases.structured_feasible_lp(6, 150, 60)
+    opt = lp_cases.highs_optimum(lp)
+    r = M.solve(lp, max_iters=60000, refine_every_check=True)
+    assert r["status"] == 1, r["kkt"][:4]
+    got = float(np.dot(lp.c, r["x"])) + lp.offset
+    assert abs(got - opt) <= 1e-5 * (1 + abs(opt)), (got, opt)

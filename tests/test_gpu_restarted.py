@@ -83,6 +83,11 @@ def test_restart_drift_parameter_range(engine):
             engine.set_params(vp.default_params(vp.MODE_RESTARTED, restart_drift=bad))
     for ok in (0.0, -1.0, 1.5, 4.0):
         engine.set_params(vp.default_params(vp.MODE_RESTARTED, restart_drift=ok))
+    for bad in (-2, 2):
+        with pytest.raises(vp.VjError):
+            engine.set_params(vp.default_params(vp.MODE_RESTARTED, refine_every_check=bad))
+    for ok in (-1, 0, 1):
+        engine.set_params(vp.default_params(vp.MODE_RESTARTED, refine_every_check=ok))
 
 
 def test_cfg0_objective_against_highs(engine):

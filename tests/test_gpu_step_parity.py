@@ -119,3 +119,51 @@ def test_reflection_half_equals_model(engine, variant, blocks):
         assert r.restart_count == restarts
         close(x, mx, (name, "x"))
         close(y, my, (name, "y"))
+
+
+@pytest.mark.parametrize("variant", sorted(VARIANTS))
+def test_refine_every_check_equals_model(engine, variant):
+    """refine_every_check = 1: the checks that do not restart move the FP64 centre to the check
+    point and keep the Halpern epoch, whose anchor becomes a vector. 8 blocks cross restarts and
+    such re-centrings; the device equals the model run with the same option."""
+    for name, lp in lps():
+        print(f"[case] step parity refine every check {variant} {name}", flush=True)
+        engine.upload(lp)
+        r = engine.solve(vp.default_params(vp.MODE_RESTARTED, precision="fp64", tolerance=0.0, max_iterations=8 * K,
+                                           check_frequency=K, refine_every_check=1, **VARIANTS[variant]))
+        x, _, y = engine.download()
+        mx, my, restarts = model_point(lp, 8, refine_every_check=True)
+        assert r.restart_count == restarts and 1 <= restarts < 8, (r.restart_count, restarts)
+        close(x, mx, (name, "x"))
+        close(y, my, (name, "y"))
+
+
+@pytest.mark.parametrize("variant", ["tile_streamed", "split_A4_At2_streamed"])
+def test_refine_every_check_reflection_half_equals_model(engine, variant):
+    for name, lp in lps()[:2]:
+        print(f"[case] step parity refine every check, reflection 0.5, {variant} {name}", flush=True)
+        engine.upload(lp)
+        r = engine.solve(vp.default_params(vp.MODE_RESTARTED, precision="fp64", tolerance=0.0, reflection=0.5,
+                                           max_iterations=8 * K, check_frequency=K, refine_every_check=1,
+                                           **VARIANTS[variant]))
+        x, _, y = engine.download()
+        mx, my, restarts = model_point(lp, 8, rho=0.5, refine_every_check=True)
+        assert r.restart_count == restarts
+        close(x, mx, (name, "x"))
+        close(y, my, (name, "y"))
+
+
+@pytest.mark.parametrize("lanes", [-1, 8, -5, -7])
+def test_refine_every_check_batch_equals_model(engine, lanes):
+    cases = lps()
+    engine.upload_batch([lp for _, lp in cases])
+    print(f"[case] batch step parity refine every check lanes={lanes}", flush=True)
+    res = engine.solve_batch(vp.default_params(vp.MODE_RESTARTED, precision="fp64", tolerance=0.0,
+                                               max_iterations=8 * K, check_frequency=K, lanes_A=lanes,
+                                               lanes_At=lanes, refine_every_check=1))
+    xs, ys = engine.download_batch()
+    for (name, lp), r, x, y in zip(cases, res, xs, ys):
+        mx, my, restarts = model_point(lp, 8, refine_every_check=True)
+        assert r.restart_count == restarts
+        close(x, mx, (name, "x"))
+        close(y, my, (name, "y"))

@@ -211,6 +211,7 @@ struct FastEngine {
     DevBuf ir, ic, pxf, pyf;
     DevBuf c_s, lc_s, uc_s, lr_s, ur_s;
     DevBuf x, xbar, y, dhat, lcs, ucs, lrs, urs, yhat_t;
+    DevBuf ax, ay;   // Halpern anchors of anchored instances (the centre moved inside the epoch)
     DevBuf xhat, yhat, xfull, yfull, ax_full, aty_full;
     DevBuf part_x, part_y, part_kr, part_kc, part_s;
     DevBuf kkt8, st, offset_d, norm_b_d, norm_c_d, objfac_d, inst_tmp_a, inst_tmp_b, hist;
@@ -310,6 +311,7 @@ int with_entries(bool rec, const DevBuf& recs, const int* idx, const V* val, con
 
 template <class V, class T>
 int launch_primal(vj_ctx* ctx, FastEngine& E, const V* tv, T* x, T* xbar, const T* y, int it, int flags) {
+    const T* ax = E.ax.as<T>();
     Problem& P = ctx->prob;
     const int tiles = tiles_for(E.n, E.GAt);
     VJ_TRY(require({P.tp.p, P.ti.p, tv, y, E.dhat.p, E.lcs.p, E.ucs.p, x, xbar, E.xhat.p, E.xfull.p, E.st.p,
@@ -330,7 +332,7 @@ int launch_primal(vj_ctx* ctx, FastEngine& E, const V* tv, T* x, T* xbar, const 
                 E.psumAt.as<T>(), E.st.as<InstState>()));
         }
         hipLaunchKernelGGL((primal_finish<T>), dim3(grid_for(E.n, kBlock * finish_rows<T>())), dim3(kBlock), 0, ctx->stream, E.n, E.SAt,
-                           E.psumAt.as<T>(), E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), x, xbar, E.xhat.as<double>(),
+                           E.psumAt.as<T>(), E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), x, xbar, ax, E.xhat.as<double>(),
                            E.xfull.as<double>(), E.st.as<InstState>(), E.part_x.as<double>(), it, flags, E.rho);
         VJ_LAUNCHED();
         return VJ_OK;
@@ -341,7 +343,7 @@ int launch_primal(vj_ctx* ctx, FastEngine& E, const V* tv, T* x, T* xbar, const 
                                [&](auto ent) {
             hipLaunchKernelGGL((ell_primal_step<decltype(ent), T>), dim3(grid_for(E.n, kBlock)), dim3(kBlock), 0,
                                ctx->stream, E.n, E.ellOffAt.as<int>(), E.ellWAt.as<int>(), ent, y, E.dhat.as<T>(),
-                               E.lcs.as<T>(), E.ucs.as<T>(), x, xbar, E.xhat.as<double>(), E.xfull.as<double>(),
+                               E.lcs.as<T>(), E.ucs.as<T>(), x, xbar, ax, E.xhat.as<double>(), E.xfull.as<double>(),
                                E.st.as<InstState>(), E.part_x.as<double>(), it, flags, E.rho);
         }));
         VJ_LAUNCHED();
@@ -356,7 +358,7 @@ int launch_primal(vj_ctx* ctx, FastEngine& E, const V* tv, T* x, T* xbar, const 
                 hipLaunchKernelGGL((staged_primal_step<Ent, T, decltype(multi)::value, decltype(rpl)::value>),
                                    dim3(E.nlistAt), dim3(kStageThreads), 0, ctx->stream, E.listGAt.as<int>(),
                                    E.descGAt.as<int4>(), P.tp.as<int>(), ent, y,
-                                   E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), x, xbar, E.xhat.as<double>(),
+                                   E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), x, xbar, ax, E.xhat.as<double>(),
                                    E.xfull.as<double>(), E.st.as<InstState>(), E.part_x.as<double>(), it, flags, E.rho);
             };
             using One = std::integral_constant<int, 1>;
@@ -389,12 +391,12 @@ int launch_primal(vj_ctx* ctx, FastEngine& E, const V* tv, T* x, T* xbar, const 
                 if (E.nlistTAt > 0)
                 hipLaunchKernelGGL((stream_primal_step<Ent, T, true>), dim3(E.nlistTAt), dim3(kBlock), 0, ctx->stream, E.n,
                                    E.listTAt.as<int>(), E.tileAt.as<int>(), P.tp.as<int>(), ent, y, E.dhat.as<T>(), E.lcs.as<T>(),
-                                   E.ucs.as<T>(), x, xbar, E.xhat.as<double>(), E.xfull.as<double>(),
+                                   E.ucs.as<T>(), x, xbar, ax, E.xhat.as<double>(), E.xfull.as<double>(),
                                    E.st.as<InstState>(), P.col_inst.as<int>(), E.part_x.as<double>(), it, flags, E.rho);
             } else
                 hipLaunchKernelGGL((stream_primal_step<Ent, T, false>), dim3(E.ntAt), dim3(kBlock), 0, ctx->stream, E.n,
                                    nullptr, E.tileAt.as<int>(), P.tp.as<int>(), ent, y, E.dhat.as<T>(), E.lcs.as<T>(),
-                                   E.ucs.as<T>(), x, xbar, E.xhat.as<double>(), E.xfull.as<double>(),
+                                   E.ucs.as<T>(), x, xbar, ax, E.xhat.as<double>(), E.xfull.as<double>(),
                                    E.st.as<InstState>(), P.col_inst.as<int>(), E.part_x.as<double>(), it, flags, E.rho);
         }));
         VJ_LAUNCHED();
@@ -402,12 +404,12 @@ int launch_primal(vj_ctx* ctx, FastEngine& E, const V* tv, T* x, T* xbar, const 
     }
     if (E.multi) {
         VJ_DISPATCH_G(E.GAt, hipLaunchKernelGGL((primal_step<V, T, GG, true>), dim3(tiles), dim3(kBlock), 0, ctx->stream,
-            E.n, P.tp.as<int>(), P.ti.as<int>(), tv, y, E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), x, xbar,
+            E.n, P.tp.as<int>(), P.ti.as<int>(), tv, y, E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), x, xbar, ax,
             E.xhat.as<double>(), E.xfull.as<double>(), E.st.as<InstState>(), P.col_inst.as<int>(),
             E.part_x.as<double>(), it, flags, E.rho));
     } else {
         VJ_DISPATCH_G(E.GAt, hipLaunchKernelGGL((primal_step<V, T, GG, false>), dim3(tiles), dim3(kBlock), 0, ctx->stream,
-            E.n, P.tp.as<int>(), P.ti.as<int>(), tv, y, E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), x, xbar,
+            E.n, P.tp.as<int>(), P.ti.as<int>(), tv, y, E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), x, xbar, ax,
             E.xhat.as<double>(), E.xfull.as<double>(), E.st.as<InstState>(), P.col_inst.as<int>(),
             E.part_x.as<double>(), it, flags, E.rho));
     }
@@ -433,6 +435,7 @@ int require(std::initializer_list<const void*> ptrs, const char* where) {
 
 template <class V, class T>
 int launch_dual(vj_ctx* ctx, FastEngine& E, const V* av, const T* xbar, T* y, int it, int flags) {
+    const T* ay = E.ay.as<T>();
     Problem& P = ctx->prob;
     const int tiles = tiles_for(E.m, E.GA);
     const T* yht = yhat_working<T>(E);
@@ -458,7 +461,7 @@ int launch_dual(vj_ctx* ctx, FastEngine& E, const V* av, const T* xbar, T* y, in
                 E.psumA.as<T>(), E.st.as<InstState>()));
         }
         hipLaunchKernelGGL((dual_finish<T>), dim3(grid_for(E.m, kBlock * finish_rows<T>())), dim3(kBlock), 0, ctx->stream, E.m, E.SA,
-                           E.psumA.as<T>(), yht, lrs, urs, y, E.yhat.as<double>(),
+                           E.psumA.as<T>(), yht, lrs, urs, y, ay, E.yhat.as<double>(),
                            E.yfull.as<double>(), E.st.as<InstState>(), E.part_y.as<double>(), it, flags, E.rho);
         VJ_LAUNCHED();
         return VJ_OK;
@@ -469,7 +472,7 @@ int launch_dual(vj_ctx* ctx, FastEngine& E, const V* av, const T* xbar, T* y, in
                                [&](auto ent) {
             hipLaunchKernelGGL((ell_dual_step<decltype(ent), T>), dim3(grid_for(E.m, kBlock)), dim3(kBlock), 0,
                                ctx->stream, E.m, E.ellOffA.as<int>(), E.ellWA.as<int>(), ent, xbar, yht,
-                               lrs, urs, y, E.yhat.as<double>(), E.yfull.as<double>(),
+                               lrs, urs, y, ay, E.yhat.as<double>(), E.yfull.as<double>(),
                                E.st.as<InstState>(), E.part_y.as<double>(), it, flags, E.rho);
         }));
         VJ_LAUNCHED();
@@ -484,7 +487,7 @@ int launch_dual(vj_ctx* ctx, FastEngine& E, const V* av, const T* xbar, T* y, in
                 hipLaunchKernelGGL((staged_dual_step<Ent, T, decltype(multi)::value, decltype(rpl)::value>),
                                    dim3(E.nlistA), dim3(kStageThreads), 0, ctx->stream, E.listGA.as<int>(),
                                    E.descGA.as<int4>(), P.rp.as<int>(), ent, xbar,
-                                   yht, lrs, urs, y, E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(),
+                                   yht, lrs, urs, y, ay, E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(),
                                    E.part_y.as<double>(), it, flags, E.rho);
             };
             using One = std::integral_constant<int, 1>;
@@ -516,12 +519,12 @@ int launch_dual(vj_ctx* ctx, FastEngine& E, const V* av, const T* xbar, T* y, in
             if (E.multi) {
                 if (E.nlistTA > 0)
                 hipLaunchKernelGGL((stream_dual_step<Ent, T, true>), dim3(E.nlistTA), dim3(kBlock), 0, ctx->stream, E.m,
-                                   E.listTA.as<int>(), E.tileA.as<int>(), P.rp.as<int>(), ent, xbar, yht, lrs, urs, y,
+                                   E.listTA.as<int>(), E.tileA.as<int>(), P.rp.as<int>(), ent, xbar, yht, lrs, urs, y, ay,
                                    E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(),
                                    P.row_inst.as<int>(), E.part_y.as<double>(), it, flags, E.rho);
             } else
                 hipLaunchKernelGGL((stream_dual_step<Ent, T, false>), dim3(E.ntA), dim3(kBlock), 0, ctx->stream, E.m,
-                                   nullptr, E.tileA.as<int>(), P.rp.as<int>(), ent, xbar, yht, lrs, urs, y,
+                                   nullptr, E.tileA.as<int>(), P.rp.as<int>(), ent, xbar, yht, lrs, urs, y, ay,
                                    E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(),
                                    P.row_inst.as<int>(), E.part_y.as<double>(), it, flags, E.rho);
         }));
@@ -530,12 +533,12 @@ int launch_dual(vj_ctx* ctx, FastEngine& E, const V* av, const T* xbar, T* y, in
     }
     if (E.multi) {
         VJ_DISPATCH_G(E.GA, hipLaunchKernelGGL((dual_step<V, T, GG, true>), dim3(tiles), dim3(kBlock), 0, ctx->stream,
-            E.m, P.rp.as<int>(), P.ci.as<int>(), av, xbar, yht, lrs, urs, y,
+            E.m, P.rp.as<int>(), P.ci.as<int>(), av, xbar, yht, lrs, urs, y, ay,
             E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(), P.row_inst.as<int>(),
             E.part_y.as<double>(), it, flags, E.rho));
     } else {
         VJ_DISPATCH_G(E.GA, hipLaunchKernelGGL((dual_step<V, T, GG, false>), dim3(tiles), dim3(kBlock), 0, ctx->stream,
-            E.m, P.rp.as<int>(), P.ci.as<int>(), av, xbar, yht, lrs, urs, y,
+            E.m, P.rp.as<int>(), P.ci.as<int>(), av, xbar, yht, lrs, urs, y, ay,
             E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(), P.row_inst.as<int>(),
             E.part_y.as<double>(), it, flags, E.rho));
     }
@@ -777,8 +780,8 @@ int setup(vj_ctx* ctx, FastEngine& E) {
     for (DevBuf* b : {&E.Csc, &E.cstep, &E.cnorm, &E.ic, &E.pxf, &E.c_s, &E.lc_s, &E.uc_s, &E.xhat, &E.xfull,
                       &E.aty_full, &E.vtmp_n, &E.utmp_n})
         VJ_TRY(b->alloc(static_cast<size_t>(n) * 8));
-    for (DevBuf* b : {&E.x, &E.xbar, &E.dhat, &E.lcs, &E.ucs}) VJ_TRY(b->alloc(static_cast<size_t>(n) * T));
-    for (DevBuf* b : {&E.y, &E.lrs, &E.urs}) VJ_TRY(b->alloc(static_cast<size_t>(m) * T));
+    for (DevBuf* b : {&E.x, &E.xbar, &E.ax, &E.dhat, &E.lcs, &E.ucs}) VJ_TRY(b->alloc(static_cast<size_t>(n) * T));
+    for (DevBuf* b : {&E.y, &E.ay, &E.lrs, &E.urs}) VJ_TRY(b->alloc(static_cast<size_t>(m) * T));
     if (E.prec != VJ_PREC_FP64) VJ_TRY(E.yhat_t.alloc(static_cast<size_t>(m) * T));
     if (!E.tiles_built) {   // row tiles of A and A^T for the tile-streamed kernels, built on the device
         VJ_TRY(device_row_tiles(s, m, 1, P.rp.as<int>(), 0, kStreamCap, kBlock, E.tileA, &E.ntA));
@@ -1069,6 +1072,13 @@ constexpr double kEllMaxPadding = 1.25;
 // their rounding noise moves the rel-KKT by more than this factor from check to check.)
 constexpr double kRefineDrift = 2.0;
 
+// Where the FP64 centre moves: 1 = at every check (16-bit matrix values by default, whose
+// rounding error otherwise grows with the distance the epoch has moved), 0 = at restarts.
+int refine_every_check(const vj_params& p, int prec) {
+    if (p.refine_every_check != 0) return p.refine_every_check > 0 ? 1 : 0;
+    return (prec == VJ_PREC_BF16 || prec == VJ_PREC_FP16) ? 1 : 0;
+}
+
 int setup_one_ell(vj_ctx* ctx, FastEngine& E, int rows, const int* rp, const int* ci, const double* vals_dev,
                   bool forced, bool& built, double& pad, int64_t& len, DevBuf& off, DevBuf& width, DevBuf& idx,
                   DevBuf& perm, DevBuf& val, DevBuf& rec) {
@@ -1117,6 +1127,7 @@ int launch_restart(vj_ctx* ctx, FastEngine& E, int force) {
     hipStream_t s = ctx->stream;
     T* yhat_t = yhat_working<T>(E);
     VJ_TRY(require({E.st.p, E.xfull.p, E.aty_full.p, E.c_s.p, E.lc_s.p, E.uc_s.p, E.xhat.p, E.x.p, E.xbar.p, E.dhat.p,
+                    E.ax.p, E.ay.p,
                     E.lcs.p, E.ucs.p, E.yfull.p, E.ax_full.p, E.lr_s.p, E.ur_s.p, E.yhat.p, yhat_t, E.y.p,
                     E.lrs.p, E.urs.p, E.multi ? P.row_inst.p : E.st.p, E.multi ? P.col_inst.p : E.st.p},
                    "restart kernels"));
@@ -1124,20 +1135,20 @@ int launch_restart(vj_ctx* ctx, FastEngine& E, int force) {
         hipLaunchKernelGGL((restart_cols<T, true>), dim3(grid_for(E.n, kBlock)), dim3(kBlock), 0, s, E.n,
                            E.st.as<InstState>(), P.col_inst.as<int>(), force, E.xfull.as<double>(), E.aty_full.as<double>(),
                            E.c_s.as<double>(), E.lc_s.as<double>(), E.uc_s.as<double>(), E.xhat.as<double>(), E.x.as<T>(),
-                           E.xbar.as<T>(), E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>());
+                           E.xbar.as<T>(), E.ax.as<T>(), E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), E.rho);
         hipLaunchKernelGGL((restart_rows<T, true>), dim3(grid_for(E.m, kBlock)), dim3(kBlock), 0, s, E.m,
                            E.st.as<InstState>(), P.row_inst.as<int>(), force, E.yfull.as<double>(), E.ax_full.as<double>(),
                            E.lr_s.as<double>(), E.ur_s.as<double>(), E.yhat.as<double>(), yhat_t, E.y.as<T>(),
-                           E.lrs.as<T>(), E.urs.as<T>());
+                           E.ay.as<T>(), E.lrs.as<T>(), E.urs.as<T>());
     } else {
         hipLaunchKernelGGL((restart_cols<T, false>), dim3(grid_for(E.n, kBlock)), dim3(kBlock), 0, s, E.n,
                            E.st.as<InstState>(), P.col_inst.as<int>(), force, E.xfull.as<double>(), E.aty_full.as<double>(),
                            E.c_s.as<double>(), E.lc_s.as<double>(), E.uc_s.as<double>(), E.xhat.as<double>(), E.x.as<T>(),
-                           E.xbar.as<T>(), E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>());
+                           E.xbar.as<T>(), E.ax.as<T>(), E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), E.rho);
         hipLaunchKernelGGL((restart_rows<T, false>), dim3(grid_for(E.m, kBlock)), dim3(kBlock), 0, s, E.m,
                            E.st.as<InstState>(), P.row_inst.as<int>(), force, E.yfull.as<double>(), E.ax_full.as<double>(),
                            E.lr_s.as<double>(), E.ur_s.as<double>(), E.yhat.as<double>(), yhat_t, E.y.as<T>(),
-                           E.lrs.as<T>(), E.urs.as<T>());
+                           E.ay.as<T>(), E.lrs.as<T>(), E.urs.as<T>());
     }
     VJ_LAUNCHED();
     return VJ_OK;
@@ -1177,6 +1188,7 @@ int enqueue_block(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     a.kkt_drift = drift > 1.0 ? drift : (drift == 0.0 && E.prec == VJ_PREC_FP32 ? kRefineDrift : 0.0);
     a.pw_theta = ctx->params.primal_weight_smoothing;
     a.pw_ki = ctx->params.primal_weight_integral;
+    a.refine_every_check = refine_every_check(ctx->params, E.prec);
     a.hist = E.hist_cap > 0 ? E.hist.as<double>() : nullptr;
     a.hist_cap = E.hist_cap;
     hipLaunchKernelGGL(control_kernel, dim3(E.count), dim3(kBlock), 0, ctx->stream, E.st.as<InstState>(), a);

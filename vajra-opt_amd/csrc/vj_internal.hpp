@@ -96,7 +96,7 @@ struct alignas(16) InstState {
     int32_t converged;
     int32_t nan_flag;
     int32_t checks;
-    int32_t pad;
+    int32_t anchored;                  // the centre moved at a check of this epoch: the steps add the anchor term
 };
 
 }  // namespace vj

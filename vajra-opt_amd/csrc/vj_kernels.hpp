@@ -166,21 +166,31 @@ struct PrimalOps {
 // bit; lam_prev is the previous iteration's lam (the same expression of s.k + it - 1), 0 at
 // the start of an epoch, where the restart kernels have zeroed x and xbar. This saves one
 // vector write per primal step. Other rho keep x in its own vector.
+// Anchored instances (the FP64 centre moved at a check inside the epoch, s.anchored = 1): the
+// Halpern anchor is no longer the origin of the correction coordinates but the vector ax (ay
+// for rows), and every update adds (1 - lam) times the anchor, with 1 - lam = 1 / (k + 2)
+// formed in FP64 (anchor_pull). The anchor is loaded where it is used (not carried from the
+// operand loads through the SpMV), so that it holds no register across the products.
 template <class T>
-__device__ __forceinline__ T x_start(int j, const T* __restrict__ x, const T* __restrict__ xbar, const InstState& s,
-                                     int it, double rho) {
+__device__ __forceinline__ T anchor_pull(const InstState& s, int it) {
+    return static_cast<T>(1.0 / (static_cast<double>(s.k + it) + 2.0));
+}
+template <class T>
+__device__ __forceinline__ T x_start(int j, const T* __restrict__ x, const T* __restrict__ xbar,
+                                     const T* __restrict__ ax, const InstState& s, int it, double rho) {
     if (rho != 1.0) return x[j];
     const double kp = static_cast<double>(s.k + it) - 1.0;
     const T lam_prev = static_cast<T>((kp + 1.0) / (kp + 2.0));
+    if (s.anchored) return lam_prev * xbar[j] + anchor_pull<T>(s, it - 1) * ax[j];
     return lam_prev * xbar[j];
 }
 
 template <class T>
 __device__ __forceinline__ PrimalOps<T> primal_ops(int j, const T* __restrict__ x, const T* __restrict__ xbar,
-                                                   const T* __restrict__ dhat, const T* __restrict__ lcs,
-                                                   const T* __restrict__ ucs, const InstState& s, int it,
-                                                   double rho) {
-    return PrimalOps<T>{x_start<T>(j, x, xbar, s, it, rho), dhat[j], lcs[j], ucs[j]};
+                                                   const T* __restrict__ ax, const T* __restrict__ dhat,
+                                                   const T* __restrict__ lcs, const T* __restrict__ ucs,
+                                                   const InstState& s, int it, double rho) {
+    return PrimalOps<T>{x_start<T>(j, x, xbar, ax, s, it, rho), dhat[j], lcs[j], ucs[j]};
 }
 
 // The projected point xt of column j (the values every primal kernel writes derive from it).
@@ -193,15 +203,17 @@ __device__ __forceinline__ T primal_point(T acc, const PrimalOps<T>& o, T tau) {
 }
 // Residual contributions of one column at the measured iterations, accumulated into c (one
 // column per thread starts from zero, so the sums equal the single terms), and the full FP64
-// point at the block's last iteration.
+// point at the block's last iteration. c[2] is the squared distance from the anchor (the
+// distance the epoch moved, for the primal weight).
 template <class T>
-__device__ __forceinline__ void primal_residuals(T xt, T xk, int j, int flags, const double* __restrict__ xhat,
+__device__ __forceinline__ void primal_residuals(T xt, T xk, T an, int j, int flags, const double* __restrict__ xhat,
                                                  double* __restrict__ xfull, double (&c)[3]) {
     const double d = static_cast<double>(xt) - static_cast<double>(xk);
     if (flags & 1) c[0] += d * d;
     if (flags & 2) {
+        const double e = static_cast<double>(xt) - static_cast<double>(an);
         c[1] += d * d;
-        c[2] += static_cast<double>(xt) * static_cast<double>(xt);
+        c[2] += e * e;
         xfull[j] = xhat[j] + static_cast<double>(xt);
     }
 }
@@ -214,24 +226,29 @@ __device__ __forceinline__ T anchor_weight(const InstState& s, int it) {
 template <class T>
 __device__ __forceinline__ void primal_update(T acc, const PrimalOps<T>& o, int j, const InstState& s, int it,
                                               int flags, double rho, T* __restrict__ x, T* __restrict__ xbar,
-                                              const double* __restrict__ xhat, double* __restrict__ xfull,
-                                              double (&c)[3]) {
+                                              const T* __restrict__ ax, const double* __restrict__ xhat,
+                                              double* __restrict__ xfull, double (&c)[3]) {
     const T xk = o.xk;
     const T xt = primal_point<T>(acc, o, static_cast<T>(s.tau));
     xbar[j] = static_cast<T>(2) * xt - xk;
-    if (rho != 1.0)
-        x[j] = anchor_weight<T>(s, it) * ((static_cast<T>(1) + static_cast<T>(rho)) * xt - static_cast<T>(rho) * xk);
-    primal_residuals<T>(xt, xk, j, flags, xhat, xfull, c);
+    T an = static_cast<T>(0);
+    if (s.anchored && (rho != 1.0 || (flags & 2))) an = ax[j];
+    if (rho != 1.0) {
+        T xn = anchor_weight<T>(s, it) * ((static_cast<T>(1) + static_cast<T>(rho)) * xt - static_cast<T>(rho) * xk);
+        if (s.anchored) xn += anchor_pull<T>(s, it) * an;
+        x[j] = xn;
+    }
+    primal_residuals<T>(xt, xk, an, j, flags, xhat, xfull, c);
 }
 
 template <class T>
 __device__ __forceinline__ void primal_epilogue(T acc, int j, const InstState& s, int it, int flags, double rho,
                                                 const T* __restrict__ dhat, const T* __restrict__ lcs,
                                                 const T* __restrict__ ucs, T* __restrict__ x, T* __restrict__ xbar,
-                                                const double* __restrict__ xhat, double* __restrict__ xfull,
-                                                double (&c)[3]) {
-    primal_update<T>(acc, primal_ops<T>(j, x, xbar, dhat, lcs, ucs, s, it, rho), j, s, it, flags, rho, x, xbar, xhat,
-                     xfull, c);
+                                                const T* __restrict__ ax, const double* __restrict__ xhat,
+                                                double* __restrict__ xfull, double (&c)[3]) {
+    primal_update<T>(acc, primal_ops<T>(j, x, xbar, ax, dhat, lcs, ucs, s, it, rho), j, s, it, flags, rho, x, xbar,
+                     ax, xhat, xfull, c);
 }
 
 // Dual epilogue for row i given w = (A x_bar)_i: prox of the row-bound support function
@@ -267,45 +284,50 @@ __device__ __forceinline__ T dual_point(T w, const DualOps<T>& o, T sig) {
     if (sig * (w - o.hi) > sfull) return yk + sig * (o.hi - w);
     return -o.yh;
 }
+// lam = anchor_weight, pull = anchor_pull of the iteration (pull is used by anchored instances only)
 template <class T>
-__device__ __forceinline__ T dual_next(T yt, T yk, T lam, double rho) {
-    return lam * ((static_cast<T>(1) + static_cast<T>(rho)) * yt - static_cast<T>(rho) * yk);
+__device__ __forceinline__ T dual_next(T yt, T yk, T an, T lam, T pull, bool anchored, double rho) {
+    const T yn = lam * ((static_cast<T>(1) + static_cast<T>(rho)) * yt - static_cast<T>(rho) * yk);
+    return anchored ? yn + pull * an : yn;
 }
 template <class T>
-__device__ __forceinline__ void dual_residuals(T yt, T yk, int i, int flags, const double* __restrict__ yhat,
+__device__ __forceinline__ void dual_residuals(T yt, T yk, T an, int i, int flags, const double* __restrict__ yhat,
                                                double* __restrict__ yfull, double (&c)[3]) {
     const double d = static_cast<double>(yt) - static_cast<double>(yk);
     if (flags & 1) c[0] += d * d;
     if (flags & 2) {
+        const double e = static_cast<double>(yt) - static_cast<double>(an);
         c[1] += d * d;
-        c[2] += static_cast<double>(yt) * static_cast<double>(yt);
+        c[2] += e * e;
         yfull[i] = yhat[i] + static_cast<double>(yt);
     }
 }
 
 template <class T>
 __device__ __forceinline__ void dual_update(T w, const DualOps<T>& o, int i, const InstState& s, int it, int flags,
-                                            double rho, T* __restrict__ y, const double* __restrict__ yhat,
-                                            double* __restrict__ yfull, double (&c)[3]) {
+                                            double rho, T* __restrict__ y, const T* __restrict__ ay,
+                                            const double* __restrict__ yhat, double* __restrict__ yfull,
+                                            double (&c)[3]) {
     const T yt = dual_point<T>(w, o, static_cast<T>(s.sigma));
-    y[i] = dual_next<T>(yt, o.yk, anchor_weight<T>(s, it), rho);
-    dual_residuals<T>(yt, o.yk, i, flags, yhat, yfull, c);
+    const T an = s.anchored ? ay[i] : static_cast<T>(0);
+    y[i] = dual_next<T>(yt, o.yk, an, anchor_weight<T>(s, it), anchor_pull<T>(s, it), s.anchored != 0, rho);
+    dual_residuals<T>(yt, o.yk, an, i, flags, yhat, yfull, c);
 }
 
 template <class T>
 __device__ __forceinline__ void dual_epilogue(T w, int i, const InstState& s, int it, int flags, double rho,
                                               const T* __restrict__ yhat_t, const T* __restrict__ lrs,
                                               const T* __restrict__ urs, T* __restrict__ y,
-                                              const double* __restrict__ yhat, double* __restrict__ yfull,
-                                              double (&c)[3]) {
-    dual_update<T>(w, dual_ops<T>(i, y, yhat_t, lrs, urs), i, s, it, flags, rho, y, yhat, yfull, c);
+                                              const T* __restrict__ ay, const double* __restrict__ yhat,
+                                              double* __restrict__ yfull, double (&c)[3]) {
+    dual_update<T>(w, dual_ops<T>(i, y, yhat_t, lrs, urs), i, s, it, flags, rho, y, ay, yhat, yfull, c);
 }
 
 template <class V, class T, int G, bool MULTI>
 __global__ __launch_bounds__(kBlock) void primal_step(
     int n, const int* __restrict__ tp, const int* __restrict__ ti, const V* __restrict__ tv,
     const T* __restrict__ y, const T* __restrict__ dhat, const T* __restrict__ lcs,
-    const T* __restrict__ ucs, T* __restrict__ x, T* __restrict__ xbar,
+    const T* __restrict__ ucs, T* __restrict__ x, T* __restrict__ xbar, const T* __restrict__ ax,
     const double* __restrict__ xhat, double* __restrict__ xfull,
     const InstState* __restrict__ st, const int* __restrict__ col_inst, double* __restrict__ part,
     int it, int flags, double rho) {
@@ -331,7 +353,7 @@ __global__ __launch_bounds__(kBlock) void primal_step(
         for (int p = tp[j] + lane; p < p1; p += G) acc += static_cast<T>(load_val(tv, p)) * y[load_idx(ti, p)];
     }
     acc = group_sum<G>(acc);
-    if (act && lane == 0) primal_epilogue<T>(acc, j, st[my], it, flags, rho, dhat, lcs, ucs, x, xbar, xhat, xfull, c);
+    if (act && lane == 0) primal_epilogue<T>(acc, j, st[my], it, flags, rho, dhat, lcs, ucs, x, xbar, ax, xhat, xfull, c);
     if (flags) {
         const int mask = ((flags & 1) ? 1 : 0) | ((flags & 2) ? 6 : 0);
         write_tile_partials<3, MULTI>(c, my, f, l, tile, part, mask);
@@ -342,7 +364,7 @@ template <class V, class T, int G, bool MULTI>
 __global__ __launch_bounds__(kBlock) void dual_step(
     int m, const int* __restrict__ rp, const int* __restrict__ ci, const V* __restrict__ av,
     const T* __restrict__ xbar, const T* __restrict__ yhat_t, const T* __restrict__ lrs,
-    const T* __restrict__ urs, T* __restrict__ y, const double* __restrict__ yhat,
+    const T* __restrict__ urs, T* __restrict__ y, const T* __restrict__ ay, const double* __restrict__ yhat,
     double* __restrict__ yfull, const InstState* __restrict__ st, const int* __restrict__ row_inst,
     double* __restrict__ part, int it, int flags, double rho) {
     constexpr int R = kBlock / G;
@@ -367,7 +389,7 @@ __global__ __launch_bounds__(kBlock) void dual_step(
         for (int p = rp[i] + lane; p < p1; p += G) w += static_cast<T>(load_val(av, p)) * xbar[load_idx(ci, p)];
     }
     w = group_sum<G>(w);
-    if (act && lane == 0) dual_epilogue<T>(w, i, st[my], it, flags, rho, yhat_t, lrs, urs, y, yhat, yfull, c);
+    if (act && lane == 0) dual_epilogue<T>(w, i, st[my], it, flags, rho, yhat_t, lrs, urs, y, ay, yhat, yfull, c);
     if (flags) {
         const int mask = ((flags & 1) ? 1 : 0) | ((flags & 2) ? 6 : 0);
         write_tile_partials<3, MULTI>(c, my, f, l, tile, part, mask);
@@ -532,7 +554,7 @@ template <class Ent, class T, bool MULTI>
 __global__ __launch_bounds__(kBlock) void stream_primal_step(
     int n, const int* __restrict__ list, const int* __restrict__ tile_row, const int* __restrict__ tp, Ent ent,
     const T* __restrict__ y, const T* __restrict__ dhat, const T* __restrict__ lcs,
-    const T* __restrict__ ucs, T* __restrict__ x, T* __restrict__ xbar, const double* __restrict__ xhat,
+    const T* __restrict__ ucs, T* __restrict__ x, T* __restrict__ xbar, const T* __restrict__ ax, const double* __restrict__ xhat,
     double* __restrict__ xfull, const InstState* __restrict__ st, const int* __restrict__ col_inst,
     double* __restrict__ part, int it, int flags, double rho) {
     __shared__ T lds[kStreamCap];
@@ -553,14 +575,14 @@ __global__ __launch_bounds__(kBlock) void stream_primal_step(
     if (s1 - s0 > kStreamCap) {  // one long row
         const T acc = long_row_sum<Ent, T>(s0, s1, ent, y, lds);
         if (threadIdx.x == 0 && (!MULTI || st[my].active != 0))
-            primal_epilogue<T>(acc, r0, st[my], it, flags, rho, dhat, lcs, ucs, x, xbar, xhat, xfull, c);
+            primal_epilogue<T>(acc, r0, st[my], it, flags, rho, dhat, lcs, ucs, x, xbar, ax, xhat, xfull, c);
     } else {
         // the row's operands and segment bounds are loaded before the gathers
         const bool mine = j < r1 && (!MULTI || st[my].active != 0);
         PrimalOps<T> o{};
         int q0 = 0, q1 = 0;
         if (mine) {
-            o = primal_ops<T>(j, x, xbar, dhat, lcs, ucs, st[my], it, rho);
+            o = primal_ops<T>(j, x, xbar, ax, dhat, lcs, ucs, st[my], it, rho);
             q0 = tp[j] - s0;
             q1 = tp[j + 1] - s0;
         }
@@ -569,7 +591,7 @@ __global__ __launch_bounds__(kBlock) void stream_primal_step(
         if (mine) {
             T acc = 0;
             for (int q = q0; q < q1; ++q) acc += lds[q];
-            primal_update<T>(acc, o, j, st[my], it, flags, rho, x, xbar, xhat, xfull, c);
+            primal_update<T>(acc, o, j, st[my], it, flags, rho, x, xbar, ax, xhat, xfull, c);
         }
     }
     if (flags) {
@@ -582,7 +604,7 @@ template <class Ent, class T, bool MULTI>
 __global__ __launch_bounds__(kBlock) void stream_dual_step(
     int m, const int* __restrict__ list, const int* __restrict__ tile_row, const int* __restrict__ rp, Ent ent,
     const T* __restrict__ xbar, const T* __restrict__ yhat_t, const T* __restrict__ lrs,
-    const T* __restrict__ urs, T* __restrict__ y, const double* __restrict__ yhat, double* __restrict__ yfull,
+    const T* __restrict__ urs, T* __restrict__ y, const T* __restrict__ ay, const double* __restrict__ yhat, double* __restrict__ yfull,
     const InstState* __restrict__ st, const int* __restrict__ row_inst, double* __restrict__ part, int it, int flags,
     double rho) {
     __shared__ T lds[kStreamCap];
@@ -603,7 +625,7 @@ __global__ __launch_bounds__(kBlock) void stream_dual_step(
     if (s1 - s0 > kStreamCap) {
         const T w = long_row_sum<Ent, T>(s0, s1, ent, xbar, lds);
         if (threadIdx.x == 0 && (!MULTI || st[my].active != 0))
-            dual_epilogue<T>(w, r0, st[my], it, flags, rho, yhat_t, lrs, urs, y, yhat, yfull, c);
+            dual_epilogue<T>(w, r0, st[my], it, flags, rho, yhat_t, lrs, urs, y, ay, yhat, yfull, c);
     } else {
         // the row's operands and segment bounds are loaded before the gathers
         const bool mine = i < r1 && (!MULTI || st[my].active != 0);
@@ -619,7 +641,7 @@ __global__ __launch_bounds__(kBlock) void stream_dual_step(
         if (mine) {
             T w = 0;
             for (int q = q0; q < q1; ++q) w += lds[q];
-            dual_update<T>(w, o, i, st[my], it, flags, rho, y, yhat, yfull, c);
+            dual_update<T>(w, o, i, st[my], it, flags, rho, y, ay, yhat, yfull, c);
         }
     }
     if (flags) {
@@ -766,7 +788,7 @@ __device__ __forceinline__ void stage_partials(const double (&v)[3], int slot, d
 template <class Ent, class T, bool MULTI, int RPL>
 __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8))) void staged_primal_step(
     const int* __restrict__ list, const int4* __restrict__ desc, const int* __restrict__ tp, Ent ent, const T* __restrict__ y, const T* __restrict__ dhat,
-    const T* __restrict__ lcs, const T* __restrict__ ucs, T* __restrict__ x, T* __restrict__ xbar,
+    const T* __restrict__ lcs, const T* __restrict__ ucs, T* __restrict__ x, T* __restrict__ xbar, const T* __restrict__ ax,
     const double* __restrict__ xhat, double* __restrict__ xfull, const InstState* __restrict__ st,
     double* __restrict__ part, int it, int flags, double rho) {
     __shared__ T lds[stage_cap<T>()];
@@ -779,7 +801,7 @@ __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8
     double c[3] = {0.0, 0.0, 0.0};
     if (s1 - s0 > stage_cap<T>()) {   // one long row
         const T acc = staged_long_row<Ent, T>(s0, s1, ent, y, lds);
-        if (threadIdx.x == 0) primal_epilogue<T>(acc, r0, st[my], it, flags, rho, dhat, lcs, ucs, x, xbar, xhat, xfull, c);
+        if (threadIdx.x == 0) primal_epilogue<T>(acc, r0, st[my], it, flags, rho, dhat, lcs, ucs, x, xbar, ax, xhat, xfull, c);
     } else {
         // the rows' operands and bounds are loaded before the staging (a lane past the tile's end
         // loads its last row's and skips the update)
@@ -789,7 +811,7 @@ __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8
         for (int u = 0; u < RPL; ++u) {
             const int j = r0 + static_cast<int>(threadIdx.x) + u * kStageThreads;
             const int jc = j < r1 ? j : r1 - 1;
-            o[u] = primal_ops<T>(jc, x, xbar, dhat, lcs, ucs, st[my], it, rho);
+            o[u] = primal_ops<T>(jc, x, xbar, ax, dhat, lcs, ucs, st[my], it, rho);
             q0[u] = tp[jc] - s0;
             q1[u] = tp[jc + 1] - s0;
         }
@@ -800,7 +822,7 @@ __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8
             if (j < r1) {
                 T acc = 0;
                 for (int q = q0[u]; q < q1[u]; ++q) acc += lds[q];
-                primal_update<T>(acc, o[u], j, st[my], it, flags, rho, x, xbar, xhat, xfull, c);
+                primal_update<T>(acc, o[u], j, st[my], it, flags, rho, x, xbar, ax, xhat, xfull, c);
             }
         }
     }
@@ -813,7 +835,7 @@ __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8
 template <class Ent, class T, bool MULTI, int RPL>
 __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8))) void staged_dual_step(
     const int* __restrict__ list, const int4* __restrict__ desc, const int* __restrict__ rp, Ent ent, const T* __restrict__ xbar, const T* __restrict__ yhat_t,
-    const T* __restrict__ lrs, const T* __restrict__ urs, T* __restrict__ y, const double* __restrict__ yhat,
+    const T* __restrict__ lrs, const T* __restrict__ urs, T* __restrict__ y, const T* __restrict__ ay, const double* __restrict__ yhat,
     double* __restrict__ yfull, const InstState* __restrict__ st, double* __restrict__ part, int it, int flags,
     double rho) {
     __shared__ T lds[stage_cap<T>()];
@@ -826,7 +848,7 @@ __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8
     double c[3] = {0.0, 0.0, 0.0};
     if (s1 - s0 > stage_cap<T>()) {
         const T w = staged_long_row<Ent, T>(s0, s1, ent, xbar, lds);
-        if (threadIdx.x == 0) dual_epilogue<T>(w, r0, st[my], it, flags, rho, yhat_t, lrs, urs, y, yhat, yfull, c);
+        if (threadIdx.x == 0) dual_epilogue<T>(w, r0, st[my], it, flags, rho, yhat_t, lrs, urs, y, ay, yhat, yfull, c);
     } else {
         DualOps<T> o[RPL];
         int q0[RPL], q1[RPL];
@@ -845,7 +867,7 @@ __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8
             if (i < r1) {
                 T w = 0;
                 for (int q = q0[u]; q < q1[u]; ++q) w += lds[q];
-                dual_update<T>(w, o[u], i, st[my], it, flags, rho, y, yhat, yfull, c);
+                dual_update<T>(w, o[u], i, st[my], it, flags, rho, y, ay, yhat, yfull, c);
             }
         }
     }
@@ -983,7 +1005,7 @@ __device__ __forceinline__ void store16(T* p, const typename Vec16<T>::type& v) 
 template <class T>
 __global__ __launch_bounds__(kBlock) void primal_finish(
     int n, int S, const T* __restrict__ psum, const T* __restrict__ dhat, const T* __restrict__ lcs,
-    const T* __restrict__ ucs, T* __restrict__ x, T* __restrict__ xbar, const double* __restrict__ xhat,
+    const T* __restrict__ ucs, T* __restrict__ x, T* __restrict__ xbar, const T* __restrict__ ax, const double* __restrict__ xhat,
     double* __restrict__ xfull, const InstState* __restrict__ st, double* __restrict__ part, int it, int flags,
     double rho) {
     if (st[0].active == 0) return;
@@ -998,20 +1020,28 @@ __global__ __launch_bounds__(kBlock) void primal_finish(
         for (int k = 1; k < S; ++k) acc += load16<T>(psum + k * ps + j0);
         const VT xb = load16<T>(xbar + j0), dh = load16<T>(dhat + j0), lo = load16<T>(lcs + j0),
                  hi = load16<T>(ucs + j0);
+        VT an = {};
+        if (s.anchored) an = load16<T>(ax + j0);
         VT xk;
         if (rho == 1.0) {
             const T lam_prev = anchor_weight<T>(s, it - 1);   // x recovered from x_bar, as x_start
-            for (int v = 0; v < V; ++v) xk[v] = lam_prev * xb[v];
+            if (s.anchored) {
+                const T pull_prev = anchor_pull<T>(s, it - 1);
+                for (int v = 0; v < V; ++v) xk[v] = lam_prev * xb[v] + pull_prev * an[v];
+            } else {
+                for (int v = 0; v < V; ++v) xk[v] = lam_prev * xb[v];
+            }
         } else {
             xk = load16<T>(x + j0);
         }
-        const T tau = static_cast<T>(s.tau), lam = anchor_weight<T>(s, it);
+        const T tau = static_cast<T>(s.tau), lam = anchor_weight<T>(s, it), pull = anchor_pull<T>(s, it);
         VT xb_new, x_new;
         for (int v = 0; v < V; ++v) {
             const T xt = primal_point<T>(acc[v], PrimalOps<T>{xk[v], dh[v], lo[v], hi[v]}, tau);
             xb_new[v] = static_cast<T>(2) * xt - xk[v];
             x_new[v] = lam * ((static_cast<T>(1) + static_cast<T>(rho)) * xt - static_cast<T>(rho) * xk[v]);
-            primal_residuals<T>(xt, xk[v], j0 + v, flags, xhat, xfull, c);
+            if (s.anchored) x_new[v] += pull * an[v];
+            primal_residuals<T>(xt, xk[v], an[v], j0 + v, flags, xhat, xfull, c);
         }
         store16<T>(xbar + j0, xb_new);
         if (rho != 1.0) store16<T>(x + j0, x_new);
@@ -1019,7 +1049,7 @@ __global__ __launch_bounds__(kBlock) void primal_finish(
         for (int j = j0; j < n; ++j) {
             T acc = psum[j];
             for (int k = 1; k < S; ++k) acc += psum[k * ps + j];
-            primal_epilogue<T>(acc, j, s, it, flags, rho, dhat, lcs, ucs, x, xbar, xhat, xfull, c);
+            primal_epilogue<T>(acc, j, s, it, flags, rho, dhat, lcs, ucs, x, xbar, ax, xhat, xfull, c);
         }
     }
     if (flags) {
@@ -1031,7 +1061,7 @@ __global__ __launch_bounds__(kBlock) void primal_finish(
 template <class T>
 __global__ __launch_bounds__(kBlock) void dual_finish(
     int m, int S, const T* __restrict__ psum, const T* __restrict__ yhat_t, const T* __restrict__ lrs,
-    const T* __restrict__ urs, T* __restrict__ y, const double* __restrict__ yhat, double* __restrict__ yfull,
+    const T* __restrict__ urs, T* __restrict__ y, const T* __restrict__ ay, const double* __restrict__ yhat, double* __restrict__ yfull,
     const InstState* __restrict__ st, double* __restrict__ part, int it, int flags, double rho) {
     if (st[0].active == 0) return;
     constexpr int V = finish_rows<T>();
@@ -1050,19 +1080,23 @@ __global__ __launch_bounds__(kBlock) void dual_finish(
             hi = load16<T>(urs + i0);
             yh = load16<T>(yhat_t + i0);
         }
-        const T sig = static_cast<T>(s.sigma), lam = anchor_weight<T>(s, it);
+        VT an = {};
+        if (s.anchored) an = load16<T>(ay + i0);
+        const T sig = static_cast<T>(s.sigma), lam = anchor_weight<T>(s, it), pull = anchor_pull<T>(s, it);
+        const bool anchored = s.anchored != 0;
         VT y_new;
         for (int v = 0; v < V; ++v) {
-            const T yt = dual_point<T>(w[v], DualOps<T>{yk[v], eq ? static_cast<T>(0) : yh[v], lo[v], hi[v], eq}, sig);
-            y_new[v] = dual_next<T>(yt, yk[v], lam, rho);
-            dual_residuals<T>(yt, yk[v], i0 + v, flags, yhat, yfull, c);
+            const T yt = dual_point<T>(w[v], DualOps<T>{yk[v], eq ? static_cast<T>(0) : yh[v], lo[v], hi[v], eq},
+                                       sig);
+            y_new[v] = dual_next<T>(yt, yk[v], an[v], lam, pull, anchored, rho);
+            dual_residuals<T>(yt, yk[v], an[v], i0 + v, flags, yhat, yfull, c);
         }
         store16<T>(y + i0, y_new);
     } else {
         for (int i = i0; i < m; ++i) {
             T w = psum[i];
             for (int k = 1; k < S; ++k) w += psum[k * ps + i];
-            dual_epilogue<T>(w, i, s, it, flags, rho, yhat_t, lrs, urs, y, yhat, yfull, c);
+            dual_epilogue<T>(w, i, s, it, flags, rho, yhat_t, lrs, urs, y, ay, yhat, yfull, c);
         }
     }
     if (flags) {
@@ -1138,15 +1172,15 @@ template <class Ent, class T>
 __global__ __launch_bounds__(kBlock) void ell_primal_step(
     int n, const int* __restrict__ off, const int* __restrict__ width, Ent ent, const T* __restrict__ y,
     const T* __restrict__ dhat, const T* __restrict__ lcs, const T* __restrict__ ucs, T* __restrict__ x,
-    T* __restrict__ xbar, const double* __restrict__ xhat, double* __restrict__ xfull,
+    T* __restrict__ xbar, const T* __restrict__ ax, const double* __restrict__ xhat, double* __restrict__ xfull,
     const InstState* __restrict__ st, double* __restrict__ part, int it, int flags, double rho) {
     if (st[0].active == 0) return;
     const int j = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
     double c[3] = {0.0, 0.0, 0.0};
     if (j < n) {
-        const PrimalOps<T> o = primal_ops<T>(j, x, xbar, dhat, lcs, ucs, st[0], it, rho);   // before the gathers
+        const PrimalOps<T> o = primal_ops<T>(j, x, xbar, ax, dhat, lcs, ucs, st[0], it, rho);   // before the gathers
         const T acc = ell_row_sum<Ent, T>(j, off, width, ent, y);
-        primal_update<T>(acc, o, j, st[0], it, flags, rho, x, xbar, xhat, xfull, c);
+        primal_update<T>(acc, o, j, st[0], it, flags, rho, x, xbar, ax, xhat, xfull, c);
     }
     if (flags) {
         const int mask = ((flags & 1) ? 1 : 0) | ((flags & 2) ? 6 : 0);
@@ -1157,7 +1191,7 @@ __global__ __launch_bounds__(kBlock) void ell_primal_step(
 template <class Ent, class T>
 __global__ __launch_bounds__(kBlock) void ell_dual_step(
     int m, const int* __restrict__ off, const int* __restrict__ width, Ent ent, const T* __restrict__ xbar,
-    const T* __restrict__ yhat_t, const T* __restrict__ lrs, const T* __restrict__ urs, T* __restrict__ y,
+    const T* __restrict__ yhat_t, const T* __restrict__ lrs, const T* __restrict__ urs, T* __restrict__ y, const T* __restrict__ ay,
     const double* __restrict__ yhat, double* __restrict__ yfull, const InstState* __restrict__ st,
     double* __restrict__ part, int it, int flags, double rho) {
     if (st[0].active == 0) return;
@@ -1166,7 +1200,7 @@ __global__ __launch_bounds__(kBlock) void ell_dual_step(
     if (i < m) {
         const DualOps<T> o = dual_ops<T>(i, y, yhat_t, lrs, urs);
         const T w = ell_row_sum<Ent, T>(i, off, width, ent, xbar);
-        dual_update<T>(w, o, i, st[0], it, flags, rho, y, yhat, yfull, c);
+        dual_update<T>(w, o, i, st[0], it, flags, rho, y, ay, yhat, yfull, c);
     }
     if (flags) {
         const int mask = ((flags & 1) ? 1 : 0) | ((flags & 2) ? 6 : 0);
@@ -1332,6 +1366,9 @@ __global__ __launch_bounds__(kBlock) void kkt_finish(
 // controller on the error e = log(omega |Dx| / |Dy|) (|Dx|, |Dy|: the distances the epoch moved):
 // log omega <- log omega - (theta e + ki sum(e)); with ki = 0 (the default, theta = 0.5) it is
 // PDLP's smoothing log omega <- theta log(|Dy|/|Dx|) + (1 - theta) log omega.
+// With refine_every_check the checks that do not restart also move the FP64 centre to the
+// check point (iterative refinement every K iterations) and keep the Halpern epoch: the
+// instance is then anchored, and its anchor is an explicit vector in the new coordinates.
 struct ControlArgs {
     const double* px; int Rx;      // primal-step partials (cols), fixed tile height Rx
     const double* py; int Ry;      // dual-step partials (rows), fixed tile height Ry
@@ -1346,6 +1383,7 @@ struct ControlArgs {
     double restart_sufficient, restart_necessary, restart_artificial, pw_theta;
     double kkt_drift;              // 0: off (FP64 and BF16 iterates)
     double pw_ki;                  // integral gain of the primal-weight controller
+    int refine_every_check;        // 1: re-centre at every check, 0: at restarts only
     double* hist;                  // per instance: [checks] rel_kkt history (may be null)
     int hist_cap;
 };
@@ -1397,6 +1435,7 @@ __global__ __launch_bounds__(kBlock) void control_kernel(InstState* __restrict__
                              (a.kkt_drift > 0.0 && k8[0] > a.kkt_drift * s.kkt_min);
         s.r_prev = r;
         s.restarted_prev = 0;
+        s.anchored = (!restart && a.refine_every_check) ? 1 : 0;
         if (restart) {
             s.restart_now = 1;
             s.restarted_prev = 1;
@@ -1420,22 +1459,34 @@ __global__ __launch_bounds__(kBlock) void control_kernel(InstState* __restrict__
 }
 
 // Restart (or initial centring, force = 1): move the centre to the last T(z) and
-// rebuild the shifted data in the working precision.
+// rebuild the shifted data in the working precision. An anchored instance that did not
+// restart moves its centre to the same point and keeps its epoch: the iterate and the anchor
+// shift by d = T(z) - old centre (with rho = 1 the iterate is carried by xbar, x = lam xbar +
+// pull an, so xbar shifts by d as well).
 template <class T, bool MULTI>
 __global__ void restart_cols(int n, const InstState* __restrict__ st, const int* __restrict__ col_inst,
                              int force, const double* __restrict__ xfull,
                              const double* __restrict__ aty_full, const double* __restrict__ c_s,
                              const double* __restrict__ lc_s, const double* __restrict__ uc_s,
-                             double* __restrict__ xhat, T* __restrict__ x, T* __restrict__ xbar,
-                             T* __restrict__ dhat, T* __restrict__ lcs, T* __restrict__ ucs) {
+                             double* __restrict__ xhat, T* __restrict__ x, T* __restrict__ xbar, T* __restrict__ ax,
+                             T* __restrict__ dhat, T* __restrict__ lcs, T* __restrict__ ucs, double rho) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     const int q = MULTI ? col_inst[j] : 0;
-    if (!force && st[q].restart_now == 0) return;
+    const bool restart = force || st[q].restart_now != 0;
+    if (!restart && !(st[q].anchored && st[q].active)) return;
     const double xf = xfull[j];
+    if (restart) {
+        x[j] = static_cast<T>(0);
+        xbar[j] = static_cast<T>(0);
+        if (ax) ax[j] = static_cast<T>(0);
+    } else {
+        const double d = xf - xhat[j];
+        if (rho == 1.0) xbar[j] = static_cast<T>(static_cast<double>(xbar[j]) - d);
+        else x[j] = static_cast<T>(static_cast<double>(x[j]) - d);
+        ax[j] = static_cast<T>(static_cast<double>(ax[j]) - d);
+    }
     xhat[j] = xf;
-    x[j] = static_cast<T>(0);
-    xbar[j] = static_cast<T>(0);
     dhat[j] = static_cast<T>(c_s[j] - aty_full[j]);
     lcs[j] = static_cast<T>(lc_s[j] - xf);
     ucs[j] = static_cast<T>(uc_s[j] - xf);
@@ -1446,16 +1497,24 @@ __global__ void restart_rows(int m, const InstState* __restrict__ st, const int*
                              int force, const double* __restrict__ yfull,
                              const double* __restrict__ ax_full, const double* __restrict__ lr_s,
                              const double* __restrict__ ur_s, double* __restrict__ yhat,
-                             T* __restrict__ yhat_t, T* __restrict__ y, T* __restrict__ lrs,
+                             T* __restrict__ yhat_t, T* __restrict__ y, T* __restrict__ ay, T* __restrict__ lrs,
                              T* __restrict__ urs) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     const int q = MULTI ? row_inst[i] : 0;
-    if (!force && st[q].restart_now == 0) return;
+    const bool restart = force || st[q].restart_now != 0;
+    if (!restart && !(st[q].anchored && st[q].active)) return;
     const double yf = yfull[i];
+    if (restart) {
+        y[i] = static_cast<T>(0);
+        if (ay) ay[i] = static_cast<T>(0);
+    } else {
+        const double d = yf - yhat[i];
+        y[i] = static_cast<T>(static_cast<double>(y[i]) - d);
+        ay[i] = static_cast<T>(static_cast<double>(ay[i]) - d);
+    }
     yhat[i] = yf;
     if (static_cast<void*>(yhat_t) != static_cast<void*>(yhat)) yhat_t[i] = static_cast<T>(yf);
-    y[i] = static_cast<T>(0);
     lrs[i] = static_cast<T>(lr_s[i] - ax_full[i]);
     urs[i] = static_cast<T>(ur_s[i] - ax_full[i]);
 }
