@@ -25,8 +25,8 @@ LARGE = dict(m=1_000_000, n=2_000_000)
 
 
 def params(prec, **kw):
-    return vp.default_params(vp.MODE_RESTARTED, precision=prec, tolerance=1e-6, max_iterations=400000,
-                             time_limit=100.0, **kw)
+    kw = {**dict(tolerance=1e-6, max_iterations=400000, time_limit=100.0), **kw}
+    return vp.default_params(vp.MODE_RESTARTED, precision=prec, **kw)
 
 
 @pytest.fixture(scope="module")
