@@ -540,13 +540,18 @@ void test_mixed_precision_option() {
     const indus::Model M = indus::io::read_mps(g_dir + "/crude_blend.mps");
     indus::Options o = pdhg_options(1e-8, 200000);
     for (const char* prec : {"fp32", "bf16", "fp16"}) {
-        o.set("pdhg_precision", prec);
-        const indus::Solution s = indus::solve(M, o);
-        CHECK(s.status == indus::SolveStatus::kOptimal);
-        CHECK_NEAR(s.objective_value, 214.14594594594598, 1e-6 * 215.0);
-        const auto t = indus::gpu::get_last_gpu_timing();
-        CHECK(t.kernel_mode.find(std::string(prec) + "_refined") != std::string::npos);
-        std::cout << "  " << prec << ": iterations " << s.iterations << " mode " << t.kernel_mode << "\n";
+        // 0: the per-precision default; -1 / 1: the FP64 centre moves at restarts only / at every check
+        for (int refine : {0, -1, 1}) {
+            o.set("pdhg_precision", prec);
+            o.set("pdhg_refine_every_check", refine);
+            const indus::Solution s = indus::solve(M, o);
+            CHECK(s.status == indus::SolveStatus::kOptimal);
+            CHECK_NEAR(s.objective_value, 214.14594594594598, 1e-6 * 215.0);
+            const auto t = indus::gpu::get_last_gpu_timing();
+            CHECK(t.kernel_mode.find(std::string(prec) + "_refined") != std::string::npos);
+            std::cout << "  " << prec << " refine_every_check " << refine << ": iterations " << s.iterations
+                      << " mode " << t.kernel_mode << "\n";
+        }
     }
 }
 

@@ -48,6 +48,8 @@ vj_params params_from(const Options& o) {
     if (o.has("check_frequency")) p.check_frequency = static_cast<int>(o.get_int("check_frequency", p.check_frequency));
     p.enable_scaling = o.enable_scaling ? 1 : 0;
     p.use_graph = o.get_bool("pdhg_graph", true) ? 1 : 0;
+    if (o.has("pdhg_refine_every_check"))
+        p.refine_every_check = static_cast<int>(o.get_int("pdhg_refine_every_check", 0));
     p.verbose = o.get_bool("verbose", false) ? 1 : 0;
     return p;
 }
