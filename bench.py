@@ -85,6 +85,9 @@ def main():
                     help="further precisions of the batch legs, comma-separated (fp16: FP16 matrix values with FP32 "
                          "iterates and FP64 refinement; fp64: the plain FP64 engine; 'none' for neither), each "
                          "reported beside --precision")
+    ap.add_argument("--single-extra", default="bf16",
+                    help="further precisions of the 1M x 2M solve, comma-separated (bf16: BF16 matrix values with FP32 "
+                         "iterates and FP64 refinement at every check; 'none' to skip), reported beside the headline")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch each check block's kernels directly instead of replaying a hipGraph (profiled "
                          "runs: graph replays under rocprofv3 kernel tracing have ended in a host fault)")
@@ -128,6 +131,15 @@ def main():
 
     extra = {}
     flags = []
+
+    def budget_left(leg):
+        used = time.perf_counter() - T0
+        if used > args.budget:
+            flags.append(f"{leg} skipped: {used:.0f}s used of the {args.budget:.0f}s budget")
+            log(flags[-1])
+            return False
+        return True
+
     # ---- plain FP64 engine (the baseline the mixed path is measured against) -----------
     r64 = None
     if not args.skip_fp64:
@@ -190,15 +202,35 @@ def main():
     spmv_gbs = {f"{k['name']}_{tag}": round(k["bytes_per_launch"] / (k["avg_us"] * 1e-6) / 1e9, 1)
                 for tag, kk in (("fp64", extra.get("fp64", {}).get("kernels", [])), (args.precision, kmix))
                 for k in kk}
-    eng.close()
 
-    def budget_left(leg):
-        used = time.perf_counter() - T0
-        if used > args.budget:
-            flags.append(f"{leg} skipped: {used:.0f}s used of the {args.budget:.0f}s budget")
-            log(flags[-1])
-            return False
-        return True
+    # ---- further value precisions on the same LP (BF16 values + FP32 iterates by default) --
+    for prec in [p for p in args.single_extra.split(",") if p and p not in ("none", args.precision)]:
+        if prec not in ("fp16", "bf16", "fp32", "fp64"):
+            raise SystemExit(f"--single-extra: unknown precision {prec}")
+        if not budget_left(f"single {prec}"):
+            continue
+        rx, wx = solve_timed(vp, eng, prec, args.tol, args.max_iters, args.time_limit)
+        xx, _, yx = eng.download()
+        kx = None
+        try:
+            import oracle as O
+            kx = float(O.rel_kkt(lp, xx, yx)[0])
+        except Exception as e:
+            log(f"oracle unavailable: {e}")
+        log(f"{prec}: {vp.STATUS[rx.status]} iters={rx.iterations} restarts={rx.restart_count} "
+            f"rel_kkt={rx.rel_kkt:.3e} (oracle {kx}) iter={rx.iteration_time:.3f}s wall={wx:.3f}s")
+        rec = dict(precision=prec, status=vp.STATUS[rx.status], iterations=rx.iterations, restarts=rx.restart_count,
+                   rel_kkt_device=rx.rel_kkt, rel_kkt_oracle=kx, objective=rx.primal_objective,
+                   time_to_target_s=rx.time_to_target, iteration_s=rx.iteration_time, solve_wall_s=wx,
+                   lanes=[rx.lanes_A, rx.lanes_At], splits=[rx.col_splits_A, rx.col_splits_At],
+                   check_interval=rx.check_interval)
+        if r64 is not None and r64.status == 1 and rx.status == 1:
+            rec["speedup_vs_fp64"] = extra["fp64"]["solve_wall_s"] / wx
+            rec["objective_rel_diff_vs_fp64"] = abs(rx.primal_objective - r64.primal_objective) / (
+                1 + abs(r64.primal_objective))
+        extra[f"single_{prec}"] = rec
+        del xx, yx
+    eng.close()
 
     # ---- CPU baseline: the reference CPU PDHG on this host, bounded sample ----------------
     cpu = None
