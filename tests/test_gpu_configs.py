@@ -70,11 +70,13 @@ def test_large_random_lp_16bit_values(large, prec):
     r = eng.solve(params(prec, max_iterations=1_000_000, time_limit=200.0))
     x, _, y = eng.download()
     print(f"[case] {prec}: {vp.STATUS[r.status]} iters={r.iterations} restarts={r.restart_count} "
-          f"rel_kkt={r.rel_kkt:.3e} iter_s={r.iteration_time:.2f}", flush=True)
+          f"rel_kkt={r.rel_kkt:.3e} K={r.check_interval} iter_s={r.iteration_time:.2f}", flush=True)
     assert r.status == 1, r.as_dict()
     k = O.rel_kkt(lp, x, y)
     assert k[0] <= 1e-6, k
     assert abs(k[0] - r.rel_kkt) <= 1e-9 + 1e-6 * k[0], (k, r.rel_kkt)
+    # BF16 values refined at every check: the autotuned check interval stays in [64, 256]
+    assert r.check_interval in ((64, 128, 256) if prec == "bf16" else (32, 64, 128, 256)), r.check_interval
     if "fp64" in seen:
         o, o64 = float(np.dot(lp.c, x)) + lp.offset, seen["fp64"]
         assert abs(o - o64) <= 1e-6 * (1 + abs(o64)), (o, o64)

@@ -1360,7 +1360,10 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     // Iterations per check: the smallest power of two in [32, 256] at which the check (two
     // FP64 KKT passes, their reduction, control and restart kernels) costs at most 2% of the
     // iterations between checks. The KKT passes are timed on the current iterates; their
-    // outputs are recomputed by the first real check.
+    // outputs are recomputed by the first real check. BF16 values refined at every check: the
+    // check is also the refinement step, whose frequency sets how far the 8-bit values'
+    // error grows, so it may cost up to 8%, in [64, 256] (on the 1M x 2M LP K = 64 took
+    // 131,136 iterations against 297,472 at K = 256; on the batches 2-7% less time).
     if (prm.check_frequency == 0) {
         ScopedEvent se0, se1;
         VJ_HIP(hipEventCreate(&se0.e));
@@ -1376,8 +1379,10 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
         VJ_HIP(hipEventElapsedTime(&ms, e0, e1));
         const double t_check = 1000.0 * ms / 3 + 20.0;   // + control and restart kernels
         const double t_iter = bestA + bestAt;
-        int K = 32;
-        while (K < 256 && t_check / K > 0.02 * t_iter) K *= 2;
+        const bool bf16_refined = E.prec == VJ_PREC_BF16 && refine_every_check(prm, E.prec) != 0;
+        const double allowance = bf16_refined ? 0.08 : 0.02;
+        int K = bf16_refined ? 64 : 32;
+        while (K < 256 && t_check / K > allowance * t_iter) K *= 2;
         E.K_tuned = K;
     }
     E.tuned.push_back(TuneDecision{E.prec, prm.lanes_A, prm.lanes_At, prm.col_splits_A, prm.col_splits_At,
