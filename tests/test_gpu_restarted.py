@@ -505,3 +505,49 @@ def test_default_solves_reproducible(engine, prec):
         assert other[:6] == runs[0][:6], (other[:6], runs[0][:6])
         assert np.array_equal(other[6], runs[0][6]) and np.array_equal(other[7], runs[0][7])
     assert O.rel_kkt(lp, runs[0][6], runs[0][7])[0] <= 1e-6
+
+
+def _hard_small_lps():
+    lps = [lp_cases.structured_feasible_lp(*s) for s in STRUCTURED]
+    for s, m, n in [(0, 12, 30), (5, 60, 150), (6, 150, 60)]:
+        lps.append(lp_cases.rescaled_lp(lp_cases.structured_feasible_lp(s, m, n), s, 3.0))
+    lps.append(lp_cases.rescaled_lp(lp_cases.structured_feasible_lp(11, 2, 64), 11, 2.0))
+    return lps
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp16", "bf16"])
+def test_refine_every_check_hard_batch(engine, prec):
+    """Refinement at every check (refine_every_check = 1) on the structured LPs (empty, fixed,
+    free and one-sided rows and columns, offsets, both senses), rows and columns rescaled by
+    10^+-3 and near-empty columns with costs, as one batch: every instance reaches rel-KKT 1e-8
+    (oracle-checked on the downloaded point) and its HiGHS objective to 1e-6, in every value
+    precision."""
+    lps = _hard_small_lps()
+    opts = [lp_cases.highs_optimum(lp) for lp in lps]
+    print(f"[case] refine every check, hard batch {prec} ({len(lps)} instances)", flush=True)
+    engine.upload_batch(lps)
+    res = engine.solve_batch(vp.default_params(vp.MODE_RESTARTED, precision=prec, tolerance=1e-8,
+                                               max_iterations=400000, refine_every_check=1))
+    xs, ys = engine.download_batch()
+    print(f"[case] max iterations {max(r.iterations for r in res)}", flush=True)
+    for lp, opt, r, x, y in zip(lps, opts, res, xs, ys):
+        assert r.status == 1, r.as_dict()
+        k = O.rel_kkt(lp, x, y)
+        assert k[0] <= 1e-8 and abs(k[0] - r.rel_kkt) <= 1e-11 + 1e-6 * k[0], (k, r.rel_kkt)
+        got = float(np.dot(lp.c, x)) + lp.offset
+        assert abs(got - opt) <= 1e-6 * (1 + abs(opt)), (got, opt)
+
+
+def test_refine_every_check_failure_statuses(engine):
+    """With refinement at every check an infeasible LP still ends at its iteration limit, never
+    OPTIMAL, and a NaN in the costs still ends the solve with NUMERICAL_ERROR."""
+    bad = lp_cases.structured_lp(1, 30, 12)
+    assert lp_cases.highs_optimum(bad) is None
+    print("[case] refine every check: infeasible fp32", flush=True)
+    r, *_ = solve(engine, bad, max_iterations=20000, precision="fp32", refine_every_check=1)
+    assert r.status != 1, r.as_dict()
+    lp = lp_cases.structured_feasible_lp(0, 12, 30)
+    lp.c[4] = np.nan
+    print("[case] refine every check: NaN cost fp32", flush=True)
+    r, *_ = solve(engine, lp, max_iterations=5000, precision="fp32", refine_every_check=1)
+    assert r.status == 9, r.as_dict()
