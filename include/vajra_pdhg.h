@@ -142,9 +142,9 @@ typedef struct vj_params {
     int power_iterations;          /* RESTARTED: power-iteration steps for ||A||_2; default 64 */
     double restart_drift;          /* RESTARTED: also restart (an FP64 refinement step) when a check's
                                       FP64 rel-KKT exceeds restart_drift times the least rel-KKT of
-                                      the epoch; 0 = default (2 with FP32 values, off with FP64,
-                                      BF16 and FP16 values), < 0 = off, > 1 = this factor for every
-                                      precision */
+                                      the epoch; 0 = default (2 with FP32 values, 4 with BF16 and
+                                      FP16 values refined at every check, off otherwise), < 0 = off,
+                                      > 1 = this factor for every precision */
     double primal_weight_integral; /* RESTARTED: integral gain ki of the primal-weight update (see
                                       primal_weight_smoothing), in [0, 1); default 0 (PDLP's
                                       smoothing) */

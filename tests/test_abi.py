@@ -43,7 +43,8 @@ def test_params_defaults_follow_reference():
     assert q.primal_weight_smoothing == 0.5 and q.primal_weight_integral == 0.0
     assert q.refine_every_check == 0   # per precision: every check with 16-bit values, restarts otherwise
     assert q.power_iterations == 64 and q.step_scale == 0.95
-    assert q.restart_drift == 0.0   # 0: the engine's default (2 with FP32 iterates, off otherwise)
+    assert q.restart_drift == 0.0   # 0: the engine's default (2 with FP32 values, 4 with 16-bit values refined
+                                    # at every check, off otherwise)
 
 
 def test_no_device_is_reported_not_faked():

@@ -1068,15 +1068,31 @@ int setup_splits(vj_ctx* ctx, FastEngine& E) {
 constexpr double kEllMaxPadding = 1.25;
 
 // FP32 iterates: a check whose FP64 rel-KKT exceeds this factor times the least rel-KKT of
-// the epoch triggers a restart, i.e. an FP64 refinement step. (BF16 iterates do not use it:
-// their rounding noise moves the rel-KKT by more than this factor from check to check.)
+// the epoch triggers a restart, i.e. an FP64 refinement step.
 constexpr double kRefineDrift = 2.0;
+// 16-bit values refined at every check: the same rule with a wider factor. Within a long
+// epoch their iteration can still drift away from the solution while its working-precision
+// residual falls (BF16 on the 1M x 2M LP: from rel-KKT 3.7e-8 up to 8e-2 over 1.2M
+// iterations without a restart); their rounding noise moves the rel-KKT by more from check
+// to check than FP32's, so the factor is 4 (at 1e-6 it never fires: the same iterations as
+// without it). With refinement at restarts only they do not use the rule: their noise then
+// exceeds even this factor (BF16: 843 restarts in 150 s).
+constexpr double kRefineDrift16 = 4.0;
 
 // Where the FP64 centre moves: 1 = at every check (16-bit matrix values by default, whose
 // rounding error otherwise grows with the distance the epoch has moved), 0 = at restarts.
 int refine_every_check(const vj_params& p, int prec) {
     if (p.refine_every_check != 0) return p.refine_every_check > 0 ? 1 : 0;
     return (prec == VJ_PREC_BF16 || prec == VJ_PREC_FP16) ? 1 : 0;
+}
+
+// The drift-restart factor: restart_drift > 1 as given, < 0 off, 0 the default per precision.
+double drift_factor(const vj_params& p, int prec) {
+    if (p.restart_drift > 1.0) return p.restart_drift;
+    if (p.restart_drift != 0.0) return 0.0;
+    if (prec == VJ_PREC_FP32) return kRefineDrift;
+    if ((prec == VJ_PREC_BF16 || prec == VJ_PREC_FP16) && refine_every_check(p, prec)) return kRefineDrift16;
+    return 0.0;
 }
 
 int setup_one_ell(vj_ctx* ctx, FastEngine& E, int rows, const int* rp, const int* ci, const double* vals_dev,
@@ -1184,8 +1200,7 @@ int enqueue_block(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     a.restart_sufficient = ctx->params.restart_sufficient;
     a.restart_necessary = ctx->params.restart_necessary;
     a.restart_artificial = ctx->params.restart_artificial;
-    const double drift = ctx->params.restart_drift;
-    a.kkt_drift = drift > 1.0 ? drift : (drift == 0.0 && E.prec == VJ_PREC_FP32 ? kRefineDrift : 0.0);
+    a.kkt_drift = drift_factor(ctx->params, E.prec);
     a.pw_theta = ctx->params.primal_weight_smoothing;
     a.pw_ki = ctx->params.primal_weight_integral;
     a.refine_every_check = refine_every_check(ctx->params, E.prec);
@@ -1363,7 +1378,8 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     // outputs are recomputed by the first real check. BF16 values refined at every check: the
     // check is also the refinement step, whose frequency sets how far the 8-bit values'
     // error grows, so it may cost up to 8%, in [64, 256] (on the 1M x 2M LP K = 64 took
-    // 131,136 iterations against 297,472 at K = 256; on the batches 2-7% less time).
+    // 131,136 iterations against 297,472 at K = 256; on the batches 2-7% less time). FP16
+    // keeps the 2% rule: at 8% its staircase batch took 3% longer (2.20 s against 2.14 s).
     if (prm.check_frequency == 0) {
         ScopedEvent se0, se1;
         VJ_HIP(hipEventCreate(&se0.e));
@@ -1379,7 +1395,7 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
         VJ_HIP(hipEventElapsedTime(&ms, e0, e1));
         const double t_check = 1000.0 * ms / 3 + 20.0;   // + control and restart kernels
         const double t_iter = bestA + bestAt;
-        const bool bf16_refined = E.prec == VJ_PREC_BF16 && refine_every_check(prm, E.prec) != 0;
+        const bool bf16_refined = E.prec == VJ_PREC_BF16 && refine_every_check(prm, E.prec);
         const double allowance = bf16_refined ? 0.08 : 0.02;
         int K = bf16_refined ? 64 : 32;
         while (K < 256 && t_check / K > allowance * t_iter) K *= 2;

@@ -1358,11 +1358,11 @@ __global__ __launch_bounds__(kBlock) void kkt_finish(
 // Restart rule (reflected restarted Halpern PDHG, Lu & Yang): with r the weighted
 // fixed-point residual sqrt(omega |dx|^2 + |dy|^2 / omega), restart when
 // r <= 0.2 r0, or r <= 0.8 r0 and r > r_prev, or the epoch exceeds a fraction (default
-// 0.5) of all iterations. With FP32 iterates (kkt_drift > 0) it also restarts when the
-// FP64 rel-KKT of a check exceeds kkt_drift times the least rel-KKT of the epoch: the
-// working-precision iteration then converges towards the fixed point of its rounded data
-// while the true error grows, and the restart's FP64 re-centring is the refinement step
-// that removes that error. At a restart the primal weight follows a proportional-integral
+// 0.5) of all iterations. With kkt_drift > 0 (by default FP32 values, and 16-bit values
+// refined at every check) it also restarts when the FP64 rel-KKT of a check exceeds
+// kkt_drift times the least rel-KKT of the epoch: the working-precision iteration then
+// drifts towards the fixed point of its rounded data while the true error grows, and the
+// restart (a new epoch from the current point) removes that error. At a restart the primal weight follows a proportional-integral
 // controller on the error e = log(omega |Dx| / |Dy|) (|Dx|, |Dy|: the distances the epoch moved):
 // log omega <- log omega - (theta e + ki sum(e)); with ki = 0 (the default, theta = 0.5) it is
 // PDLP's smoothing log omega <- theta log(|Dy|/|Dx|) + (1 - theta) log omega.
@@ -1381,7 +1381,7 @@ struct ControlArgs {
     const int* col_off;
     int K;
     double restart_sufficient, restart_necessary, restart_artificial, pw_theta;
-    double kkt_drift;              // 0: off (FP64 and BF16 iterates)
+    double kkt_drift;              // drift-restart factor, 0: off
     double pw_ki;                  // integral gain of the primal-weight controller
     int refine_every_check;        // 1: re-centre at every check, 0: at restarts only
     double* hist;                  // per instance: [checks] rel_kkt history (may be null)
