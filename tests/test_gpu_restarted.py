@@ -290,26 +290,28 @@ def test_structured_batch_against_highs(engine, prec, lanes):
         assert abs(got - opt) <= 1e-6 * (1 + abs(opt)), (got, opt)
 
 
+@pytest.mark.parametrize("threads", [1024, 512])
 @pytest.mark.parametrize("prec,stream,staged", [("fp64", -1, -5), ("fp32", -2, -6), ("bf16", -1, -5),
                                                 ("fp64", -1, -7), ("fp32", -2, -7), ("bf16", -1, -7),
                                                 ("fp16", -1, -5), ("fp16", -1, -7)])
-def test_staged_equals_tile_streamed(engine, prec, stream, staged):
+def test_staged_equals_tile_streamed(engine, prec, stream, staged, threads):
     """The LDS-staged steps form the same products and add each row in index order, as the
     tile-streamed steps do, so one check block of a batch (before the first restart decision,
     whose residual partials the two kernels sum in different tile orders) returns the same bits.
     The batch mixes random and structured LPs, with windows wider than one LDS chunk (the
     7,000 x 17,500 instance copies its x_bar window in two chunks in FP32, three in FP64).
     Lanes -7 reads 16-bit column offsets from each tile's window start (with BF16 and FP16
-    values, packed with the value into one 4-byte record)."""
+    values, packed with the value into one 4-byte record). Both workgroup shapes (1,024 and
+    512 threads, with tiles and LDS windows of their own sizes) give the same bits."""
     lps = [vp.gen_random_lp(4100 + i, m, int(2.5 * m)) for i, m in enumerate([500, 7000, 1300])]
     lps += [lp_cases.structured_feasible_lp(*s) for s in STRUCTURED[:3]]
     engine.upload_batch(lps)
     out = []
     for lanes in (stream, staged):
-        print(f"[case] staged vs tile-streamed {prec} lanes={lanes}", flush=True)
+        print(f"[case] staged vs tile-streamed {prec} lanes={lanes} threads={threads}", flush=True)
         res = engine.solve_batch(vp.default_params(vp.MODE_RESTARTED, precision=prec, tolerance=0.0,
                                                    max_iterations=64, check_frequency=64, lanes_A=lanes,
-                                                   lanes_At=lanes))
+                                                   lanes_At=lanes, stage_threads=threads))
         want = 0 if lanes == stream else staged   # the result reports the tile-streamed kernel as 0
         assert all((r.lanes_A, r.lanes_At) == (want, want) and r.iterations == 64 for r in res), res[0].as_dict()
         out.append(engine.download_batch())
@@ -318,8 +320,9 @@ def test_staged_equals_tile_streamed(engine, prec, stream, staged):
         assert np.array_equal(x0, x1) and np.array_equal(y0, y1)
 
 
-@pytest.mark.parametrize("prec,lanes", [("fp32", -7), ("fp64", -5)])
-def test_staged_batch_graph_and_stream_identical(engine, prec, lanes):
+@pytest.mark.parametrize("prec,lanes,threads", [("fp32", -7, 1024), ("fp64", -5, 1024), ("fp16", -7, 512),
+                                                ("fp64", -5, 512)])
+def test_staged_batch_graph_and_stream_identical(engine, prec, lanes, threads):
     """A batch whose instances stop at very different iteration counts, with the LDS-staged
     kernels: as instances finish, the host drops their tiles from the staged launches (and
     captures the block graph again with the smaller grids). Graph replays and direct launches
@@ -330,10 +333,10 @@ def test_staged_batch_graph_and_stream_identical(engine, prec, lanes):
     engine.upload_batch(lps)
     out = []
     for graph in (1, 0):
-        print(f"[case] staged batch {prec} lanes={lanes} use_graph={graph}", flush=True)
+        print(f"[case] staged batch {prec} lanes={lanes} threads={threads} use_graph={graph}", flush=True)
         res = engine.solve_batch(vp.default_params(vp.MODE_RESTARTED, precision=prec, tolerance=1e-7,
                                                    max_iterations=200000, lanes_A=lanes, lanes_At=lanes,
-                                                   use_graph=graph))
+                                                   use_graph=graph, stage_threads=threads))
         xs, ys = engine.download_batch()
         for lp, r, x, y in zip(lps, res, xs, ys):
             assert r.status == 1 and (r.lanes_A, r.lanes_At) == (lanes, lanes), r.as_dict()

@@ -28,6 +28,8 @@ VARIANTS = {
     "split_A2_At4_groups": dict(lanes_A=2, lanes_At=4, col_splits_A=2, col_splits_At=4),
     "lds_staged": dict(lanes_A=-5, lanes_At=-5, col_splits_A=1, col_splits_At=1),
     "lds_staged_w16": dict(lanes_A=-7, lanes_At=-7, col_splits_A=1, col_splits_At=1),
+    "lds_staged_512": dict(lanes_A=-5, lanes_At=-5, col_splits_A=1, col_splits_At=1, stage_threads=512),
+    "lds_staged_w16_512": dict(lanes_A=-7, lanes_At=-7, col_splits_A=1, col_splits_At=1, stage_threads=512),
 }
 
 
@@ -67,16 +69,17 @@ def test_single_lp_blocks_equal_model(engine, variant, blocks):
 
 
 @pytest.mark.parametrize("blocks", [1, 3])
-@pytest.mark.parametrize("lanes", [-1, 8, -5, -7])
-def test_batch_blocks_equal_model(engine, lanes, blocks):
+@pytest.mark.parametrize("lanes,threads", [(-1, 0), (8, 0), (-5, 1024), (-7, 1024), (-5, 512), (-7, 512)])
+def test_batch_blocks_equal_model(engine, lanes, threads, blocks):
     """The batched kernels (every instance in one launch, per-instance state) on a batch of
-    LPs of different shapes: each instance equals its own model run."""
+    LPs of different shapes: each instance equals its own model run (the staged kernels in
+    both workgroup shapes)."""
     cases = lps()
     engine.upload_batch([lp for _, lp in cases])
-    print(f"[case] batch step parity lanes={lanes} blocks={blocks}", flush=True)
+    print(f"[case] batch step parity lanes={lanes} threads={threads} blocks={blocks}", flush=True)
     res = engine.solve_batch(vp.default_params(vp.MODE_RESTARTED, precision="fp64", tolerance=0.0,
                                                max_iterations=blocks * K, check_frequency=K, lanes_A=lanes,
-                                               lanes_At=lanes))
+                                               lanes_At=lanes, stage_threads=threads))
     xs, ys = engine.download_batch()
     for (name, lp), r, x, y in zip(cases, res, xs, ys):
         assert r.iterations == blocks * K, r.as_dict()

@@ -186,16 +186,38 @@ struct StepCfg {
     bool ell;   // sliced-ELL kernel (S = 1)
     bool stage = false;   // LDS-staged kernel (S = 1)
     bool w16 = false;     // ... reading 16-bit window offsets
+    int shape = 0;        // ... in workgroups of kStageShapeThreads[shape] threads
 };
 
 // An autotuner decision, kept for the rest of the upload: the same precision and kernel
 // parameters on the same problem reuse it, so repeated default solves run the same kernels
 // (and, the kernels being deterministic, return the same bits).
 struct TuneDecision {
-    int prec, lanes_A, lanes_At, col_splits_A, col_splits_At, check_frequency;
+    int prec, lanes_A, lanes_At, col_splits_A, col_splits_At, check_frequency, stage_threads;
     StepCfg a, at;
     int GA, GAt, K_tuned;
 };
+
+// Row tiles of the LDS-staged kernels for one step and workgroup shape: tiles that never span
+// two instances, at most rpl * threads rows and cap entries (cap: the entry capacity they were
+// built for, 0 while none are; it differs between 8-byte and 4-byte iterates), each tile's
+// column window (win), per instance the first / last tile (tf / tl), and the sum of the
+// windows' widths (gwin). With every window at most 65,536 columns wide (w16ok), 16-bit window
+// offsets of the entries (off16) replace the 4-byte column indices, and BF16/FP16 values come
+// as 4-byte (offset, value) records (w16rec). inst: each tile's instance (hInst: on the host);
+// desc: per tile rows, entries, window and instance (8 ints). list: the tiles a staged launch
+// runs, nlist of them (its grid): every tile at the start of a solve, then the tiles of the
+// instances still active, rebuilt by the host (with the stream drained) when half of the
+// listed tiles belong to finished instances.
+struct StageLayout {
+    DevBuf tile, win, tf, tl, off16, inst, list, desc, w16rec;
+    std::vector<int> hInst;
+    int threads = 0, rpl = 1, nt = 0, nlist = 0, cap = 0;
+    int64_t gwin = 0;
+    bool w16ok = false;
+};
+// Threads per workgroup of the staged steps, per shape index.
+constexpr int kStageShapeThreads[2] = {1024, 512};
 
 struct FastEngine {
     int prec = VJ_PREC_FP64;
@@ -249,27 +271,16 @@ struct FastEngine {
     DevBuf listTA, listTAt;
     std::vector<int> hFirstTA, hLastTA, hFirstTAt, hLastTAt;
     int nlistTA = 0, nlistTAt = 0;
-    // LDS-staged kernels (batches): row tiles that never span two instances, at most
-    // kStageThreads rows and stage_cap entries (gcap*: the cap they were built for, which
-    // differs between 8-byte and 4-byte iterates), each tile's column window, per instance the
-    // first / last tile, and the sum of the windows' widths (gwin*); stageA / stageAt select them
-    // with every window at most 65,536 columns wide, 16-bit window offsets of the entries
-    // (off16*, built when w16ok*) replace the 4-byte column indices (w16A / w16At select them)
-    // instG*: each staged tile's instance (hInstG*: on the host). listG*: the tiles a staged
-    // launch runs, nlist* of them (its grid): every tile at the start of a solve, then the tiles
-    // of the instances still active, rebuilt by the host (with the stream drained) when half of
-    // the listed tiles belong to finished instances; pruned: some list leaves tiles out.
-    DevBuf tileGA, tileGAt, winGA, winGAt, tfGA, tlGA, tfGAt, tlGAt, off16A, off16At, instGA, instGAt;
-    DevBuf listGA, listGAt;
-    DevBuf descGA, descGAt;   // per staged tile: rows, entries, window, instance (8 ints)
-    DevBuf w16recA, w16recAt;   // BF16/FP16: 4-byte (16-bit window offset, value) records of the staged tiles
-    std::vector<int> hInstGA, hInstGAt;
-    int rplA = 1, rplAt = 1;   // rows per lane of the staged tiles (1 or 2)
-    int nlistA = 0, nlistAt = 0;
+    // LDS-staged kernels (batches), one StageLayout per step and workgroup shape (stA / stAt
+    // [0]: 1,024 threads, [1]: 512; shA / shAt: the shape the steps run, LA() / LAt() its
+    // layout); stageA / stageAt select the staged steps, w16A / w16At their 16-bit window
+    // offsets. pruned: some list leaves tiles out.
+    StageLayout stA[2], stAt[2];
+    int shA = 0, shAt = 0;
+    StageLayout& LA() { return stA[shA]; }
+    StageLayout& LAt() { return stAt[shAt]; }
     bool pruned = false;
-    int ntGA = 0, ntGAt = 0, gcapA = 0, gcapAt = 0;
-    int64_t gwinA = 0, gwinAt = 0;
-    bool stageA = false, stageAt = false, w16okA = false, w16okAt = false, w16A = false, w16At = false;
+    bool stageA = false, stageAt = false, w16A = false, w16At = false;
     int hist_cap = 0;
     hipGraphExec_t gexec = nullptr;
     InstState* h_state[2] = {nullptr, nullptr};   // pinned, h_state_cap entries each
@@ -350,32 +361,37 @@ int launch_primal(vj_ctx* ctx, FastEngine& E, const V* tv, T* x, T* xbar, const 
         return VJ_OK;
     }
     if (E.stageAt) {
-        VJ_TRY(require({E.descGAt.p, E.listGAt.p}, "staged primal_step"));
-        if (E.nlistAt == 0) return VJ_OK;   // every instance has finished
+        VJ_TRY(require({E.LAt().desc.p, E.LAt().list.p}, "staged primal_step"));
+        if (E.LAt().nlist == 0) return VJ_OK;   // every instance has finished
         auto go = [&](auto ent) {
             using Ent = decltype(ent);
             auto go2 = [&](auto multi, auto rpl) {
-                hipLaunchKernelGGL((staged_primal_step<Ent, T, decltype(multi)::value, decltype(rpl)::value>),
-                                   dim3(E.nlistAt), dim3(kStageThreads), 0, ctx->stream, E.listGAt.as<int>(),
-                                   E.descGAt.as<int4>(), P.tp.as<int>(), ent, y,
+                auto go3 = [&](auto nt) {
+                hipLaunchKernelGGL((staged_primal_step<Ent, T, decltype(multi)::value, decltype(rpl)::value,
+                                                       decltype(nt)::value>),
+                                   dim3(E.LAt().nlist), dim3(decltype(nt)::value), 0, ctx->stream, E.LAt().list.as<int>(),
+                                   E.LAt().desc.as<int4>(), P.tp.as<int>(), ent, y,
                                    E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), x, xbar, ax, E.xhat.as<double>(),
                                    E.xfull.as<double>(), E.st.as<InstState>(), E.part_x.as<double>(), it, flags, E.rho);
+                };
+                if (E.LAt().threads == 512) go3(std::integral_constant<int, 512>{});
+                else go3(std::integral_constant<int, 1024>{});
             };
             using One = std::integral_constant<int, 1>;
             using Two = std::integral_constant<int, 2>;
             if (E.multi) {
-                if (E.rplAt == 2) go2(std::true_type{}, Two{}); else go2(std::true_type{}, One{});
+                if (E.LAt().rpl == 2) go2(std::true_type{}, Two{}); else go2(std::true_type{}, One{});
             } else {
-                if (E.rplAt == 2) go2(std::false_type{}, Two{}); else go2(std::false_type{}, One{});
+                if (E.LAt().rpl == 2) go2(std::false_type{}, Two{}); else go2(std::false_type{}, One{});
             }
         };
         if (E.w16At) {
-            VJ_TRY(require({E.off16At.p}, "staged primal_step (16-bit offsets)"));
+            VJ_TRY(require({E.LAt().off16.p}, "staged primal_step (16-bit offsets)"));
             if constexpr (std::is_same<V, bf16_t>::value || std::is_same<V, f16_t>::value) {
-                VJ_TRY(require({E.w16recAt.p}, "staged primal_step (16-bit value records)"));
-                go(Win16Records<V>{E.w16recAt.as<uint32_t>()});
+                VJ_TRY(require({E.LAt().w16rec.p}, "staged primal_step (16-bit value records)"));
+                go(Win16Records<V>{E.LAt().w16rec.as<uint32_t>()});
             } else {
-                go(Win16Entries<V>{E.off16At.as<uint16_t>(), tv});
+                go(Win16Entries<V>{E.LAt().off16.as<uint16_t>(), tv});
             }
         } else {
             VJ_TRY(with_entries<V>(E.recAt, E.recAt_buf, P.ti.as<int>(), tv, "staged records", go));
@@ -479,32 +495,37 @@ int launch_dual(vj_ctx* ctx, FastEngine& E, const V* av, const T* xbar, T* y, in
         return VJ_OK;
     }
     if (E.stageA) {
-        VJ_TRY(require({E.descGA.p, E.listGA.p}, "staged dual_step"));
-        if (E.nlistA == 0) return VJ_OK;
+        VJ_TRY(require({E.LA().desc.p, E.LA().list.p}, "staged dual_step"));
+        if (E.LA().nlist == 0) return VJ_OK;
         auto go = [&](auto ent) {
             using Ent = decltype(ent);
             auto go2 = [&](auto multi, auto rpl) {
-                hipLaunchKernelGGL((staged_dual_step<Ent, T, decltype(multi)::value, decltype(rpl)::value>),
-                                   dim3(E.nlistA), dim3(kStageThreads), 0, ctx->stream, E.listGA.as<int>(),
-                                   E.descGA.as<int4>(), P.rp.as<int>(), ent, xbar,
+                auto go3 = [&](auto nt) {
+                hipLaunchKernelGGL((staged_dual_step<Ent, T, decltype(multi)::value, decltype(rpl)::value,
+                                                     decltype(nt)::value>),
+                                   dim3(E.LA().nlist), dim3(decltype(nt)::value), 0, ctx->stream, E.LA().list.as<int>(),
+                                   E.LA().desc.as<int4>(), P.rp.as<int>(), ent, xbar,
                                    yht, lrs, urs, y, ay, E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(),
                                    E.part_y.as<double>(), it, flags, E.rho);
+                };
+                if (E.LA().threads == 512) go3(std::integral_constant<int, 512>{});
+                else go3(std::integral_constant<int, 1024>{});
             };
             using One = std::integral_constant<int, 1>;
             using Two = std::integral_constant<int, 2>;
             if (E.multi) {
-                if (E.rplA == 2) go2(std::true_type{}, Two{}); else go2(std::true_type{}, One{});
+                if (E.LA().rpl == 2) go2(std::true_type{}, Two{}); else go2(std::true_type{}, One{});
             } else {
-                if (E.rplA == 2) go2(std::false_type{}, Two{}); else go2(std::false_type{}, One{});
+                if (E.LA().rpl == 2) go2(std::false_type{}, Two{}); else go2(std::false_type{}, One{});
             }
         };
         if (E.w16A) {
-            VJ_TRY(require({E.off16A.p}, "staged dual_step (16-bit offsets)"));
+            VJ_TRY(require({E.LA().off16.p}, "staged dual_step (16-bit offsets)"));
             if constexpr (std::is_same<V, bf16_t>::value || std::is_same<V, f16_t>::value) {
-                VJ_TRY(require({E.w16recA.p}, "staged dual_step (16-bit value records)"));
-                go(Win16Records<V>{E.w16recA.as<uint32_t>()});
+                VJ_TRY(require({E.LA().w16rec.p}, "staged dual_step (16-bit value records)"));
+                go(Win16Records<V>{E.LA().w16rec.as<uint32_t>()});
             } else {
-                go(Win16Entries<V>{E.off16A.as<uint16_t>(), av});
+                go(Win16Entries<V>{E.LA().off16.as<uint16_t>(), av});
             }
         } else {
             VJ_TRY(with_entries<V>(E.recA, E.recA_buf, P.ci.as<int>(), av, "staged records", go));
@@ -659,66 +680,58 @@ int tile_instance_ranges(hipStream_t s, int ntiles, const DevBuf& tile_row, cons
 // Rows per lane of a staged tile: 2 where 1,024 rows of the average length fill at most three
 // quarters of the tile's entry capacity (short rows: the columns of a random instance, 6
 // entries, or of a staircase instance, about 8), else 1. Tiles still stop at the capacity.
-int stage_rows_per_lane(double avg_row, int cap) { return kStageThreads * avg_row <= 0.75 * cap ? 2 : 1; }
+int stage_rows_per_lane(double avg_row, int cap, int threads) { return threads * avg_row <= 0.75 * cap ? 2 : 1; }
 
-// Row tiles of the LDS-staged kernels, built on the device once per upload and entry size:
-// for batches (the kernels are autotuner candidates there), and for a single LP only when
-// the caller forces them (lanes -5 / -6): a single LP's tiles read windows about as wide as
-// the whole vector, which only a small LP can afford.
+// The staged tiles of one step (rows rp / indices ci, `rows` rows) for one workgroup shape,
+// built on the device, with every tile listed.
+int build_stage_layout(hipStream_t s, FastEngine& E, StageLayout& L, int threads, int rows, const int* rp,
+                       const int* ci, const int* inst_of, const int* off, const int* d_off) {
+    const int cap = threads * kStageLaneBytes / static_cast<int>(E.prec == VJ_PREC_FP64 ? 8 : 4);
+    int widest = 0;
+    L.threads = threads;
+    L.rpl = stage_rows_per_lane(static_cast<double>(E.nnz) / std::max(1, rows), cap, threads);
+    VJ_TRY(device_row_tiles(s, rows, 1, rp, 0, cap, L.rpl * threads, L.tile, &L.nt, inst_of, off));
+    VJ_TRY(device_tile_windows(s, L.nt, L.tile.as<int>(), rp, ci, L.win, &L.gwin, &widest));
+    VJ_TRY(device_instance_tiles(s, E.count, L.nt, L.tile.as<int>(), d_off, L.tf, L.tl));
+    L.w16ok = widest <= 65536;
+    if (L.w16ok) VJ_TRY(device_window_offsets(s, L.nt, L.tile.as<int>(), rp, ci, L.win.as<int>(), E.nnz, L.off16));
+    VJ_TRY(device_tile_instances(s, L.nt, L.tile.as<int>(), inst_of, L.inst));
+    VJ_TRY(device_stage_descs(s, L.nt, L.tile.as<int>(), rp, L.win.as<int>(), L.inst.as<int>(), L.desc));
+    VJ_TRY(L.list.alloc(std::max<size_t>(static_cast<size_t>(L.nt), 1) * 4));
+    L.hInst.resize(L.nt);
+    if (L.nt > 0)
+        VJ_HIP(hipMemcpyAsync(L.hInst.data(), L.inst.p, static_cast<size_t>(L.nt) * 4, hipMemcpyDeviceToHost, s));
+    std::vector<int> all(L.nt);
+    for (int t = 0; t < L.nt; ++t) all[t] = t;
+    if (L.nt > 0) VJ_HIP(hipMemcpyAsync(L.list.p, all.data(), static_cast<size_t>(L.nt) * 4, hipMemcpyHostToDevice, s));
+    L.nlist = L.nt;
+    VJ_HIP(hipStreamSynchronize(s));   // all and hInst are complete
+    L.cap = cap;
+    return VJ_OK;
+}
+
+// Row tiles of the LDS-staged kernels, built on the device once per upload and entry size,
+// for both workgroup shapes: for batches (the kernels are autotuner candidates there), and
+// for a single LP only when the caller forces them (lanes -5 / -6 / -7): a single LP's tiles
+// read windows about as wide as the whole vector, which only a small LP can afford.
 int setup_stage(vj_ctx* ctx, FastEngine& E) {
     Problem& P = ctx->prob;
     const vj_params& prm = ctx->params;
     hipStream_t s = ctx->stream;
-    const int cap = E.prec == VJ_PREC_FP64 ? stage_cap<double>() : stage_cap<float>();
     const int* rinst = E.multi ? P.row_inst.as<int>() : nullptr;
     const int* cinst = E.multi ? P.col_inst.as<int>() : nullptr;
     const int* roff = E.multi ? P.d_row_off.as<int>() : nullptr;
     const int* coff = E.multi ? P.d_col_off.as<int>() : nullptr;
-    if ((E.multi || prm.lanes_A <= -5) && E.gcapA != cap) {
-        int widest = 0;
-        E.rplA = stage_rows_per_lane(static_cast<double>(E.nnz) / std::max(1, E.m), cap);
-        VJ_TRY(device_row_tiles(s, E.m, 1, P.rp.as<int>(), 0, cap, E.rplA * kStageThreads, E.tileGA, &E.ntGA, rinst,
-                                roff));
-        VJ_TRY(device_tile_windows(s, E.ntGA, E.tileGA.as<int>(), P.rp.as<int>(), P.ci.as<int>(), E.winGA, &E.gwinA,
-                                   &widest));
-        VJ_TRY(device_instance_tiles(s, E.count, E.ntGA, E.tileGA.as<int>(), P.d_row_off.as<int>(), E.tfGA, E.tlGA));
-        E.w16okA = widest <= 65536;
-        if (E.w16okA)
-            VJ_TRY(device_window_offsets(s, E.ntGA, E.tileGA.as<int>(), P.rp.as<int>(), P.ci.as<int>(),
-                                         E.winGA.as<int>(), E.nnz, E.off16A));
-        VJ_TRY(device_tile_instances(s, E.ntGA, E.tileGA.as<int>(), rinst, E.instGA));
-        VJ_TRY(device_stage_descs(s, E.ntGA, E.tileGA.as<int>(), P.rp.as<int>(), E.winGA.as<int>(), E.instGA.as<int>(),
-                                  E.descGA));
-        VJ_TRY(E.listGA.alloc(std::max<size_t>(static_cast<size_t>(E.ntGA), 1) * 4));
-        E.hInstGA.resize(E.ntGA);
-        if (E.ntGA > 0)
-            VJ_HIP(hipMemcpyAsync(E.hInstGA.data(), E.instGA.p, static_cast<size_t>(E.ntGA) * 4, hipMemcpyDeviceToHost, s));
-        VJ_HIP(hipStreamSynchronize(s));
-        E.gcapA = cap;
-    }
-    if ((E.multi || prm.lanes_At <= -5) && E.gcapAt != cap) {
-        int widest = 0;
-        E.rplAt = stage_rows_per_lane(static_cast<double>(E.nnz) / std::max(1, E.n), cap);
-        VJ_TRY(device_row_tiles(s, E.n, 1, P.tp.as<int>(), 0, cap, E.rplAt * kStageThreads, E.tileGAt, &E.ntGAt, cinst,
-                                coff));
-        VJ_TRY(device_tile_windows(s, E.ntGAt, E.tileGAt.as<int>(), P.tp.as<int>(), P.ti.as<int>(), E.winGAt,
-                                   &E.gwinAt, &widest));
-        VJ_TRY(device_instance_tiles(s, E.count, E.ntGAt, E.tileGAt.as<int>(), P.d_col_off.as<int>(), E.tfGAt,
-                                     E.tlGAt));
-        E.w16okAt = widest <= 65536;
-        if (E.w16okAt)
-            VJ_TRY(device_window_offsets(s, E.ntGAt, E.tileGAt.as<int>(), P.tp.as<int>(), P.ti.as<int>(),
-                                         E.winGAt.as<int>(), E.nnz, E.off16At));
-        VJ_TRY(device_tile_instances(s, E.ntGAt, E.tileGAt.as<int>(), cinst, E.instGAt));
-        VJ_TRY(device_stage_descs(s, E.ntGAt, E.tileGAt.as<int>(), P.tp.as<int>(), E.winGAt.as<int>(),
-                                  E.instGAt.as<int>(), E.descGAt));
-        VJ_TRY(E.listGAt.alloc(std::max<size_t>(static_cast<size_t>(E.ntGAt), 1) * 4));
-        E.hInstGAt.resize(E.ntGAt);
-        if (E.ntGAt > 0)
-            VJ_HIP(hipMemcpyAsync(E.hInstGAt.data(), E.instGAt.p, static_cast<size_t>(E.ntGAt) * 4,
-                                  hipMemcpyDeviceToHost, s));
-        VJ_HIP(hipStreamSynchronize(s));
-        E.gcapAt = cap;
+    const int esz = E.prec == VJ_PREC_FP64 ? 8 : 4;
+    for (int sh = 0; sh < 2; ++sh) {
+        const int threads = kStageShapeThreads[sh];
+        const int cap = threads * kStageLaneBytes / esz;
+        if ((E.multi || prm.lanes_A <= -5) && E.stA[sh].cap != cap)
+            VJ_TRY(build_stage_layout(s, E, E.stA[sh], threads, E.m, P.rp.as<int>(), P.ci.as<int>(), rinst, roff,
+                                      P.d_row_off.as<int>()));
+        if ((E.multi || prm.lanes_At <= -5) && E.stAt[sh].cap != cap)
+            VJ_TRY(build_stage_layout(s, E, E.stAt[sh], threads, E.n, P.tp.as<int>(), P.ti.as<int>(), cinst, coff,
+                                      P.d_col_off.as<int>()));
     }
     return VJ_OK;
 }
@@ -747,14 +760,21 @@ int set_launch_lists(vj_ctx* ctx, FastEngine& E, const InstState* hs) {
             if (active(q)) return true;
         return false;
     };
-    if (E.gcapA) VJ_TRY(upload(E.ntGA, [&](int t) { return active(E.hInstGA[t]); }, E.listGA, E.nlistA));
-    if (E.gcapAt) VJ_TRY(upload(E.ntGAt, [&](int t) { return active(E.hInstGAt[t]); }, E.listGAt, E.nlistAt));
+    // staged tiles: the running shape's lists; with hs null, every shape's (the autotuner may pick
+    // either, and a list pruned by an earlier solve must not carry over)
+    for (int sh = 0; sh < 2; ++sh) {
+        for (StageLayout* L : {&E.stA[sh], &E.stAt[sh]}) {
+            const bool running = L == &E.LA() || L == &E.LAt();
+            if (L->cap && (hs == nullptr || running))
+                VJ_TRY(upload(L->nt, [&](int t) { return active(L->hInst[t]); }, L->list, L->nlist));
+        }
+    }
     if (E.multi && E.tiles_built) {
         VJ_TRY(upload(E.ntA, [&](int t) { return any_active(E.hFirstTA[t], E.hLastTA[t]); }, E.listTA, E.nlistTA));
         VJ_TRY(upload(E.ntAt, [&](int t) { return any_active(E.hFirstTAt[t], E.hLastTAt[t]); }, E.listTAt,
                       E.nlistTAt));
     }
-    E.pruned = hs != nullptr && ((E.gcapA && E.nlistA < E.ntGA) || (E.gcapAt && E.nlistAt < E.ntGAt) ||
+    E.pruned = hs != nullptr && ((E.LA().cap && E.LA().nlist < E.LA().nt) || (E.LAt().cap && E.LAt().nlist < E.LAt().nt) ||
                                  (E.multi && E.tiles_built && (E.nlistTA < E.ntA || E.nlistTAt < E.ntAt)));
     if (E.gexec) {
         VJ_HIP(hipGraphExecDestroy(E.gexec));
@@ -797,8 +817,8 @@ int setup(vj_ctx* ctx, FastEngine& E) {
         E.tiles_built = true;
     }
     VJ_TRY(setup_stage(ctx, E));
-    const int px_slots = std::max({partial_slots(n, cnt), E.ntAt + cnt + 1, E.ntGAt + cnt + 1});
-    const int py_slots = std::max({partial_slots(m, cnt), E.ntA + cnt + 1, E.ntGA + cnt + 1});
+    const int px_slots = std::max({partial_slots(n, cnt), E.ntAt + cnt + 1, E.stAt[0].nt + cnt + 1, E.stAt[1].nt + cnt + 1});
+    const int py_slots = std::max({partial_slots(m, cnt), E.ntA + cnt + 1, E.stA[0].nt + cnt + 1, E.stA[1].nt + cnt + 1});
     VJ_TRY(E.part_x.alloc(static_cast<size_t>(px_slots) * 3 * 8));
     VJ_TRY(E.part_y.alloc(static_cast<size_t>(py_slots) * 3 * 8));
     VJ_TRY(E.part_kr.alloc(static_cast<size_t>(py_slots) * 3 * 8));
@@ -957,8 +977,10 @@ int setup(vj_ctx* ctx, FastEngine& E) {
             VJ_LAUNCHED();
             return VJ_OK;
         };
-        if (E.w16okA) VJ_TRY(pack(E.off16A, E.av_lo, E.w16recA));
-        if (E.w16okAt) VJ_TRY(pack(E.off16At, E.tv_lo, E.w16recAt));
+        for (int sh = 0; sh < 2; ++sh) {
+            if (E.stA[sh].w16ok) VJ_TRY(pack(E.stA[sh].off16, E.av_lo, E.stA[sh].w16rec));
+            if (E.stAt[sh].w16ok) VJ_TRY(pack(E.stAt[sh].off16, E.tv_lo, E.stAt[sh].w16rec));
+        }
     }
 
     // state
@@ -1189,10 +1211,10 @@ int enqueue_block(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     a.py = E.part_y.as<double>();
     a.Ry = E.SA > 1 ? kBlock * finish_rows<T>() : (E.ellA ? kBlock : kBlock / E.GA);
     const bool sx = E.SAt == 1 && E.streamAt && !E.ellAt, sy = E.SA == 1 && E.streamA && !E.ellA;
-    a.xt_first = E.stageAt ? E.tfGAt.as<int>() : (sx ? E.tfAt.as<int>() : nullptr);
-    a.xt_last = E.stageAt ? E.tlGAt.as<int>() : (sx ? E.tlAt.as<int>() : nullptr);
-    a.yt_first = E.stageA ? E.tfGA.as<int>() : (sy ? E.tfA.as<int>() : nullptr);
-    a.yt_last = E.stageA ? E.tlGA.as<int>() : (sy ? E.tlA.as<int>() : nullptr);
+    a.xt_first = E.stageAt ? E.LAt().tf.as<int>() : (sx ? E.tfAt.as<int>() : nullptr);
+    a.xt_last = E.stageAt ? E.LAt().tl.as<int>() : (sx ? E.tlAt.as<int>() : nullptr);
+    a.yt_first = E.stageA ? E.LA().tf.as<int>() : (sy ? E.tfA.as<int>() : nullptr);
+    a.yt_last = E.stageA ? E.LA().tl.as<int>() : (sy ? E.tlA.as<int>() : nullptr);
     a.kkt = E.kkt8.as<double>();
     a.row_off = ctx->prob.d_row_off.as<int>();
     a.col_off = ctx->prob.d_col_off.as<int>();
@@ -1262,11 +1284,23 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     // forced: the caller's col_splits (0 = free choice, 1 = unsplit only, >1 = split only)
     const bool f32 = E.prec == VJ_PREC_FP32;
     // ell: the sliced-ELL copy exists (single LP, little padding, or forced by lanes -3 / -4)
-    auto base = [&](bool tune, int lanes, double avg, int Sm, int forced, bool ell, bool stage, bool w16) {
+    // staged candidates per workgroup shape: stage[sh] (its tiles exist and qualify), w16[sh]
+    // (its 16-bit offsets exist); stage_threads forces one shape (a forced staged kernel runs
+    // the 1,024-thread shape unless stage_threads asks for 512)
+    const int forced_shape = prm.stage_threads == 512 ? 1 : 0;
+    auto shape_ok = [&](int sh) { return prm.stage_threads == 0 || sh == forced_shape; };
+    auto staged = [&](double avg, int sh, bool rec, bool w16) {
+        StepCfg c{1, choose_lanes(avg), false, rec, false, true, w16};
+        c.shape = sh;
+        return c;
+    };
+    auto base = [&](bool tune, int lanes, double avg, int Sm, int forced, bool ell, const bool (&stage)[2],
+                    const bool (&w16)[2]) {
         std::vector<StepCfg> c;
         if (!tune) {
-            if (lanes == -7 && w16) c.push_back(StepCfg{1, choose_lanes(avg), false, false, false, true, true});
-            else if (lanes <= -5) c.push_back(StepCfg{1, choose_lanes(avg), false, lanes == -6 && f32, false, true});
+            const int sh = forced_shape;
+            if (lanes == -7 && w16[sh]) c.push_back(staged(avg, sh, false, true));
+            else if (lanes <= -5) c.push_back(staged(avg, sh, lanes == -6 && f32, false));
             else if (lanes <= -3 && ell) c.push_back({1, choose_lanes(avg), false, lanes == -4 && f32, true});
             else if (lanes < 0) c.push_back({Sm, Sm > 1 ? 0 : choose_lanes(avg), Sm == 1, (lanes == -2 || lanes == -4) && f32, false});
             else c.push_back({Sm, lanes, false, false, false});
@@ -1292,10 +1326,12 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
             }
             if (f32) c.push_back({1, choose_lanes(avg), true, true, false});
             c.push_back({1, choose_lanes(avg), true, false, false});
-            if (stage) {   // LDS-staged (batches whose tiles read narrow windows)
-                if (w16) c.push_back(StepCfg{1, choose_lanes(avg), false, false, false, true, true});
-                if (f32) c.push_back(StepCfg{1, choose_lanes(avg), false, true, false, true});
-                c.push_back(StepCfg{1, choose_lanes(avg), false, false, false, true});
+            // LDS-staged (batches whose tiles read narrow windows), the 1,024-thread shape first
+            for (int sh = 0; sh < 2; ++sh) {
+                if (!stage[sh] || !shape_ok(sh)) continue;
+                if (w16[sh]) c.push_back(staged(avg, sh, false, true));
+                if (f32) c.push_back(staged(avg, sh, true, false));
+                c.push_back(staged(avg, sh, false, false));
             }
             for (int g : {4, 8, 16, 32}) c.push_back({1, g, false, false, false});
         }
@@ -1304,34 +1340,40 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     // the staged kernels are candidates where their tiles exist (batches) and the windows they
     // copy to LDS add up to at most kStageMaxWindow times the entries they serve
     constexpr int64_t kStageMaxWindow = 8;
-    const int scap = E.prec == VJ_PREC_FP64 ? stage_cap<double>() : stage_cap<float>();
-    const bool stgA = E.gcapA == scap && E.gwinA <= kStageMaxWindow * E.nnz;
-    const bool stgAt = E.gcapAt == scap && E.gwinAt <= kStageMaxWindow * E.nnz;
-    std::vector<StepCfg> cA = base(tuneA, prm.lanes_A, avgA, SmA, prm.col_splits_A, E.ellLenA > 0, stgA, E.w16okA);
-    std::vector<StepCfg> cAt = base(tuneAt, prm.lanes_At, avgAt, SmAt, prm.col_splits_At, E.ellLenAt > 0, stgAt,
-                                    E.w16okAt);
+    const int esz = E.prec == VJ_PREC_FP64 ? 8 : 4;
+    bool stgA[2], stgAt[2], w16A[2], w16At[2];
+    for (int sh = 0; sh < 2; ++sh) {
+        const int scap = kStageShapeThreads[sh] * kStageLaneBytes / esz;
+        stgA[sh] = E.stA[sh].cap == scap && E.stA[sh].gwin <= kStageMaxWindow * E.nnz;
+        stgAt[sh] = E.stAt[sh].cap == scap && E.stAt[sh].gwin <= kStageMaxWindow * E.nnz;
+        w16A[sh] = E.stA[sh].w16ok;
+        w16At[sh] = E.stAt[sh].w16ok;
+    }
+    std::vector<StepCfg> cA = base(tuneA, prm.lanes_A, avgA, SmA, prm.col_splits_A, E.ellLenA > 0, stgA, w16A);
+    std::vector<StepCfg> cAt = base(tuneAt, prm.lanes_At, avgAt, SmAt, prm.col_splits_At, E.ellLenAt > 0, stgAt, w16At);
     auto apply = [](const StepCfg& c, int& S, int& G, int& Gs, bool& stream, bool& rec, bool& ell, bool& stage,
-                    bool& w16) {
+                    bool& w16, int& shape) {
         S = c.S;
         stream = c.S == 1 && c.stream;
         rec = c.rec;
         ell = c.ell;
         stage = c.stage;
         w16 = c.w16;
+        shape = c.shape;
         if (c.S > 1) Gs = c.G;
         else G = std::max(c.G, 2);
     };
-    apply(cA[0], E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA, E.stageA, E.w16A);
-    apply(cAt[0], E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt, E.stageAt, E.w16At);
+    apply(cA[0], E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA, E.stageA, E.w16A, E.shA);
+    apply(cAt[0], E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt, E.stageAt, E.w16At, E.shAt);
     E.K_tuned = 0;
     if (E.nnz < 200000 || (cA.size() == 1 && cAt.size() == 1))
         return VJ_OK;  // launch-bound sizes: the first candidate is as good as timing
     for (const TuneDecision& d : E.tuned)
         if (d.prec == E.prec && d.lanes_A == prm.lanes_A && d.lanes_At == prm.lanes_At &&
             d.col_splits_A == prm.col_splits_A && d.col_splits_At == prm.col_splits_At &&
-            d.check_frequency == prm.check_frequency) {
-            apply(d.a, E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA, E.stageA, E.w16A);
-            apply(d.at, E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt, E.stageAt, E.w16At);
+            d.check_frequency == prm.check_frequency && d.stage_threads == prm.stage_threads) {
+            apply(d.a, E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA, E.stageA, E.w16A, E.shA);
+            apply(d.at, E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt, E.stageAt, E.w16At, E.shAt);
             E.GA = d.GA;
             E.GAt = d.GAt;
             E.K_tuned = d.K_tuned;
@@ -1347,27 +1389,28 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     {   // one untimed pass first: the first candidate is otherwise timed on colder clocks and
         // caches than the rest, which biases the comparison against it
         double up = 0, ud = 0;
-        apply(cA[0], E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA, E.stageA, E.w16A);
-        apply(cAt[0], E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt, E.stageAt, E.w16At);
+        apply(cA[0], E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA, E.stageA, E.w16A, E.shA);
+        apply(cAt[0], E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt, E.stageAt, E.w16At, E.shAt);
         VJ_TRY((time_kernels<V, T>(ctx, E, av, tv, kTuneReps, &up, &ud)));
     }
     for (size_t r = 0; r < rounds; ++r) {
         const StepCfg& a = cA[std::min(r, cA.size() - 1)];
         const StepCfg& b = cAt[std::min(r, cAt.size() - 1)];
-        apply(a, E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA, E.stageA, E.w16A);
-        apply(b, E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt, E.stageAt, E.w16At);
+        apply(a, E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA, E.stageA, E.w16A, E.shA);
+        apply(b, E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt, E.stageAt, E.w16At, E.shAt);
         double up = 0, ud = 0;
         VJ_TRY((time_kernels<V, T>(ctx, E, av, tv, kTuneReps, &up, &ud)));
         if (prm.verbose >= 2)
             std::fprintf(stderr, "[autotune] A: S=%d G=%d stream=%d rec=%d ell=%d stage=%d w16=%d %.1f us | A^T: S=%d "
                          "G=%d stream=%d rec=%d ell=%d stage=%d w16=%d %.1f us\n",
-                         a.S, a.G, a.stream ? 1 : 0, a.rec ? 1 : 0, a.ell ? 1 : 0, a.stage ? 1 : 0, a.w16 ? 1 : 0, ud,
-                         b.S, b.G, b.stream ? 1 : 0, b.rec ? 1 : 0, b.ell ? 1 : 0, b.stage ? 1 : 0, b.w16 ? 1 : 0, up);
+                         a.S, a.G, a.stream ? 1 : 0, a.rec ? 1 : 0, a.ell ? 1 : 0,
+                         a.stage ? kStageShapeThreads[a.shape] : 0, a.w16 ? 1 : 0, ud, b.S, b.G, b.stream ? 1 : 0,
+                         b.rec ? 1 : 0, b.ell ? 1 : 0, b.stage ? kStageShapeThreads[b.shape] : 0, b.w16 ? 1 : 0, up);
         if (r == 0 || (r < cA.size() && ud < (1.0 - kTuneMargin) * bestA)) { bestA = ud; wA = a; }
         if (r == 0 || (r < cAt.size() && up < (1.0 - kTuneMargin) * bestAt)) { bestAt = up; wAt = b; }
     }
-    apply(wA, E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA, E.stageA, E.w16A);
-    apply(wAt, E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt, E.stageAt, E.w16At);
+    apply(wA, E.SA, E.GA, E.GsA, E.streamA, E.recA, E.ellA, E.stageA, E.w16A, E.shA);
+    apply(wAt, E.SAt, E.GAt, E.GsAt, E.streamAt, E.recAt, E.ellAt, E.stageAt, E.w16At, E.shAt);
     // The FP64 check passes run lane-group kernels with E.GA / E.GAt lanes per row. A split
     // winner leaves them at the last candidate timed: give them the width that suits the rows.
     if (E.SA > 1 || E.ellA) E.GA = std::max(choose_lanes(avgA), 2);
@@ -1402,7 +1445,7 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
         E.K_tuned = K;
     }
     E.tuned.push_back(TuneDecision{E.prec, prm.lanes_A, prm.lanes_At, prm.col_splits_A, prm.col_splits_At,
-                                   prm.check_frequency, wA, wAt, E.GA, E.GAt, E.K_tuned});
+                                   prm.check_frequency, prm.stage_threads, wA, wAt, E.GA, E.GAt, E.K_tuned});
     return VJ_OK;
 }
 
@@ -1449,9 +1492,9 @@ int prune_launch_lists(vj_ctx* ctx, FastEngine& E, const InstState* polled, cons
     };
     const bool sa = E.SA == 1 && E.streamA && !E.ellA && !E.stageA;   // tile-streamed steps of the batch
     const bool sat = E.SAt == 1 && E.streamAt && !E.ellAt && !E.stageAt;
-    const bool a = (E.stageA && live(E.hInstGA) * 2 <= E.nlistA) ||
+    const bool a = (E.stageA && live(E.LA().hInst) * 2 <= E.LA().nlist) ||
                    (sa && live_ranges(E.hFirstTA, E.hLastTA) * 2 <= E.nlistTA);
-    const bool at = (E.stageAt && live(E.hInstGAt) * 2 <= E.nlistAt) ||
+    const bool at = (E.stageAt && live(E.LAt().hInst) * 2 <= E.LAt().nlist) ||
                     (sat && live_ranges(E.hFirstTAt, E.hLastTAt) * 2 <= E.nlistTAt);
     if (!a && !at) return VJ_OK;
     VJ_HIP(hipStreamSynchronize(ctx->stream));   // `latest` now holds the state after the last block
@@ -1478,7 +1521,7 @@ int run_typed(vj_ctx* ctx, FastEngine& E, vj_result* out, clk::time_point t_star
     VJ_TRY(setup_ell(ctx, E));
     VJ_HIP(hipStreamSynchronize(s));
     const double layout_time = secs(t_layout, clk::now());
-    if (E.gcapA || E.gcapAt || E.multi) VJ_TRY(set_launch_lists(ctx, E, nullptr));   // every tile
+    if (E.stA[0].cap || E.stAt[0].cap || E.multi) VJ_TRY(set_launch_lists(ctx, E, nullptr));   // every tile
     VJ_TRY(launch_restart<T>(ctx, E, 1));  // initial centring
     VJ_TRY((autotune<V, T>(ctx, E, av, tv)));
     // after autotuning the scratch runs left the real iterates untouched; recentre anyway
@@ -1827,13 +1870,15 @@ int fast_benchmark(vj_ctx* ctx, int reps, vj_kernel_stat* out, int cap, int* wri
     if (E.SAt > 1 && E.GsAt == 0) std::snprintf(names[0], 48, "split_stream_partials+primal_finish<%s%s>", pt, rt);
     else if (E.SAt > 1) std::snprintf(names[0], 48, "split_partials+primal_finish<%s,G=%d>", pt, E.GsAt);
     else if (E.ellAt) std::snprintf(names[0], 48, "ell_primal_step<%s%s>", pt, rt);
-    else if (E.stageAt) std::snprintf(names[0], 48, "staged_primal_step<%s%s>", pt, E.w16At ? ",w16" : rt);
+    else if (E.stageAt)
+        std::snprintf(names[0], 48, "staged_primal_step<%s%s,%d>", pt, E.w16At ? ",w16" : rt, E.LAt().threads);
     else if (E.streamAt) std::snprintf(names[0], 48, "stream_primal_step<%s%s>", pt, rt);
     else std::snprintf(names[0], 48, "primal_step<%s,G=%d>", pt, E.GAt);
     if (E.SA > 1 && E.GsA == 0) std::snprintf(names[1], 48, "split_stream_partials+dual_finish<%s%s>", pt, ra);
     else if (E.SA > 1) std::snprintf(names[1], 48, "split_partials+dual_finish<%s,G=%d>", pt, E.GsA);
     else if (E.ellA) std::snprintf(names[1], 48, "ell_dual_step<%s%s>", pt, ra);
-    else if (E.stageA) std::snprintf(names[1], 48, "staged_dual_step<%s%s>", pt, E.w16A ? ",w16" : ra);
+    else if (E.stageA)
+        std::snprintf(names[1], 48, "staged_dual_step<%s%s,%d>", pt, E.w16A ? ",w16" : ra, E.LA().threads);
     else if (E.streamA) std::snprintf(names[1], 48, "stream_dual_step<%s%s>", pt, ra);
     else std::snprintf(names[1], 48, "dual_step<%s,G=%d>", pt, E.GA);
     const double us[2] = {up, ud};

@@ -653,55 +653,57 @@ __global__ __launch_bounds__(kBlock) void stream_dual_step(
 // ---------------------------------------------------------------------------------
 // LDS-staged fused steps, for batches of small LPs. A batched tile-streamed step pays one L2
 // line per nonzero for its gathers, and that line traffic, not HBM, sets its speed. Here every
-// tile holds rows of a single instance (at most kStageThreads rows and stage_cap<T>() entries,
+// tile holds rows of a single instance (at most RPL * NT rows and stage_cap<T, NT>() entries,
 // built on the device) and carries the window [lo, hi) of columns its entries touch. The
 // workgroup copies the window of the gathered vector into LDS with coalesced loads (chunk by
 // chunk when it is wider than the buffer) and each entry then reads its vector element from
 // LDS. A lane keeps its stage_depth<T>() entries in registers across the chunks; every entry
 // is multiplied in exactly one chunk. The products then go to the same buffer by entry
 // position and one lane per row adds its segment in index order: the row sums are those of
-// the tile-streamed kernels, bit for bit. 64 KB of LDS per workgroup leaves room for two
-// workgroups (32 waves) per CU, which the kernels keep by asking for 8 waves per SIMD (at most
-// 64 VGPRs: the first build, with predicated entry loads, spilled to scratch and ran 9x slower).
+// the tile-streamed kernels, bit for bit. Workgroups of NT = 1024 or 512 threads (the
+// autotuner times both: the 512-thread shape keeps four workgroups per CU, which paid on the
+// staircase batch, whose windows are narrow; the 1024-thread shape's larger tiles paid on the
+// random batch) hold NT * 64 bytes of LDS, 64 KB or 32 KB, which leaves room for 32 waves per
+// CU; the kernels keep them by asking for 8 waves per SIMD (at most 64 VGPRs: the first build,
+// with predicated entry loads, spilled to scratch and ran 9x slower). A lane holds 64 bytes of
+// entries (16 FP32 or 8 FP64) in either shape.
 // Workgroup b runs tile list[b]: the host launches the tiles of the instances still active
 // (the list shrinks as instances finish; each workgroup still checks its instance's state),
-// because with 64 KB of LDS per workgroup only 512 are resident and workgroups of finished
-// instances, each exiting after two loads, cost about 64 us per iteration in the tail of a
-// 512-LP batch.
-constexpr int kStageThreads = 1024;
-constexpr int kStageBytes = 65536;
+// because only 512 (1024) workgroups are resident and workgroups of finished instances, each
+// exiting after two loads, cost about 64 us per iteration in the tail of a 512-LP batch.
+constexpr int kStageLaneBytes = 64;
+template <class T, int NT>
+constexpr int stage_cap() { return NT * kStageLaneBytes / static_cast<int>(sizeof(T)); }
 template <class T>
-constexpr int stage_cap() { return kStageBytes / static_cast<int>(sizeof(T)); }
-template <class T>
-constexpr int stage_depth() { return stage_cap<T>() / kStageThreads; }
+constexpr int stage_depth() { return kStageLaneBytes / static_cast<int>(sizeof(T)); }
 
 // lds[q] = vec[base + q] for q < len, with direct global-to-LDS loads (global_load_lds_dword):
-// wave w copies 4-byte words q0 .. q0+63 (q0 = 64 w, 64 w + 1024, ...), lane l word q0 + l, into
+// wave w copies 4-byte words q0 .. q0+63 (q0 = 64 w, 64 w + NT, ...), lane l word q0 + l, into
 // LDS at the wave-uniform base lds + q0. The loads occupy no VGPRs, so every word of a lane is
 // in flight at once (a copy through registers, four loads at a time, waited for each group);
 // the caller's barrier waits for them (vmcnt). Lanes past the end re-read the last word and
 // write it to LDS words past len, which no entry reads (the buffer holds whole waves' words).
-template <class T>
+template <class T, int NT>
 __device__ __forceinline__ void stage_window(const T* __restrict__ vec, int base, int len, T* __restrict__ lds) {
     constexpr int W = static_cast<int>(sizeof(T) / 4);
     const uint32_t* src = reinterpret_cast<const uint32_t*>(vec) + static_cast<int64_t>(base) * W;
     uint32_t* dst = reinterpret_cast<uint32_t*>(lds);
     const int words = len * W;
     const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
-    for (int q0 = static_cast<int>(threadIdx.x) - lane; q0 < words; q0 += kStageThreads) {
+    for (int q0 = static_cast<int>(threadIdx.x) - lane; q0 < words; q0 += NT) {
         const int q = q0 + lane < words ? q0 + lane : words - 1;
         __builtin_amdgcn_global_load_lds(src + q, dst + q0, 4, 0, 0);
     }
 }
 
-// Products of the tile's entries [s0, s1) (at most stage_cap<T>()) with vec, whose elements
+// Products of the tile's entries [s0, s1) (at most stage_cap<T, NT>()) with vec, whose elements
 // the entries read in the window [wlo, whi); on return lds[p - s0] = a_p * vec[col_p]. Entry
 // columns are absolute, or relative to wlo for the 16-bit offset format.
-template <class Ent, class T>
+template <class Ent, class T, int NT>
 __device__ __forceinline__ void staged_products(int s0, int s1, int wlo, int whi, const Ent& ent,
                                                 const T* __restrict__ vec, T* __restrict__ lds) {
     constexpr int U = stage_depth<T>();
-    constexpr int CAP = stage_cap<T>();
+    constexpr int CAP = stage_cap<T, NT>();
     if (s1 <= s0) return;   // no entries (uniform over the workgroup)
     int c[U];
     T a[U];
@@ -710,7 +712,7 @@ __device__ __forceinline__ void staged_products(int s0, int s1, int wlo, int whi
     // base is out of range: unconditional loads keep the slots in plain registers
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const int p = s0 + static_cast<int>(threadIdx.x) + u * kStageThreads;
+        const int p = s0 + static_cast<int>(threadIdx.x) + u * NT;
         int cc;
         T aa;
         ent.load(p < s1 ? p : s1 - 1, cc, aa);
@@ -720,7 +722,7 @@ __device__ __forceinline__ void staged_products(int s0, int s1, int wlo, int whi
     for (int base = wlo; base < whi; base += CAP) {
         const int len = whi - base < CAP ? whi - base : CAP;
         if (base != wlo) __syncthreads();   // the previous chunk has been read
-        stage_window<T>(vec, base, len, lds);
+        stage_window<T, NT>(vec, base, len, lds);
         __syncthreads();
         const int cb = window_relative<Ent>::value ? base - wlo : base;
 #pragma unroll
@@ -732,18 +734,18 @@ __device__ __forceinline__ void staged_products(int s0, int s1, int wlo, int whi
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const int p = s0 + static_cast<int>(threadIdx.x) + u * kStageThreads;
+        const int p = s0 + static_cast<int>(threadIdx.x) + u * NT;
         if (p < s1) lds[p - s0] = a[u];
     }
     __syncthreads();
 }
 
 // Whole-workgroup sum of one row longer than the buffer (gathers from memory; fixed order).
-template <class Ent, class T>
+template <class Ent, class T, int NT>
 __device__ __forceinline__ T staged_long_row(int s0, int s1, const Ent& ent, const T* __restrict__ vec,
                                              T* __restrict__ lds) {
     T acc = 0;
-    for (int p = s0 + static_cast<int>(threadIdx.x); p < s1; p += kStageThreads) {
+    for (int p = s0 + static_cast<int>(threadIdx.x); p < s1; p += NT) {
         int c;
         T a;
         ent.load(p, c, a);
@@ -751,7 +753,7 @@ __device__ __forceinline__ T staged_long_row(int s0, int s1, const Ent& ent, con
     }
     lds[threadIdx.x] = acc;
     __syncthreads();
-    for (int w = kStageThreads / 2; w > 0; w >>= 1) {
+    for (int w = NT / 2; w > 0; w >>= 1) {
         if (static_cast<int>(threadIdx.x) < w) lds[threadIdx.x] += lds[threadIdx.x + w];
         __syncthreads();
     }
@@ -761,6 +763,7 @@ __device__ __forceinline__ T staged_long_row(int s0, int s1, const Ent& ent, con
 }
 
 // The tile's residual partials (one instance per tile) to part[slot * 3 + q].
+template <int NT>
 __device__ __forceinline__ void stage_partials(const double (&v)[3], int slot, double* __restrict__ part, int mask,
                                                double (*red)[3]) {
     double t[3];
@@ -776,31 +779,31 @@ __device__ __forceinline__ void stage_partials(const double (&v)[3], int slot, d
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             double s = red[0][q];
-            for (int w = 1; w < kStageThreads / kWave; ++w) s += red[w][q];
+            for (int w = 1; w < NT / kWave; ++w) s += red[w][q];
             if ((mask >> q) & 1) part[static_cast<int64_t>(slot) * 3 + q] = s;
         }
     }
 }
 
-// RPL rows per lane: tiles of up to RPL * kStageThreads rows. Matrices with short rows (the
-// columns of a random batch instance have 6 entries) take RPL = 2, so that a tile still holds
-// enough entries to pay for copying its window.
-template <class Ent, class T, bool MULTI, int RPL>
-__global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8))) void staged_primal_step(
+// RPL rows per lane: tiles of up to RPL * NT rows. Matrices with short rows (the columns of a
+// random batch instance have 6 entries) take RPL = 2, so that a tile still holds enough
+// entries to pay for copying its window.
+template <class Ent, class T, bool MULTI, int RPL, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void staged_primal_step(
     const int* __restrict__ list, const int4* __restrict__ desc, const int* __restrict__ tp, Ent ent, const T* __restrict__ y, const T* __restrict__ dhat,
     const T* __restrict__ lcs, const T* __restrict__ ucs, T* __restrict__ x, T* __restrict__ xbar, const T* __restrict__ ax,
     const double* __restrict__ xhat, double* __restrict__ xfull, const InstState* __restrict__ st,
     double* __restrict__ part, int it, int flags, double rho) {
-    __shared__ T lds[stage_cap<T>()];
-    __shared__ double red[kStageThreads / kWave][3];
+    __shared__ T lds[stage_cap<T, NT>()];
+    __shared__ double red[NT / kWave][3];
     const int tile = list[blockIdx.x];
     const int4 d0 = desc[2 * tile], d1 = desc[2 * tile + 1];   // rows, entries | window, instance
     const int my = MULTI ? d1.z : 0;   // every row of the tile belongs to this instance
     if (st[my].active == 0) return;
     const int r0 = d0.x, r1 = d0.y, s0 = d0.z, s1 = d0.w;
     double c[3] = {0.0, 0.0, 0.0};
-    if (s1 - s0 > stage_cap<T>()) {   // one long row
-        const T acc = staged_long_row<Ent, T>(s0, s1, ent, y, lds);
+    if (s1 - s0 > stage_cap<T, NT>()) {   // one long row
+        const T acc = staged_long_row<Ent, T, NT>(s0, s1, ent, y, lds);
         if (threadIdx.x == 0) primal_epilogue<T>(acc, r0, st[my], it, flags, rho, dhat, lcs, ucs, x, xbar, ax, xhat, xfull, c);
     } else {
         // the rows' operands and bounds are loaded before the staging (a lane past the tile's end
@@ -809,16 +812,16 @@ __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8
         int q0[RPL], q1[RPL];
 #pragma unroll
         for (int u = 0; u < RPL; ++u) {
-            const int j = r0 + static_cast<int>(threadIdx.x) + u * kStageThreads;
+            const int j = r0 + static_cast<int>(threadIdx.x) + u * NT;
             const int jc = j < r1 ? j : r1 - 1;
             o[u] = primal_ops<T>(jc, x, xbar, ax, dhat, lcs, ucs, st[my], it, rho);
             q0[u] = tp[jc] - s0;
             q1[u] = tp[jc + 1] - s0;
         }
-        staged_products<Ent, T>(s0, s1, d1.x, d1.y, ent, y, lds);
+        staged_products<Ent, T, NT>(s0, s1, d1.x, d1.y, ent, y, lds);
 #pragma unroll
         for (int u = 0; u < RPL; ++u) {
-            const int j = r0 + static_cast<int>(threadIdx.x) + u * kStageThreads;
+            const int j = r0 + static_cast<int>(threadIdx.x) + u * NT;
             if (j < r1) {
                 T acc = 0;
                 for (int q = q0[u]; q < q1[u]; ++q) acc += lds[q];
@@ -828,42 +831,42 @@ __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8
     }
     if (flags) {
         const int mask = ((flags & 1) ? 1 : 0) | ((flags & 2) ? 6 : 0);
-        stage_partials(c, tile + my, part, mask, red);
+        stage_partials<NT>(c, tile + my, part, mask, red);
     }
 }
 
-template <class Ent, class T, bool MULTI, int RPL>
-__global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8))) void staged_dual_step(
+template <class Ent, class T, bool MULTI, int RPL, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void staged_dual_step(
     const int* __restrict__ list, const int4* __restrict__ desc, const int* __restrict__ rp, Ent ent, const T* __restrict__ xbar, const T* __restrict__ yhat_t,
     const T* __restrict__ lrs, const T* __restrict__ urs, T* __restrict__ y, const T* __restrict__ ay, const double* __restrict__ yhat,
     double* __restrict__ yfull, const InstState* __restrict__ st, double* __restrict__ part, int it, int flags,
     double rho) {
-    __shared__ T lds[stage_cap<T>()];
-    __shared__ double red[kStageThreads / kWave][3];
+    __shared__ T lds[stage_cap<T, NT>()];
+    __shared__ double red[NT / kWave][3];
     const int tile = list[blockIdx.x];
     const int4 d0 = desc[2 * tile], d1 = desc[2 * tile + 1];
     const int my = MULTI ? d1.z : 0;
     if (st[my].active == 0) return;
     const int r0 = d0.x, r1 = d0.y, s0 = d0.z, s1 = d0.w;
     double c[3] = {0.0, 0.0, 0.0};
-    if (s1 - s0 > stage_cap<T>()) {
-        const T w = staged_long_row<Ent, T>(s0, s1, ent, xbar, lds);
+    if (s1 - s0 > stage_cap<T, NT>()) {
+        const T w = staged_long_row<Ent, T, NT>(s0, s1, ent, xbar, lds);
         if (threadIdx.x == 0) dual_epilogue<T>(w, r0, st[my], it, flags, rho, yhat_t, lrs, urs, y, ay, yhat, yfull, c);
     } else {
         DualOps<T> o[RPL];
         int q0[RPL], q1[RPL];
 #pragma unroll
         for (int u = 0; u < RPL; ++u) {
-            const int i = r0 + static_cast<int>(threadIdx.x) + u * kStageThreads;
+            const int i = r0 + static_cast<int>(threadIdx.x) + u * NT;
             const int ic = i < r1 ? i : r1 - 1;
             o[u] = dual_ops<T>(ic, y, yhat_t, lrs, urs);
             q0[u] = rp[ic] - s0;
             q1[u] = rp[ic + 1] - s0;
         }
-        staged_products<Ent, T>(s0, s1, d1.x, d1.y, ent, xbar, lds);
+        staged_products<Ent, T, NT>(s0, s1, d1.x, d1.y, ent, xbar, lds);
 #pragma unroll
         for (int u = 0; u < RPL; ++u) {
-            const int i = r0 + static_cast<int>(threadIdx.x) + u * kStageThreads;
+            const int i = r0 + static_cast<int>(threadIdx.x) + u * NT;
             if (i < r1) {
                 T w = 0;
                 for (int q = q0[u]; q < q1[u]; ++q) w += lds[q];
@@ -873,7 +876,7 @@ __global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8
     }
     if (flags) {
         const int mask = ((flags & 1) ? 1 : 0) | ((flags & 2) ? 6 : 0);
-        stage_partials(c, tile + my, part, mask, red);
+        stage_partials<NT>(c, tile + my, part, mask, red);
     }
 }
 

@@ -52,7 +52,7 @@ class Params(ctypes.Structure):
                 ("col_splits_A", _c_int), ("col_splits_At", _c_int),
                 ("restart_sufficient", _c_dbl), ("restart_necessary", _c_dbl), ("restart_artificial", _c_dbl),
                 ("primal_weight_smoothing", _c_dbl), ("power_iterations", _c_int), ("restart_drift", _c_dbl),
-                ("primal_weight_integral", _c_dbl), ("refine_every_check", _c_int)]
+                ("primal_weight_integral", _c_dbl), ("refine_every_check", _c_int), ("stage_threads", _c_int)]
 
 
 class Result(ctypes.Structure):
