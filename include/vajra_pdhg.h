@@ -154,8 +154,8 @@ typedef struct vj_params {
                                       restarts only; 0 = default (every check with BF16 and FP16
                                       values, restarts only with FP32 and FP64) */
     int stage_threads;             /* RESTARTED: workgroup size of the LDS-staged kernels (lanes -5 /
-                                      -6 / -7): 0 = the autotuner times 1,024 and 512 threads (a
-                                      forced staged kernel runs 1,024), 512 or 1024 = that size */
+                                      -6 / -7): 0 = the autotuner times 1,024 and 256 threads (a
+                                      forced staged kernel runs 1,024), 256 or 1024 = that size */
 } vj_params;
 /* Fill defaults for a mode (reference defaults for VJ_MODE_REFERENCE). */
 void vj_params_default(vj_params* p, int mode);

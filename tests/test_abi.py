@@ -42,7 +42,7 @@ def test_params_defaults_follow_reference():
     assert (q.restart_sufficient, q.restart_necessary, q.restart_artificial) == (0.2, 0.8, 0.5)
     assert q.primal_weight_smoothing == 0.5 and q.primal_weight_integral == 0.0
     assert q.refine_every_check == 0   # per precision: every check with 16-bit values, restarts otherwise
-    assert q.stage_threads == 0        # staged kernels: the autotuner times 1,024 and 512 threads
+    assert q.stage_threads == 0        # staged kernels: the autotuner times 1,024 and 256 threads
     assert q.power_iterations == 64 and q.step_scale == 0.95
     assert q.restart_drift == 0.0   # 0: the engine's default (2 with FP32 values, 4 with 16-bit values refined
                                     # at every check, off otherwise)

@@ -290,7 +290,7 @@ def test_structured_batch_against_highs(engine, prec, lanes):
         assert abs(got - opt) <= 1e-6 * (1 + abs(opt)), (got, opt)
 
 
-@pytest.mark.parametrize("threads", [1024, 512])
+@pytest.mark.parametrize("threads", [1024, 256])
 @pytest.mark.parametrize("prec,stream,staged", [("fp64", -1, -5), ("fp32", -2, -6), ("bf16", -1, -5),
                                                 ("fp64", -1, -7), ("fp32", -2, -7), ("bf16", -1, -7),
                                                 ("fp16", -1, -5), ("fp16", -1, -7)])
@@ -302,7 +302,7 @@ def test_staged_equals_tile_streamed(engine, prec, stream, staged, threads):
     7,000 x 17,500 instance copies its x_bar window in two chunks in FP32, three in FP64).
     Lanes -7 reads 16-bit column offsets from each tile's window start (with BF16 and FP16
     values, packed with the value into one 4-byte record). Both workgroup shapes (1,024 and
-    512 threads, with tiles and LDS windows of their own sizes) give the same bits."""
+    256 threads, with tiles and LDS windows of their own sizes) give the same bits."""
     lps = [vp.gen_random_lp(4100 + i, m, int(2.5 * m)) for i, m in enumerate([500, 7000, 1300])]
     lps += [lp_cases.structured_feasible_lp(*s) for s in STRUCTURED[:3]]
     engine.upload_batch(lps)
@@ -320,8 +320,8 @@ def test_staged_equals_tile_streamed(engine, prec, stream, staged, threads):
         assert np.array_equal(x0, x1) and np.array_equal(y0, y1)
 
 
-@pytest.mark.parametrize("prec,lanes,threads", [("fp32", -7, 1024), ("fp64", -5, 1024), ("fp16", -7, 512),
-                                                ("fp64", -5, 512)])
+@pytest.mark.parametrize("prec,lanes,threads", [("fp32", -7, 1024), ("fp64", -5, 1024), ("fp16", -7, 256),
+                                                ("fp64", -5, 256)])
 def test_staged_batch_graph_and_stream_identical(engine, prec, lanes, threads):
     """A batch whose instances stop at very different iteration counts, with the LDS-staged
     kernels: as instances finish, the host drops their tiles from the staged launches (and

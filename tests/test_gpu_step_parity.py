@@ -28,8 +28,8 @@ VARIANTS = {
     "split_A2_At4_groups": dict(lanes_A=2, lanes_At=4, col_splits_A=2, col_splits_At=4),
     "lds_staged": dict(lanes_A=-5, lanes_At=-5, col_splits_A=1, col_splits_At=1),
     "lds_staged_w16": dict(lanes_A=-7, lanes_At=-7, col_splits_A=1, col_splits_At=1),
-    "lds_staged_512": dict(lanes_A=-5, lanes_At=-5, col_splits_A=1, col_splits_At=1, stage_threads=512),
-    "lds_staged_w16_512": dict(lanes_A=-7, lanes_At=-7, col_splits_A=1, col_splits_At=1, stage_threads=512),
+    "lds_staged_256": dict(lanes_A=-5, lanes_At=-5, col_splits_A=1, col_splits_At=1, stage_threads=256),
+    "lds_staged_w16_256": dict(lanes_A=-7, lanes_At=-7, col_splits_A=1, col_splits_At=1, stage_threads=256),
 }
 
 
@@ -69,7 +69,7 @@ def test_single_lp_blocks_equal_model(engine, variant, blocks):
 
 
 @pytest.mark.parametrize("blocks", [1, 3])
-@pytest.mark.parametrize("lanes,threads", [(-1, 0), (8, 0), (-5, 1024), (-7, 1024), (-5, 512), (-7, 512)])
+@pytest.mark.parametrize("lanes,threads", [(-1, 0), (8, 0), (-5, 1024), (-7, 1024), (-5, 256), (-7, 256)])
 def test_batch_blocks_equal_model(engine, lanes, threads, blocks):
     """The batched kernels (every instance in one launch, per-instance state) on a batch of
     LPs of different shapes: each instance equals its own model run (the staged kernels in

@@ -268,7 +268,7 @@ int vj_params_set(vj_ctx* ctx, const vj_params* p) {
         !(p->primal_weight_integral >= 0.0 && p->primal_weight_integral < 1.0) || p->power_iterations < 1 ||
         p->power_iterations > 10000 || std::isnan(p->restart_drift) ||
         (p->restart_drift > 0.0 && p->restart_drift <= 1.0) || p->refine_every_check < -1 ||
-        p->refine_every_check > 1 || (p->stage_threads != 0 && p->stage_threads != 512 && p->stage_threads != 1024)) {
+        p->refine_every_check > 1 || (p->stage_threads != 0 && p->stage_threads != 256 && p->stage_threads != 1024)) {
         set_error("restart/primal-weight/power-iteration parameters out of range");
         return VJ_ERR_INVALID_ARGUMENT;
     }

@@ -217,7 +217,7 @@ struct StageLayout {
     bool w16ok = false;
 };
 // Threads per workgroup of the staged steps, per shape index.
-constexpr int kStageShapeThreads[2] = {1024, 512};
+constexpr int kStageShapeThreads[2] = {1024, 256};
 
 struct FastEngine {
     int prec = VJ_PREC_FP64;
@@ -272,7 +272,7 @@ struct FastEngine {
     std::vector<int> hFirstTA, hLastTA, hFirstTAt, hLastTAt;
     int nlistTA = 0, nlistTAt = 0;
     // LDS-staged kernels (batches), one StageLayout per step and workgroup shape (stA / stAt
-    // [0]: 1,024 threads, [1]: 512; shA / shAt: the shape the steps run, LA() / LAt() its
+    // [0]: 1,024 threads, [1]: 256; shA / shAt: the shape the steps run, LA() / LAt() its
     // layout); stageA / stageAt select the staged steps, w16A / w16At their 16-bit window
     // offsets. pruned: some list leaves tiles out.
     StageLayout stA[2], stAt[2];
@@ -374,7 +374,7 @@ int launch_primal(vj_ctx* ctx, FastEngine& E, const V* tv, T* x, T* xbar, const 
                                    E.dhat.as<T>(), E.lcs.as<T>(), E.ucs.as<T>(), x, xbar, ax, E.xhat.as<double>(),
                                    E.xfull.as<double>(), E.st.as<InstState>(), E.part_x.as<double>(), it, flags, E.rho);
                 };
-                if (E.LAt().threads == 512) go3(std::integral_constant<int, 512>{});
+                if (E.LAt().threads == 256) go3(std::integral_constant<int, 256>{});
                 else go3(std::integral_constant<int, 1024>{});
             };
             using One = std::integral_constant<int, 1>;
@@ -508,7 +508,7 @@ int launch_dual(vj_ctx* ctx, FastEngine& E, const V* av, const T* xbar, T* y, in
                                    yht, lrs, urs, y, ay, E.yhat.as<double>(), E.yfull.as<double>(), E.st.as<InstState>(),
                                    E.part_y.as<double>(), it, flags, E.rho);
                 };
-                if (E.LA().threads == 512) go3(std::integral_constant<int, 512>{});
+                if (E.LA().threads == 256) go3(std::integral_constant<int, 256>{});
                 else go3(std::integral_constant<int, 1024>{});
             };
             using One = std::integral_constant<int, 1>;
@@ -1286,8 +1286,8 @@ int autotune(vj_ctx* ctx, FastEngine& E, const V* av, const V* tv) {
     // ell: the sliced-ELL copy exists (single LP, little padding, or forced by lanes -3 / -4)
     // staged candidates per workgroup shape: stage[sh] (its tiles exist and qualify), w16[sh]
     // (its 16-bit offsets exist); stage_threads forces one shape (a forced staged kernel runs
-    // the 1,024-thread shape unless stage_threads asks for 512)
-    const int forced_shape = prm.stage_threads == 512 ? 1 : 0;
+    // the 1,024-thread shape unless stage_threads asks for 256)
+    const int forced_shape = prm.stage_threads == 256 ? 1 : 0;
     auto shape_ok = [&](int sh) { return prm.stage_threads == 0 || sh == forced_shape; };
     auto staged = [&](double avg, int sh, bool rec, bool w16) {
         StepCfg c{1, choose_lanes(avg), false, rec, false, true, w16};

@@ -660,16 +660,16 @@ __global__ __launch_bounds__(kBlock) void stream_dual_step(
 // LDS. A lane keeps its stage_depth<T>() entries in registers across the chunks; every entry
 // is multiplied in exactly one chunk. The products then go to the same buffer by entry
 // position and one lane per row adds its segment in index order: the row sums are those of
-// the tile-streamed kernels, bit for bit. Workgroups of NT = 1024 or 512 threads (the
-// autotuner times both: the 512-thread shape keeps four workgroups per CU, which paid on the
+// the tile-streamed kernels, bit for bit. Workgroups of NT = 1024 or 256 threads (the
+// autotuner times both: the 256-thread shape keeps eight workgroups per CU, which paid on the
 // staircase batch, whose windows are narrow; the 1024-thread shape's larger tiles paid on the
-// random batch) hold NT * 64 bytes of LDS, 64 KB or 32 KB, which leaves room for 32 waves per
+// random batch) hold NT * 64 bytes of LDS, 64 KB or 16 KB, which leaves room for 32 waves per
 // CU; the kernels keep them by asking for 8 waves per SIMD (at most 64 VGPRs: the first build,
 // with predicated entry loads, spilled to scratch and ran 9x slower). A lane holds 64 bytes of
 // entries (16 FP32 or 8 FP64) in either shape.
 // Workgroup b runs tile list[b]: the host launches the tiles of the instances still active
 // (the list shrinks as instances finish; each workgroup still checks its instance's state),
-// because only 512 (1024) workgroups are resident and workgroups of finished instances, each
+// because only 512 (2,048) workgroups are resident and workgroups of finished instances, each
 // exiting after two loads, cost about 64 us per iteration in the tail of a 512-LP batch.
 constexpr int kStageLaneBytes = 64;
 template <class T, int NT>
