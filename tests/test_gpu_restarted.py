@@ -88,6 +88,11 @@ def test_restart_drift_parameter_range(engine):
             engine.set_params(vp.default_params(vp.MODE_RESTARTED, refine_every_check=bad))
     for ok in (-1, 0, 1):
         engine.set_params(vp.default_params(vp.MODE_RESTARTED, refine_every_check=ok))
+    for bad in (512, 7, -1):
+        with pytest.raises(vp.VjError):
+            engine.set_params(vp.default_params(vp.MODE_RESTARTED, stage_threads=bad))
+    for ok in (0, 256, 1024):
+        engine.set_params(vp.default_params(vp.MODE_RESTARTED, stage_threads=ok))
 
 
 def test_cfg0_objective_against_highs(engine):
