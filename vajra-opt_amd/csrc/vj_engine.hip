@@ -677,9 +677,10 @@ int tile_instance_ranges(hipStream_t s, int ntiles, const DevBuf& tile_row, cons
     return VJ_OK;
 }
 
-// Rows per lane of a staged tile: 2 where 1,024 rows of the average length fill at most three
-// quarters of the tile's entry capacity (short rows: the columns of a random instance, 6
-// entries, or of a staircase instance, about 8), else 1. Tiles still stop at the capacity.
+// Rows per lane of a staged tile: 2 where one row per thread of the average length fills at
+// most three quarters of the tile's entry capacity (short rows: the columns of a random
+// instance, 6 entries, or of a staircase instance, about 8), else 1. Tiles still stop at the
+// capacity.
 int stage_rows_per_lane(double avg_row, int cap, int threads) { return threads * avg_row <= 0.75 * cap ? 2 : 1; }
 
 // The staged tiles of one step (rows rp / indices ci, `rows` rows) for one workgroup shape,
