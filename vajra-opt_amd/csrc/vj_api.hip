@@ -269,7 +269,7 @@ int vj_params_set(vj_ctx* ctx, const vj_params* p) {
         p->power_iterations > 10000 || std::isnan(p->restart_drift) ||
         (p->restart_drift > 0.0 && p->restart_drift <= 1.0) || p->refine_every_check < -1 ||
         p->refine_every_check > 1 || (p->stage_threads != 0 && p->stage_threads != 256 && p->stage_threads != 1024)) {
-        set_error("restart/primal-weight/power-iteration parameters out of range");
+        set_error("restart, primal-weight, power-iteration, refine_every_check or stage_threads parameter out of range");
         return VJ_ERR_INVALID_ARGUMENT;
     }
     if (!lanes_ok(p->lanes_A) || !lanes_ok(p->lanes_At)) {
