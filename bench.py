@@ -71,6 +71,8 @@ def main():
     ap.add_argument("--budget", type=float, default=400.0,
                     help="seconds after which no further leg (CPU baseline, batches) is started")
     ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--batches", default="random,staircase",
+                    help="batch legs to run, comma-separated (random: 512 uneven random LPs, staircase: 128 staircase LPs)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--batch-count", type=int, default=CFG3["count"])
@@ -240,7 +242,9 @@ def main():
 
     # ---- batched LPs/s ---------------------------------------------------------------------
     if not args.skip_batch:
-        for kind in ("random", "staircase"):
+        for kind in [k for k in args.batches.split(",") if k]:
+            if kind not in ("random", "staircase"):
+                raise SystemExit(f"--batches: unknown batch {kind}")
             if budget_left(f"batch {kind}"):
                 extra[f"batch_{kind}"] = bench_batch(vp, kind, args, budget_left)
 
