@@ -314,14 +314,20 @@ def bench_batch(vp, kind, args, budget_left):
         t0 = time.perf_counter()
         res = eng.solve_batch(p)
         t_batch = time.perf_counter() - t0
+        # the steps the autotuner chose, timed with every instance active (HIP events, after the solve)
+        kern = eng.benchmark_kernels(reps=20)
         ok = sum(1 for r in res if r.status == 1)
         worst = max(r.rel_kkt for r in res)
         log(f"batch {kind} {prec}: {ok}/{cnt} optimal in {t_batch:.2f}s (max iters {max(r.iterations for r in res)}), "
             f"worst rel_kkt {worst:.2e}, check interval {res[0].check_interval}, "
             f"lanes {res[0].lanes_A},{res[0].lanes_At}")
+        log(f"batch {kind} {prec} steps: " + ", ".join(f"{k['name']} {k['avg_us']:.1f} us "
+                                                      f"({k['bytes_per_launch'] / (k['avg_us'] * 1e3):.0f} GB/s)"
+                                                      for k in kern))
         return dict(optimal=ok, worst_rel_kkt=worst, check_interval=res[0].check_interval,
                     lanes=[res[0].lanes_A, res[0].lanes_At], max_iterations=max(r.iterations for r in res),
-                    batch_s=t_batch, batch_lps_per_s=cnt / t_batch, precision=prec)
+                    batch_s=t_batch, batch_lps_per_s=cnt / t_batch, precision=prec,
+                    kernels=[dict(k, gbs=round(k["bytes_per_launch"] / (k["avg_us"] * 1e-6) / 1e9, 1)) for k in kern])
 
     out = dict(count=cnt, sum_nnz=nnz, **run(args.precision))
     extras = [p for p in args.batch_extra.split(",") if p and p not in ("none", args.precision)]
