@@ -559,3 +559,23 @@ def test_refine_every_check_failure_statuses(engine):
     print("[case] refine every check: NaN cost fp32", flush=True)
     r, *_ = solve(engine, lp, max_iterations=5000, precision="fp32", refine_every_check=1)
     assert r.status == 9, r.as_dict()
+
+
+@pytest.mark.parametrize("threads", [1024, 256])
+def test_batch_benchmark_kernels_names_the_staged_shape(engine, threads):
+    """vj_benchmark_kernels after a batch solve times the staged steps the solve ran and names
+    them with their workgroup size; the timing leaves the solved state in place (a second
+    download returns the same point)."""
+    lps = [vp.gen_random_lp(4300 + i, m, int(2.5 * m)) for i, m in enumerate([400, 900])]
+    engine.upload_batch(lps)
+    print(f"[case] batch benchmark_kernels staged threads={threads}", flush=True)
+    engine.solve_batch(vp.default_params(vp.MODE_RESTARTED, precision="fp32", tolerance=1e-6, max_iterations=20000,
+                                         lanes_A=-7, lanes_At=-7, stage_threads=threads))
+    xs0, ys0 = engine.download_batch()
+    ks = engine.benchmark_kernels(reps=3)
+    names = [k["name"] for k in ks]
+    assert len(ks) == 2 and all(n.startswith("staged_") and n.endswith(f",{threads}>") for n in names), names
+    assert all(k["avg_us"] > 0 and k["bytes_per_launch"] > 0 for k in ks), ks
+    xs1, ys1 = engine.download_batch()
+    for a, b in zip(xs0 + ys0, xs1 + ys1):
+        assert np.array_equal(a, b)
