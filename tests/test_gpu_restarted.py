@@ -579,3 +579,20 @@ def test_batch_benchmark_kernels_names_the_staged_shape(engine, threads):
     xs1, ys1 = engine.download_batch()
     for a, b in zip(xs0 + ys0, xs1 + ys1):
         assert np.array_equal(a, b)
+
+
+def test_bf16_values_to_1e8_at_100k(engine):
+    """BF16 matrix values (FP32 iterates, FP64 refinement at every check, drift restarts of
+    factor 4: the defaults for 16-bit values) to rel-KKT 1e-8 on a 100,000 x 200,000 LP of the
+    benchmark family (10% of the columns scaled by 10^U[-3,3]): oracle-checked, and the
+    objective within 1e-7 of the FP64 engine's at the same tolerance."""
+    lp = vp.gen_random_lp(0x5EED2026 + 11, 100000, 200000, 6, 0.1, 3.0)
+    print("[case] bf16 values to 1e-8, 100000 x 200000", flush=True)
+    r, x, _, y = solve(engine, lp, tol=1e-8, max_iterations=3000000, precision="bf16", time_limit=150.0)
+    print(f"[case] bf16: iterations {r.iterations}, restarts {r.restart_count}, K {r.check_interval}", flush=True)
+    assert r.status == 1, r.as_dict()
+    check_point(lp, r, x, y, 1e-8)
+    r64, x64, _, _ = solve(engine, lp, tol=1e-8, max_iterations=3000000, precision="fp64", time_limit=150.0)
+    assert r64.status == 1, r64.as_dict()
+    o16, o64 = float(np.dot(lp.c, x)) + lp.offset, float(np.dot(lp.c, x64)) + lp.offset
+    assert abs(o16 - o64) <= 1e-7 * (1 + abs(o64)), (o16, o64)
