@@ -596,3 +596,59 @@ def test_bf16_values_to_1e8_at_100k(engine):
     assert r64.status == 1, r64.as_dict()
     o16, o64 = float(np.dot(lp.c, x)) + lp.offset, float(np.dot(lp.c, x64)) + lp.offset
     assert abs(o16 - o64) <= 1e-7 * (1 + abs(o64)), (o16, o64)
+
+
+def _long_row_lp(seed, m=30, n=5000):
+    """A feasible, bounded LP whose row 0 holds every column (5,000 entries: longer than a
+    256-thread staged tile, 4,096 FP32 / 2,048 FP64 entries, and than a tile-streamed tile,
+    4,096), the other rows about 500 entries each."""
+    rng = np.random.default_rng(seed)
+    A = np.zeros((m, n))
+    for j in range(n):
+        rows = rng.choice(np.arange(1, m), 3, replace=False)
+        A[rows, j] = rng.standard_normal(3)
+    A[0, :] = rng.standard_normal(n)
+    b = A @ rng.uniform(0.0, 1.0, n)
+    c = A.T @ rng.standard_normal(m) + rng.uniform(0.0, 1.0, n)
+    return vp.LP.from_dense(A, c, np.zeros(n), np.full(n, 10.0), b, b)
+
+
+@pytest.mark.parametrize("prec,stream,staged", [("fp64", -1, -5), ("fp32", -2, -7)])
+def test_staged_long_row_equals_tile_streamed(engine, prec, stream, staged):
+    """A row longer than a tile is summed by the whole workgroup (a strided pass and a tree):
+    in 256-thread staged workgroups the same order as the tile-streamed kernels' 256-thread
+    long-row sum, so one check block of a batch holding such a row gives the same bits."""
+    lps = [_long_row_lp(5), vp.gen_random_lp(4400, 800, 2000)]
+    engine.upload_batch(lps)
+    out = []
+    for lanes in (stream, staged):
+        print(f"[case] long row {prec} lanes={lanes}", flush=True)
+        res = engine.solve_batch(vp.default_params(vp.MODE_RESTARTED, precision=prec, tolerance=0.0,
+                                                   max_iterations=64, check_frequency=64, lanes_A=lanes,
+                                                   lanes_At=lanes, stage_threads=256))
+        assert all(r.iterations == 64 for r in res), res[0].as_dict()
+        out.append(engine.download_batch())
+    (xs0, ys0), (xs1, ys1) = out
+    for x0, y0, x1, y1 in zip(xs0, ys0, xs1, ys1):
+        assert np.array_equal(x0, x1) and np.array_equal(y0, y1)
+
+
+@pytest.mark.parametrize("threads", [1024, 256])
+def test_staged_long_row_batch_against_highs(engine, threads):
+    """The batch with the long row solved with the staged kernels in either workgroup shape
+    (1,024 threads: the row fits a tile and one lane adds it; 256: the workgroup adds it):
+    every instance reaches rel-KKT 1e-8 (oracle) and the long-row LP its HiGHS objective."""
+    lps = [_long_row_lp(6), vp.gen_random_lp(4401, 800, 2000)]
+    opt = lp_cases.highs_optimum(lps[0])
+    assert opt is not None
+    engine.upload_batch(lps)
+    print(f"[case] long-row batch staged threads={threads}", flush=True)
+    res = engine.solve_batch(vp.default_params(vp.MODE_RESTARTED, precision="fp64", tolerance=1e-8,
+                                               max_iterations=200000, lanes_A=-5, lanes_At=-5,
+                                               stage_threads=threads))
+    xs, ys = engine.download_batch()
+    for lp, r, x, y in zip(lps, res, xs, ys):
+        assert r.status == 1, r.as_dict()
+        assert O.rel_kkt(lp, x, y)[0] <= 1e-8
+    got = float(np.dot(lps[0].c, xs[0])) + lps[0].offset
+    assert abs(got - opt) <= 1e-6 * (1 + abs(opt)), (got, opt)
