@@ -170,3 +170,20 @@ def test_refine_every_check_batch_equals_model(engine, lanes):
         assert r.restart_count == restarts
         close(x, mx, (name, "x"))
         close(y, my, (name, "y"))
+
+
+@pytest.mark.parametrize("variant", ["tile_streamed", "split_A4_At2_streamed", "lds_staged_w16_256"])
+def test_refine_every_check_short_blocks_equal_model(engine, variant):
+    """Refinement at every check with blocks of 8 iterations (the centre moves every 8
+    iterations, restarts come often): 24 blocks equal the model run with the same option."""
+    k = 8
+    for name, lp in lps()[:2]:
+        print(f"[case] step parity refine every check K={k} {variant} {name}", flush=True)
+        engine.upload(lp)
+        r = engine.solve(vp.default_params(vp.MODE_RESTARTED, precision="fp64", tolerance=0.0, max_iterations=24 * k,
+                                           check_frequency=k, refine_every_check=1, **VARIANTS[variant]))
+        x, _, y = engine.download()
+        m = M.solve(lp, K=k, tol=0.0, max_iters=24 * k, refine_every_check=True)
+        assert m["iters"] == 24 * k and r.restart_count == m["restarts"], (r.restart_count, m["restarts"])
+        close(x, m["x"], (name, "x"))
+        close(y, m["y"], (name, "y"))
