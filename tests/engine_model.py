@@ -6,7 +6,9 @@ norms leave out rows and columns at that bound, 64-step power iteration), the co
 coordinates around an FP64 centre, the reflected Halpern step, and the restart rule and
 primal-weight controller of the control kernel, checked every K iterations on the original
 LP's rel-KKT, and (refine_every_check) the centre moved at every check with an explicit
-Halpern anchor. It reproduces the device's FP64 iteration counts on the LPs it was compared on;
+Halpern anchor. solve_rounded runs the same algorithm with the low-precision sweeps of the
+FP32, BF16 and FP16 engines (values rounded, iterates in FP32), for studies of the
+refinement. It reproduces the device's FP64 iteration counts on the LPs it was compared on;
 it is not the product and nothing in the product calls it."""
 import numpy as np
 import scipy.sparse as sp
@@ -132,3 +134,92 @@ def solve(lp, S=None, K=64, tol=1e-6, max_iters=200000, rho=1.0, artificial=0.5,
             xh, yh = xf, yf
             dh, lcs, ucs, lrs, urs = recentre(xh, yh)
             x, y, ax, ay = x - xT, y - yT, ax - xT, ay - yT
+
+
+def _bf16(a):
+    b = np.asarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    return ((b + 0x7FFF + ((b >> 16) & 1)) & 0xFFFF0000).astype(np.uint32).view(np.float32)
+
+
+ROUNDING = {"fp32": lambda a: np.asarray(a, np.float32), "bf16": _bf16,
+            "fp16": lambda a: np.asarray(a, np.float32).astype(np.float16).astype(np.float32)}
+
+
+def solve_rounded(lp, S=None, values="fp32", refine_every_check=False, K=64, tol=1e-6, max_iters=200000, rho=1.0,
+                  artificial=0.5, smoothing=0.5, sufficient=0.2, necessary=0.8, step=0.95):
+    """The engine's low-precision sweeps: matrix values rounded to `values` (FP32, BF16 or FP16),
+    iterates, shifted data and sums in FP32, the centre, checks and refinement in FP64; the
+    centre moves at restarts, or at every check with the Halpern anchor kept as a vector
+    (refine_every_check). No drift restart. Returns status, iterations, restarts and rel-KKT."""
+    S = S or setup(lp)
+    f32 = np.float32
+    A, AT, c, lc, uc, lr, ur = S["A"], S["AT"], S["c"], S["lc"], S["uc"], S["lr"], S["ur"]
+    rnd = ROUNDING[values]
+    Al = sp.csr_matrix((rnd(A.data), A.indices, A.indptr), shape=A.shape)
+    ATl = sp.csr_matrix((rnd(AT.data), AT.indices, AT.indptr), shape=AT.shape)
+    m, n = A.shape
+    eta, omega = step / S["norm"], S["omega"]
+    xh, yh = np.clip(0.0, lc, uc), np.zeros(m)
+
+    def recentre(xh, yh):
+        ax = A @ xh
+        return tuple(v.astype(f32) for v in (c - AT @ yh, lc - xh, uc - xh, lr - ax, ur - ax))
+
+    dh, lcs, ucs, lrs, urs = recentre(xh, yh)
+    yhf = yh.astype(f32)
+    x, y = np.zeros(n, f32), np.zeros(m, f32)
+    ax, ay = np.zeros(n, f32), np.zeros(m, f32)
+    k = iters = restarts = 0
+    r0 = r_prev = np.inf
+    restarted_prev = True
+    while True:
+        tau, sig = f32(eta / omega), f32(eta * omega)
+        for it in range(K):
+            lam, pull = f32((k + it + 1) / (k + it + 2)), f32(1.0 / (k + it + 2))
+            xt = np.clip(x - tau * (dh - ATl @ y), lcs, ucs).astype(f32)
+            if it == 0:
+                xq0 = float(np.sum((xt.astype(float) - x) ** 2))
+            if it == K - 1:
+                xq1, xT = float(np.sum((xt.astype(float) - x) ** 2)), xt
+                xq2 = float(np.sum((xt.astype(float) - ax) ** 2))
+            xbar = (f32(2) * xt - x).astype(f32)
+            x = (lam * ((1 + rho) * xt - rho * x) + pull * ax).astype(f32)
+            w = (Al @ xbar).astype(f32)
+            full = yhf + y
+            yt = np.where(sig * (w - lrs) <= full, y + sig * (lrs - w),
+                          np.where(sig * (w - urs) > full, y + sig * (urs - w), -yhf)).astype(f32)
+            if it == 0:
+                yq0 = float(np.sum((yt.astype(float) - y) ** 2))
+            if it == K - 1:
+                yq1, yT = float(np.sum((yt.astype(float) - y) ** 2)), yt
+                yq2 = float(np.sum((yt.astype(float) - ay) ** 2))
+            y = (lam * ((1 + rho) * yt - rho * y) + pull * ay).astype(f32)
+        iters += K
+        k += K
+        xf, yf = xh + xT.astype(float), yh + yT.astype(float)
+        kkt = O.rel_kkt(lp, xf * S["C"] / S["bs"], yf * S["R"] / S["cs"])
+        if kkt[0] <= tol or iters >= max_iters:
+            return dict(status=1 if kkt[0] <= tol else 6, iters=iters, restarts=restarts, kkt=kkt[0])
+        r = np.sqrt(omega * xq1 + yq1 / omega)
+        rf = np.sqrt(omega * xq0 + yq0 / omega)
+        if restarted_prev:
+            r0 = r_prev = rf
+        restart = r <= sufficient * r0 or (r <= necessary * r0 and r > r_prev) or k >= artificial * iters
+        r_prev, restarted_prev = r, False
+        if restart:
+            restarted_prev, restarts, k = True, restarts + 1, 0
+            dx, dy = np.sqrt(xq2), np.sqrt(yq2)
+            if dx > 1e-10 and dy > 1e-10:
+                omega = np.exp(smoothing * np.log(dy / dx) + (1 - smoothing) * np.log(omega))
+            xh, yh = xf, yf
+            dh, lcs, ucs, lrs, urs = recentre(xh, yh)
+            yhf = yh.astype(f32)
+            x, y = np.zeros(n, f32), np.zeros(m, f32)
+            ax, ay = np.zeros(n, f32), np.zeros(m, f32)
+        elif refine_every_check:
+            xh, yh = xf, yf
+            dh, lcs, ucs, lrs, urs = recentre(xh, yh)
+            yhf = yh.astype(f32)
+            x, y = (x - xT).astype(f32), (y - yT).astype(f32)
+            ax, ay = (ax - xT).astype(f32), (ay - yT).astype(f32)
+

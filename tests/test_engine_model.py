@@ -70,3 +70,32 @@ def test_refine_every_check_reaches_the_optimum():
     assert r["status"] == 1, r["kkt"][:4]
     got = float(np.dot(lp.c, r["x"])) + lp.offset
     assert abs(got - opt) <= 1e-5 * (1 + abs(opt)), (got, opt)
+
+
+def test_bf16_rounding_is_nearest_even():
+    """The model's BF16 rounding against a brute-force choice between the two neighbouring
+    BF16 values (nearest, ties to the even significand)."""
+    rng = np.random.default_rng(7)
+    v = np.concatenate([rng.standard_normal(5000).astype(np.float32) * np.float32(10.0) ** rng.integers(-8, 8, 5000),
+                        np.array([1.0, -1.0, 1 + 2 ** -8, 1 + 3 * 2 ** -8, 0.0, 65504.0], np.float32)]).astype(np.float32)
+    got = M._bf16(v)
+    bits = v.view(np.uint32)
+    lo = (bits & np.uint32(0xFFFF0000)).view(np.float32)
+    hi = ((bits & np.uint32(0xFFFF0000)) + np.uint32(0x10000)).view(np.float32)
+    dl, dh = np.abs(v.astype(float) - lo.astype(float)), np.abs(hi.astype(float) - v.astype(float))
+    even_lo = ((lo.view(np.uint32) >> 16) & 1) == 0
+    want = np.where(dl < dh, lo, np.where(dh < dl, hi, np.where(even_lo, lo, hi)))
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.parametrize("values,refine", [("fp32", False), ("bf16", True), ("fp16", True)])
+def test_rounded_sweeps_reach_1e6(values, refine):
+    """The low-precision model (values rounded, iterates in FP32, FP64 centre and checks)
+    reaches rel-KKT 1e-6 on the 2,000 x 5,000 LP (with at most 3x the FP64 model's iterations,
+    a sanity bound against a near-stall: BF16 takes about 1.7x)."""
+    lp = lp_cases.cfg0_lp()
+    S = M.setup(lp)
+    ref = M.solve(lp, S=S, max_iters=100000)
+    r = M.solve_rounded(lp, S=S, values=values, refine_every_check=refine, max_iters=100000)
+    assert r["status"] == 1 and r["kkt"] <= 1e-6, r
+    assert r["iters"] <= 3 * ref["iters"], (r["iters"], ref["iters"])
